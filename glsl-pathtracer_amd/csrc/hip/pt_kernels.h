@@ -1,0 +1,59 @@
+// Launch interface between the renderer (host) and the kernels.
+#pragma once
+
+#include "pt_device.h"
+
+namespace ptd {
+
+// Wavefront path state, structure of arrays indexed by path slot
+// (slot = sample_in_pass * pixels + pixel item).
+struct PathBuf {
+    float4* ro;    // origin, previous BSDF/phase pdf
+    float4* rd;    // direction, previous material roughness
+    float4* thr;   // throughput
+    float4* rad;   // radiance, alpha
+    uint4* rng;    // pcg4d state
+    int4* meta;    // depth, pass-through count | in-medium << 16 | surface-scatter << 17, previous material, medium type
+    float4* aux;   // previous texCoord (u, v), medium anisotropy
+    float4* med;   // medium colour, density (only with OPT_MEDIUM, else null)
+    float4* hit;   // closest hit: t, u, v, triangle row (int bits)
+    int2* hit2;    // instance, light
+};
+
+// Shadow (NEE visibility) queue.
+struct ShadowBuf {
+    float4* o;  // origin, max distance
+    float4* d;  // direction, path slot (int bits)
+    float4* c;  // contribution (already times throughput)
+};
+
+}  // namespace ptd
+
+namespace ptk {
+enum { KIND_PLAIN = 0, KIND_EXTEND = 1, KIND_SHADE = 2, KIND_SHADOW = 3 };
+
+int persistent_blocks(int kind, int bs, int S, bool count, int num_cu);
+void launch_plain(hipStream_t s, int blocks, int bs, const ptd::DevScene& sc, const ptd::DevOpts& op, float4* accum,
+                  int sample, int num_items, int* work, int S, int* spill, unsigned long long* counters, int* ovf,
+                  bool count);
+void launch_raygen(hipStream_t s, const ptd::DevOpts& op, const ptd::PathBuf& pb, int npix, int first_sample,
+                   int num_paths, int* queue, int* counts);
+void launch_extend(hipStream_t s, int blocks, int bs, const ptd::DevScene& sc, const ptd::DevOpts& op,
+                   const ptd::PathBuf& pb, const int* queue, const int* count_in, int* work, int S, int* spill,
+                   unsigned long long* counters, int* ovf, bool count);
+void launch_shade(hipStream_t s, int blocks, int bs, const ptd::DevScene& sc, const ptd::DevOpts& op,
+                  const ptd::PathBuf& pb, const ptd::ShadowBuf& sb, const int* qin, const int* count_in, int* qout,
+                  int* count_out, int* shadow_count, int* work);
+void launch_shadow(hipStream_t s, int blocks, int bs, const ptd::DevScene& sc, const ptd::DevOpts& op,
+                   const ptd::PathBuf& pb, const ptd::ShadowBuf& sb, const int* count_in, int* work, int S,
+                   int* spill, unsigned long long* counters, int* ovf, bool count);
+void launch_accum(hipStream_t s, const ptd::DevOpts& op, const ptd::PathBuf& pb, float4* accum, int npix, int nsamp);
+void launch_fill(hipStream_t s, int* p, int n, int v);
+void launch_query_closest(hipStream_t s, const ptd::DevScene& sc, const ptd::DevOpts& op, const float* o,
+                          const float* d, int n, float* t, int* inst, int* tri, int* light, float* uv, int S,
+                          int* spill, int* ovf);
+void launch_query_any(hipStream_t s, const ptd::DevScene& sc, const ptd::DevOpts& op, const float* o, const float* d,
+                      const float* maxd, int n, int* occ, int S, int* spill, int* ovf);
+void launch_tonemap(hipStream_t s, const float4* accum, int w, int h, float inv_spp, int tm, int aces, int simple,
+                    int bg_mode, const float* bg, uchar4* out);
+}  // namespace ptk

@@ -1,0 +1,845 @@
+// Kernels of the path: plain one-launch-per-frame megakernel, wavefront
+// stages, ray queries, accumulation and tonemapping. gfx950, wave64.
+//
+// Reference behaviour (file:line in knightcrawler25/GLSL-PathTracer):
+//   ray generation      src/shaders/tile.glsl:41-75
+//   bounce loop         src/shaders/common/pathtrace.glsl:285-476
+//   DirectLight (NEE)   src/shaders/common/pathtrace.glsl:158-283
+//   tonemap             src/shaders/tonemap.glsl:59-133
+#include "pt_kernels.h"
+
+namespace ptd {
+
+// ------------------------------------------------------------------ helpers
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+// One atomic per wave fetches the next 64 work items.
+__device__ __forceinline__ int fetch_batch(int* counter) {
+    int base = 0;
+    if (lane_id() == 0) base = atomicAdd(counter, 64);
+    return __builtin_amdgcn_readfirstlane(base);
+}
+
+// Wave-level stream compaction: one atomic per wave, lanes write at their
+// prefix position. Must be called by every lane of the wave.
+__device__ __forceinline__ int wave_append(bool keep, int* counter) {
+    const uint64_t mask = __ballot(keep);
+    const int n = __popcll(mask);
+    const int prefix = __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(mask >> 32),
+                                                 __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0u));
+    int base = 0;
+    if (lane_id() == 0 && n > 0) base = atomicAdd(counter, n);
+    base = __builtin_amdgcn_readfirstlane(base);
+    return keep ? base + prefix : -1;
+}
+
+template <bool COUNT>
+__device__ __forceinline__ void flush_counts(const TravCount& c, unsigned long long* out) {
+    if (!COUNT) return;
+    atomicAdd(out + 0, static_cast<unsigned long long>(c.nodes));
+    atomicAdd(out + 1, static_cast<unsigned long long>(c.tris));
+    atomicAdd(out + 2, static_cast<unsigned long long>(c.insts));
+    atomicAdd(out + 3, static_cast<unsigned long long>(c.lights));
+    atomicAdd(out + 4, static_cast<unsigned long long>(c.spills));
+}
+
+// Pixel of work item `i`: 8x8 pixel tiles, one tile per wave, rows bottom-up.
+__device__ __forceinline__ void item_pixel(int i, int width, int& px, int& py) {
+    const int tiles_x = (width + 7) >> 3;
+    const int tile = i >> 6, l = i & 63;
+    px = (tile % tiles_x) * 8 + (l & 7);
+    py = (tile / tiles_x) * 8 + (l >> 3);
+}
+
+// Camera ray of tile.glsl:41-69 for pixel (px, py).
+__device__ void camera_ray(const DevOpts& op, int px, int py, Rng& rng, vec3& ro, vec3& rd) {
+    const float resx = static_cast<float>(op.width), resy = static_cast<float>(op.height);
+    float cx = (static_cast<float>(px) + 0.5f) / resx;
+    float cy = (static_cast<float>(py) + 0.5f) / resy;
+    float r1 = 2.0f * rnd(rng);
+    float r2 = 2.0f * rnd(rng);
+    float jx = r1 < 1.0f ? sqrtf(r1) - 1.0f : 1.0f - sqrtf(2.0f - r1);
+    float jy = r2 < 1.0f ? sqrtf(r2) - 1.0f : 1.0f - sqrtf(2.0f - r2);
+    jx = jx / (resx * 0.5f);
+    jy = jy / (resy * 0.5f);
+    float dx = (cx * 2.0f - 1.0f) + jx;
+    float dy = (cy * 2.0f - 1.0f) + jy;
+    float scale = tanf(op.cam_fov * 0.5f);
+    dy = dy * (resy / resx * scale);
+    dx = dx * scale;
+    vec3 right = V(op.cam_right[0], op.cam_right[1], op.cam_right[2]);
+    vec3 up = V(op.cam_up[0], op.cam_up[1], op.cam_up[2]);
+    vec3 fwd = V(op.cam_fwd[0], op.cam_fwd[1], op.cam_fwd[2]);
+    vec3 rayDir = normalize(dx * right + dy * up + fwd);
+    vec3 focalPoint = op.cam_focal * rayDir;
+    float cam_r1 = rnd(rng) * TWO_PI;
+    float cam_r2 = rnd(rng) * op.cam_aperture;
+    vec3 ap = (cosf(cam_r1) * right + sinf(cam_r1) * up) * sqrtf(cam_r2);
+    rd = normalize(focalPoint - ap);
+    ro = V(op.cam_pos[0], op.cam_pos[1], op.cam_pos[2]) + ap;
+}
+
+template <bool COUNT>
+__device__ __forceinline__ bool occluded(const DevScene& sc, const DevOpts& op, vec3 ro, vec3 rd, float maxDist,
+                                         Stack& stk, TravCount& cnt, bool& ovf) {
+    if ((op.flags & OPT_LIGHTS) && any_light(sc, ro, rd, maxDist)) return true;
+    Hit h;
+    return traverse<true, COUNT>(sc, ro, rd, maxDist, stk, h, cnt, ovf);
+}
+
+template <bool COUNT>
+__device__ __forceinline__ Hit closest(const DevScene& sc, const DevOpts& op, vec3 ro, vec3 rd, int depth, Stack& stk,
+                                       TravCount& cnt, bool& ovf) {
+    Hit h;
+    h.t = INF; h.u = 0.f; h.v = 0.f; h.tri = -1; h.inst = -1; h.light = -1;
+    if ((op.flags & OPT_LIGHTS) && (!(op.flags & OPT_HIDE_EMITTERS) || depth > 0)) hit_lights<COUNT>(sc, ro, rd, h, cnt);
+    traverse<false, COUNT>(sc, ro, rd, h.t, stk, h, cnt, ovf);
+    return h;
+}
+
+// EvalTransmittance (pathtrace.glsl:118-156): product of medium
+// transmittances along a shadow ray, zero when an opaque surface blocks it.
+template <bool COUNT>
+__device__ vec3 eval_transmittance(const DevScene& sc, const DevOpts& op, vec3 ro, vec3 rd, Rng& rng, Stack& stk,
+                                   TravCount& cnt, bool& ovf) {
+    vec3 tr = V1(1.0f);
+    SurfState st;
+    st.matID = 0; st.tu = 0.f; st.tv = 0.f; st.normal = V1(0.f); st.ffnormal = V1(0.f);
+    st.tangent = V1(0.f); st.bitangent = V1(0.f); st.fhp = V1(0.f);
+    float prev_rough = 0.f;
+    for (int depth = 0; depth < op.max_depth; depth++) {
+        Hit h = closest<COUNT>(sc, op, ro, rd, 0, stk, cnt, ovf);
+        bool hit = h.t != INF;
+        if (!hit || h.tri < 0) break;
+        surface_attributes(sc, h, ro, rd, st);
+        float eta;
+        Material m = GetMaterial(sc, op, st, rd, 0, prev_rough, eta);
+        prev_rough = m.roughness;
+        bool alphatest = (m.alphaMode == ALPHA_MODE_MASK && m.opacity < m.alphaCutoff) ||
+                         (m.alphaMode == ALPHA_MODE_BLEND && rnd(rng) > m.opacity);
+        bool refractive = (1.0f - m.metallic) * m.specTrans > 0.0f;
+        if (hit && !(alphatest || refractive)) return V1(0.0f);
+        if (dot(rd, st.normal) > 0.0f && m.medium.type != MEDIUM_NONE) {
+            vec3 color = m.medium.type == MEDIUM_ABSORB ? V1(1.0f) - m.medium.color : V1(1.0f);
+            tr = tr * vexp(-color * m.medium.density * h.t);
+        }
+        ro = st.fhp + rd * EPS;
+    }
+    return tr;
+}
+
+// Per-path state carried between bounces (the locals of PathTrace).
+struct PathRegs {
+    vec3 ro, rd, thr, rad;
+    float alpha, prev_pdf, prev_rough;
+    int depth, pass, prev_mat;
+    bool in_medium, surface_scatter;
+    float tu, tv;
+    Medium med;
+    Rng rng;
+};
+
+// Result of DirectLight's sampling step; the visibility test is separate so
+// the wavefront form can defer it to the shadow stage.
+struct NeeRay { vec3 o, d, contrib; float maxd; bool valid; };
+
+// The sampling and BSDF part of DirectLight for one light kind. kind 0 =
+// environment (SampleEnvMap), 1 = analytic light. RNG draws happen in the
+// reference order; contrib excludes the final `* throughput`.
+__device__ NeeRay nee_setup(const DevScene& sc, const DevOpts& op, int kind, vec3 rd_in, const SurfState& st,
+                            const Material& m, float eta, Rng& rng) {
+    NeeRay r;
+    r.valid = false;
+    r.contrib = V1(0.0f);
+    vec3 scatterPos = st.fhp + st.normal * EPS;
+    r.o = scatterPos;
+    if (kind == 0) {
+        vec3 Li;
+        float4 dp = SampleEnvMap(sc, op, rng, Li);
+        vec3 lightDir = xyz(dp);
+        float lightPdf = dp.w;
+        r.d = lightDir;
+        r.maxd = INF - EPS;
+        float pdf;
+        vec3 f = DisneyEval(m, eta, -rd_in, st.ffnormal, lightDir, pdf);
+        if (pdf > 0.0f) {
+            float mis = PowerHeuristic(lightPdf, pdf);
+            if (mis > 0.0f) {
+                r.contrib = mis * Li * f * op.env_intensity / lightPdf;
+                r.valid = true;
+            }
+        }
+        return r;
+    }
+    int index = static_cast<int>(rnd(rng) * static_cast<float>(sc.num_lights));
+    if (index > sc.num_lights - 1) index = sc.num_lights - 1;
+    const GLight L = sc.lights[index];
+    LightSample ls;
+    SampleOneLight(L, scatterPos, sc.num_lights, rng, ls);
+    vec3 Li = ls.emission;
+    r.d = ls.direction;
+    r.maxd = ls.dist - EPS;
+    if (dot(ls.direction, ls.normal) < 0.0f) {
+        float pdf;
+        vec3 f = DisneyEval(m, eta, -rd_in, st.ffnormal, ls.direction, pdf);
+        float mis = 1.0f;
+        if (L.e.w > 0.0f) mis = PowerHeuristic(ls.pdf, pdf);
+        if (pdf > 0.0f) {
+            r.contrib = mis * Li * f / ls.pdf;
+            r.valid = true;
+        }
+    }
+    return r;
+}
+
+// DirectLight with the volume-MIS variant (transmittance instead of a binary
+// test); used by the plain kernel only.
+template <bool COUNT>
+__device__ vec3 direct_light_volmis(const DevScene& sc, const DevOpts& op, vec3 rd_in, const SurfState& st,
+                                    const Material& m, float eta, const Medium& med, bool isSurface, Rng& rng,
+                                    Stack& stk, TravCount& cnt, bool& ovf) {
+    vec3 Ld = V1(0.0f);
+    vec3 scatterPos = st.fhp + st.normal * EPS;
+    if ((op.flags & OPT_ENVMAP) && !(op.flags & OPT_UNIFORM_LIGHT)) {
+        vec3 Li;
+        float4 dp = SampleEnvMap(sc, op, rng, Li);
+        vec3 lightDir = xyz(dp);
+        float lightPdf = dp.w;
+        Li = Li * eval_transmittance<COUNT>(sc, op, scatterPos, lightDir, rng, stk, cnt, ovf);
+        vec3 f;
+        float pdf;
+        if (isSurface) f = DisneyEval(m, eta, -rd_in, st.ffnormal, lightDir, pdf);
+        else { float p = PhaseHG(dot(-rd_in, lightDir), med.anisotropy); f = V1(p); pdf = p; }
+        if (pdf > 0.0f) {
+            float mis = PowerHeuristic(lightPdf, pdf);
+            if (mis > 0.0f) Ld = Ld + mis * Li * f * op.env_intensity / lightPdf;
+        }
+    }
+    if (op.flags & OPT_LIGHTS) {
+        int index = static_cast<int>(rnd(rng) * static_cast<float>(sc.num_lights));
+        if (index > sc.num_lights - 1) index = sc.num_lights - 1;
+        const GLight L = sc.lights[index];
+        LightSample ls;
+        SampleOneLight(L, scatterPos, sc.num_lights, rng, ls);
+        vec3 Li = ls.emission;
+        if (dot(ls.direction, ls.normal) < 0.0f) {
+            Li = Li * eval_transmittance<COUNT>(sc, op, scatterPos, ls.direction, rng, stk, cnt, ovf);
+            vec3 f;
+            float pdf;
+            if (isSurface) f = DisneyEval(m, eta, -rd_in, st.ffnormal, ls.direction, pdf);
+            else { float p = PhaseHG(dot(-rd_in, ls.direction), med.anisotropy); f = V1(p); pdf = p; }
+            float mis = 1.0f;
+            if (L.e.w > 0.0f) mis = PowerHeuristic(ls.pdf, pdf);
+            if (pdf > 0.0f) Ld = Ld + mis * f * Li / ls.pdf;
+        }
+    }
+    return Ld;
+}
+
+// DirectLight with binary visibility (pathtrace.glsl:158-283, non-VOL_MIS).
+template <bool COUNT>
+__device__ vec3 direct_light(const DevScene& sc, const DevOpts& op, vec3 rd_in, const SurfState& st,
+                             const Material& m, float eta, Rng& rng, Stack& stk, TravCount& cnt, bool& ovf) {
+    vec3 Ld = V1(0.0f);
+    if ((op.flags & OPT_ENVMAP) && !(op.flags & OPT_UNIFORM_LIGHT)) {
+        NeeRay r = nee_setup(sc, op, 0, rd_in, st, m, eta, rng);
+        if (r.valid && !occluded<COUNT>(sc, op, r.o, r.d, r.maxd, stk, cnt, ovf)) Ld = Ld + r.contrib;
+    }
+    if (op.flags & OPT_LIGHTS) {
+        NeeRay r = nee_setup(sc, op, 1, rd_in, st, m, eta, rng);
+        if (r.valid && !occluded<COUNT>(sc, op, r.o, r.d, r.maxd, stk, cnt, ovf)) Ld = Ld + r.contrib;
+    }
+    return Ld;
+}
+
+// Event of one bounce after the closest hit.
+enum BounceEvent { EV_END = 0, EV_SURFACE = 1, EV_PASS = 2, EV_MEDIUM = 3 };
+
+// Miss branch of PathTrace (pathtrace.glsl:305-339).
+__device__ void on_miss(const DevScene& sc, const DevOpts& op, PathRegs& p) {
+    if ((op.flags & (OPT_BACKGROUND | OPT_TRANSPARENT_BACKGROUND)) && p.depth == 0) p.alpha = 0.0f;
+    if (!(op.flags & OPT_HIDE_EMITTERS) || p.depth > 0) {
+        if (op.flags & OPT_UNIFORM_LIGHT) {
+            p.rad = p.rad + V(op.ucol_x, op.ucol_y, op.ucol_z) * p.thr;
+        } else if (op.flags & OPT_ENVMAP) {
+            float4 e = EvalEnvMap(sc, op, p.rd);
+            float mis = 1.0f;
+            if (p.depth > 0) mis = PowerHeuristic(p.prev_pdf, e.w);
+            if ((op.flags & OPT_MEDIUM) && !(op.flags & OPT_VOL_MIS) && !p.surface_scatter) mis = 1.0f;
+            if (mis > 0.0f) p.rad = p.rad + mis * xyz(e) * p.thr * op.env_intensity;
+        }
+    }
+}
+
+// Bounce body from the hit up to next-event estimation (pathtrace.glsl:341-425):
+// material, emission, emitter MIS, depth limit, medium events, alpha test.
+// `st` keeps the previous bounce's attributes when a light is hit, as the
+// shader's State does. Returns the event that decides what follows.
+__device__ int bounce_pre(const DevScene& sc, const DevOpts& op, const Hit& h, PathRegs& p, SurfState& st,
+                          Material& m, float& eta) {
+    const bool is_emitter = h.tri < 0;
+    float light_pdf = 0.f;
+    vec3 light_em = V1(0.f);
+    if (is_emitter) {
+        light_pdf = light_hit_pdf(sc, h.light, p.ro, p.rd, h.t);
+        light_em = xyz(sc.lights[h.light].e);
+        st.fhp = p.ro + p.rd * h.t;
+        st.matID = p.prev_mat;
+        st.tu = p.tu; st.tv = p.tv;
+    } else {
+        surface_attributes(sc, h, p.ro, p.rd, st);
+    }
+    m = GetMaterial(sc, op, st, p.rd, p.depth, p.prev_rough, eta);
+    p.prev_rough = m.roughness;
+    p.prev_mat = st.matID;
+    p.tu = st.tu; p.tv = st.tv;
+    p.rad = p.rad + m.emission * p.thr;
+    if ((op.flags & OPT_LIGHTS) && is_emitter) {
+        float mis = 1.0f;
+        if (p.depth > 0) mis = PowerHeuristic(p.prev_pdf, light_pdf);
+        if ((op.flags & OPT_MEDIUM) && !(op.flags & OPT_VOL_MIS) && !p.surface_scatter) mis = 1.0f;
+        p.rad = p.rad + mis * light_em * p.thr;
+        return EV_END;
+    }
+    if (p.depth == op.max_depth) return EV_END;
+    if (op.flags & OPT_MEDIUM) {
+        p.surface_scatter = false;
+        if (p.in_medium) {
+            if (p.med.type == MEDIUM_ABSORB) {
+                p.thr = p.thr * vexp(-(V1(1.0f) - p.med.color) * h.t * p.med.density);
+            } else if (p.med.type == MEDIUM_EMISSIVE) {
+                p.rad = p.rad + p.med.color * h.t * p.med.density * p.thr;
+            } else {
+                float scatterDist = gmin(-logf(rnd(p.rng)) / p.med.density, h.t);
+                if (scatterDist < h.t) {
+                    p.thr = p.thr * p.med.color;
+                    p.ro = p.ro + p.rd * scatterDist;
+                    st.fhp = p.ro;
+                    return EV_MEDIUM;
+                }
+            }
+        }
+    }
+    if (op.flags & OPT_ALPHA_TEST) {
+        bool passed = (m.alphaMode == ALPHA_MODE_MASK && m.opacity < m.alphaCutoff) ||
+                      (m.alphaMode == ALPHA_MODE_BLEND && rnd(p.rng) > m.opacity);
+        if (passed) {
+            p.depth--;
+            p.pass++;
+            return EV_PASS;
+        }
+    }
+    p.surface_scatter = true;
+    return EV_SURFACE;
+}
+
+// Bounce body after NEE: BSDF or phase sampling, medium bookkeeping, Russian
+// roulette (pathtrace.glsl:392-471). Returns false when the path ends.
+__device__ bool bounce_post(const DevOpts& op, PathRegs& p, const SurfState& st, const Material& m, float eta, int ev) {
+    if (ev == EV_MEDIUM) {
+        float ra = rnd(p.rng);
+        float rb = rnd(p.rng);
+        vec3 scatterDir = SampleHG(-p.rd, p.med.anisotropy, ra, rb);
+        p.prev_pdf = PhaseHG(dot(-p.rd, scatterDir), p.med.anisotropy);
+        p.rd = scatterDir;
+    } else {
+        vec3 L;
+        if (ev == EV_PASS) {
+            if (p.pass > 1024) return false;
+            L = p.rd;
+        } else {
+            float pdf;
+            vec3 f = DisneySample(m, eta, -p.rd, st.ffnormal, p.rng, L, pdf);
+            p.prev_pdf = pdf;
+            if (pdf > 0.0f) p.thr = p.thr * (f / pdf);
+            else return false;
+        }
+        p.rd = L;
+        p.ro = st.fhp + p.rd * EPS;
+        if (op.flags & OPT_MEDIUM) {
+            if (dot(p.rd, st.normal) < 0.0f && m.medium.type != MEDIUM_NONE) {
+                p.in_medium = true;
+                p.med = m.medium;
+            } else if (m.medium.type != MEDIUM_NONE) {
+                p.in_medium = false;
+            }
+        }
+    }
+    if ((op.flags & OPT_RR) && p.depth >= op.rr_depth) {
+        float q = gmin(gmax(p.thr.x, gmax(p.thr.y, p.thr.z)) + 0.001f, 0.95f);
+        if (rnd(p.rng) > q) return false;
+        p.thr = p.thr / q;
+    }
+    return true;
+}
+
+__device__ __forceinline__ void init_path(PathRegs& p) {
+    p.thr = V1(1.0f); p.rad = V1(0.0f);
+    p.alpha = 1.0f; p.prev_pdf = 0.0f; p.prev_rough = 0.0f;
+    p.depth = 0; p.pass = 0; p.prev_mat = 0;
+    p.in_medium = false; p.surface_scatter = false;
+    p.tu = 0.f; p.tv = 0.f;
+    p.med.type = 0; p.med.density = 0.f; p.med.color = V1(0.f); p.med.anisotropy = 0.f;
+}
+
+__device__ __forceinline__ void init_surf(SurfState& st) {
+    st.fhp = V1(0.f); st.normal = V1(0.f); st.ffnormal = V1(0.f); st.tangent = V1(0.f); st.bitangent = V1(0.f);
+    st.tu = 0.f; st.tv = 0.f; st.matID = 0;
+}
+
+// NEE rays of one SURFACE or MEDIUM event in the reference's draw order
+// (environment first, then one analytic light).
+__device__ __forceinline__ void nee_rays(const DevScene& sc, const DevOpts& op, const PathRegs& p0, Rng& rng,
+                                         const SurfState& st, const Material& m, float eta, NeeRay nee[2]) {
+    nee[0].valid = nee[1].valid = false;
+    if ((op.flags & OPT_ENVMAP) && !(op.flags & OPT_UNIFORM_LIGHT)) nee[0] = nee_setup(sc, op, 0, p0.rd, st, m, eta, rng);
+    if (op.flags & OPT_LIGHTS) nee[1] = nee_setup(sc, op, 1, p0.rd, st, m, eta, rng);
+}
+
+// Full PathTrace for one sample (megakernel form).
+template <bool COUNT>
+__device__ float4 path_trace(const DevScene& sc, const DevOpts& op, PathRegs& p, Stack& stk, TravCount& cnt,
+                             bool& ovf) {
+    SurfState st;
+    init_surf(st);
+    for (;; p.depth++) {
+        Hit h = closest<COUNT>(sc, op, p.ro, p.rd, p.depth, stk, cnt, ovf);
+        if (h.t == INF) { on_miss(sc, op, p); break; }
+        Material m;
+        float eta;
+        const int ev = bounce_pre(sc, op, h, p, st, m, eta);
+        if (ev == EV_END) break;
+        if (ev != EV_PASS) {
+            vec3 Ld = V1(0.0f);
+            if ((op.flags & OPT_MEDIUM) && (op.flags & OPT_VOL_MIS)) {
+                Ld = direct_light_volmis<COUNT>(sc, op, p.rd, st, m, eta, p.med, ev == EV_SURFACE, p.rng, stk, cnt, ovf);
+            } else {
+                NeeRay nee[2];
+                nee_rays(sc, op, p, p.rng, st, m, eta, nee);
+                for (int k = 0; k < 2; ++k)
+                    if (nee[k].valid && !occluded<COUNT>(sc, op, nee[k].o, nee[k].d, nee[k].maxd, stk, cnt, ovf))
+                        Ld = Ld + nee[k].contrib;
+            }
+            p.rad = p.rad + Ld * p.thr;
+        }
+        if (!bounce_post(op, p, st, m, eta, ev)) break;
+    }
+    return make_float4(p.rad.x, p.rad.y, p.rad.z, p.alpha);
+}
+
+// ------------------------------------------------------------------ plain kernel
+template <bool COUNT>
+__global__ void __launch_bounds__(256) k_plain(DevScene sc, DevOpts op, float4* __restrict__ accum, int sample,
+                                               int num_items, int* work, int S, int* spill,
+                                               unsigned long long* counters, int* ovf_flag) {
+    extern __shared__ int lds[];
+    Stack stk{lds, spill, static_cast<int>(threadIdx.x), static_cast<int>(blockDim.x), S,
+              blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x,
+              static_cast<size_t>(gridDim.x) * blockDim.x};
+    TravCount cnt{0, 0, 0, 0, 0};
+    bool ovf = false;
+    for (;;) {
+        const int base = fetch_batch(work);
+        if (base >= num_items) break;
+        const int i = base + lane_id();
+        int px, py;
+        item_pixel(i, op.width, px, py);
+        if (i < num_items && px < op.width && py < op.height) {
+            PathRegs p;
+            init_path(p);
+            p.rng = init_rng(static_cast<uint32_t>(px), static_cast<uint32_t>(py), static_cast<uint32_t>(sample + 2));
+            camera_ray(op, px, py, p.rng, p.ro, p.rd);
+            float4 c = path_trace<COUNT>(sc, op, p, stk, cnt, ovf);
+            float4* a = accum + static_cast<size_t>(py) * op.width + px;
+            float4 acc = *a;
+            acc.x += c.x; acc.y += c.y; acc.z += c.z; acc.w += c.w;
+            *a = acc;
+        }
+    }
+    flush_counts<COUNT>(cnt, counters);
+    if (ovf) atomicOr(ovf_flag, 1);
+}
+
+// ------------------------------------------------------------------ wavefront
+// Raygen: slot = s_local * npix + pixel; writes the path state and queue.
+__global__ void __launch_bounds__(256) k_raygen(DevOpts op, PathBuf pb, int npix, int first_sample, int num_paths,
+                                                int* queue, int* counts) {
+    const int slot = blockIdx.x * blockDim.x + threadIdx.x;
+    if (slot == 0) counts[0] = num_paths;
+    if (slot >= num_paths) return;
+    const int s = slot / npix;
+    const int item = slot - s * npix;
+    int px, py;
+    item_pixel(item, op.width, px, py);
+    Rng rng = init_rng(static_cast<uint32_t>(px), static_cast<uint32_t>(py), static_cast<uint32_t>(first_sample + s + 2));
+    vec3 ro, rd;
+    if (px < op.width && py < op.height) camera_ray(op, px, py, rng, ro, rd);
+    else { ro = V1(0.f); rd = V(0.f, 0.f, 1.f); }
+    pb.ro[slot] = F4(ro, 0.0f);
+    pb.rd[slot] = F4(rd, 0.0f);
+    pb.thr[slot] = make_float4(1.f, 1.f, 1.f, 0.f);
+    pb.rad[slot] = make_float4(0.f, 0.f, 0.f, 1.f);
+    pb.rng[slot] = make_uint4(rng.x, rng.y, rng.z, rng.w);
+    pb.meta[slot] = make_int4(0, 0, 0, 0);
+    pb.aux[slot] = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (pb.med) pb.med[slot] = make_float4(0.f, 0.f, 0.f, 0.f);
+    // pixels outside the image (partial tiles) never enter the queue
+    queue[slot] = (px < op.width && py < op.height) ? slot : -1;
+}
+
+// Extend: closest hit for every queued path.
+template <bool COUNT>
+__global__ void __launch_bounds__(256) k_extend(DevScene sc, DevOpts op, PathBuf pb, const int* __restrict__ queue,
+                                                const int* count_in, int* work, int S, int* spill,
+                                                unsigned long long* counters, int* ovf_flag) {
+    extern __shared__ int lds[];
+    Stack stk{lds, spill, static_cast<int>(threadIdx.x), static_cast<int>(blockDim.x), S,
+              blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x,
+              static_cast<size_t>(gridDim.x) * blockDim.x};
+    TravCount cnt{0, 0, 0, 0, 0};
+    bool ovf = false;
+    const int count = *count_in;
+    for (;;) {
+        const int base = fetch_batch(work);
+        if (base >= count) break;
+        const int i = base + lane_id();
+        if (i < count) {
+            const int slot = queue[i];
+            if (slot >= 0) {
+                const float4 o4 = pb.ro[slot], d4 = pb.rd[slot];
+                const int depth = pb.meta[slot].x;
+                Hit h = closest<COUNT>(sc, op, xyz(o4), xyz(d4), depth, stk, cnt, ovf);
+                pb.hit[slot] = make_float4(h.t, h.u, h.v, __int_as_float(h.tri));
+                pb.hit2[slot] = make_int2(h.inst, h.light);
+            }
+        }
+    }
+    flush_counts<COUNT>(cnt, counters);
+    if (ovf) atomicOr(ovf_flag, 1);
+}
+
+__device__ __forceinline__ void load_path(const PathBuf& pb, int slot, PathRegs& p) {
+    float4 o4 = pb.ro[slot], d4 = pb.rd[slot], t4 = pb.thr[slot], r4 = pb.rad[slot], a4 = pb.aux[slot];
+    uint4 g = pb.rng[slot];
+    int4 mt = pb.meta[slot];
+    p.ro = xyz(o4); p.prev_pdf = o4.w;
+    p.rd = xyz(d4); p.prev_rough = d4.w;
+    p.thr = xyz(t4);
+    p.rad = xyz(r4); p.alpha = r4.w;
+    p.rng = Rng{g.x, g.y, g.z, g.w};
+    p.depth = mt.x; p.pass = mt.y & 0xffff;
+    p.in_medium = (mt.y >> 16) & 1; p.surface_scatter = (mt.y >> 17) & 1;
+    p.prev_mat = mt.z; p.med.type = mt.w;
+    p.tu = a4.x; p.tv = a4.y; p.med.anisotropy = a4.z;
+    if (pb.med) { float4 m4 = pb.med[slot]; p.med.color = xyz(m4); p.med.density = m4.w; }
+    else { p.med.color = V1(0.f); p.med.density = 0.f; }
+}
+
+__device__ __forceinline__ void store_path(const PathBuf& pb, int slot, const PathRegs& p) {
+    pb.ro[slot] = F4(p.ro, p.prev_pdf);
+    pb.rd[slot] = F4(p.rd, p.prev_rough);
+    pb.thr[slot] = F4(p.thr, 0.f);
+    pb.rad[slot] = F4(p.rad, p.alpha);
+    pb.rng[slot] = make_uint4(p.rng.x, p.rng.y, p.rng.z, p.rng.w);
+    pb.meta[slot] = make_int4(p.depth, (p.pass & 0xffff) | (p.in_medium ? 1 << 16 : 0) | (p.surface_scatter ? 1 << 17 : 0),
+                              p.prev_mat, p.med.type);
+    pb.aux[slot] = make_float4(p.tu, p.tv, p.med.anisotropy, 0.f);
+    if (pb.med) pb.med[slot] = F4(p.med.color, p.med.density);
+}
+
+// Shade: everything of one bounce except the NEE visibility tests, which are
+// appended to the shadow queue; continuing paths are compacted into qout.
+__global__ void __launch_bounds__(256) k_shade(DevScene sc, DevOpts op, PathBuf pb, ShadowBuf sb,
+                                               const int* __restrict__ qin, const int* count_in, int* qout,
+                                               int* count_out, int* shadow_count, int* work) {
+    const int count = *count_in;
+    for (;;) {
+        const int base = fetch_batch(work);
+        if (base >= count) break;
+        const int i = base + lane_id();
+        int slot = -1;
+        bool keep = false;
+        NeeRay nee[2];
+        nee[0].valid = nee[1].valid = false;
+        vec3 thr_nee = V1(0.f);
+        if (i < count) {
+            slot = qin[i];
+            if (slot >= 0) {
+                PathRegs p;
+                load_path(pb, slot, p);
+                const float4 h4 = pb.hit[slot];
+                const int2 h2 = pb.hit2[slot];
+                Hit h;
+                h.t = h4.x; h.u = h4.y; h.v = h4.z; h.tri = __float_as_int(h4.w); h.inst = h2.x; h.light = h2.y;
+                bool alive = false;
+                if (h.t == INF) {
+                    on_miss(sc, op, p);
+                } else {
+                    SurfState st;
+                    init_surf(st);
+                    Material m;
+                    float eta;
+                    const int ev = bounce_pre(sc, op, h, p, st, m, eta);
+                    if (ev != EV_END) {
+                        if (ev != EV_PASS) {
+                            nee_rays(sc, op, p, p.rng, st, m, eta, nee);
+                            thr_nee = p.thr;
+                        }
+                        alive = bounce_post(op, p, st, m, eta, ev);
+                        if (alive) p.depth++;
+                    }
+                }
+                store_path(pb, slot, p);
+                keep = alive;
+            }
+        }
+        // NEE rays: contribution * throughput at NEE time, added if unoccluded.
+        for (int k = 0; k < 2; ++k) {
+            const int pos = wave_append(nee[k].valid, shadow_count);
+            if (pos >= 0) {
+                vec3 c = nee[k].contrib * thr_nee;
+                sb.o[pos] = F4(nee[k].o, nee[k].maxd);
+                sb.d[pos] = F4(nee[k].d, __int_as_float(slot));
+                sb.c[pos] = F4(c, 0.f);
+            }
+        }
+        const int pos = wave_append(keep, count_out);
+        if (pos >= 0) qout[pos] = slot;
+    }
+}
+
+// Shadow: any-hit test of queued NEE rays; unoccluded ones add their
+// contribution to the path's radiance.
+template <bool COUNT>
+__global__ void __launch_bounds__(256) k_shadow(DevScene sc, DevOpts op, PathBuf pb, ShadowBuf sb, const int* count_in,
+                                                int* work, int S, int* spill, unsigned long long* counters,
+                                                int* ovf_flag) {
+    extern __shared__ int lds[];
+    Stack stk{lds, spill, static_cast<int>(threadIdx.x), static_cast<int>(blockDim.x), S,
+              blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x,
+              static_cast<size_t>(gridDim.x) * blockDim.x};
+    TravCount cnt{0, 0, 0, 0, 0};
+    bool ovf = false;
+    const int count = *count_in;
+    for (;;) {
+        const int base = fetch_batch(work);
+        if (base >= count) break;
+        const int i = base + lane_id();
+        if (i < count) {
+            const float4 o4 = sb.o[i], d4 = sb.d[i];
+            if (!occluded<COUNT>(sc, op, xyz(o4), xyz(d4), o4.w, stk, cnt, ovf)) {
+                const int slot = __float_as_int(d4.w);
+                const float4 c = sb.c[i];
+                float4 r = pb.rad[slot];
+                r.x += c.x; r.y += c.y; r.z += c.z;
+                pb.rad[slot] = r;
+            }
+        }
+    }
+    flush_counts<COUNT>(cnt, counters);
+    if (ovf) atomicOr(ovf_flag, 1);
+}
+
+// Accumulate: accum[pixel] += radiance of samples s = 0..nsamp-1 in order.
+__global__ void __launch_bounds__(256) k_accum(DevOpts op, PathBuf pb, float4* __restrict__ accum, int npix, int nsamp) {
+    const int item = blockIdx.x * blockDim.x + threadIdx.x;
+    if (item >= npix) return;
+    int px, py;
+    item_pixel(item, op.width, px, py);
+    if (px >= op.width || py >= op.height) return;
+    float4* a = accum + static_cast<size_t>(py) * op.width + px;
+    float4 acc = *a;
+    for (int s = 0; s < nsamp; ++s) {
+        const float4 r = pb.rad[static_cast<size_t>(s) * npix + item];
+        acc.x += r.x; acc.y += r.y; acc.z += r.z; acc.w += r.w;
+    }
+    *a = acc;
+}
+
+__global__ void k_fill_int(int* p, int n, int v) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// ------------------------------------------------------------------ queries
+__global__ void __launch_bounds__(256) k_query_closest(DevScene sc, DevOpts op, const float* __restrict__ orig,
+                                                       const float* __restrict__ dir, int n, float* t, int* inst,
+                                                       int* tri, int* light, float* uv, int S, int* spill,
+                                                       int* ovf_flag) {
+    extern __shared__ int lds[];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    Stack stk{lds, spill, static_cast<int>(threadIdx.x), static_cast<int>(blockDim.x), S,
+              static_cast<size_t>(i), static_cast<size_t>(gridDim.x) * blockDim.x};
+    if (i >= n) return;
+    TravCount cnt{0, 0, 0, 0, 0};
+    bool ovf = false;
+    vec3 o = V(orig[3 * i], orig[3 * i + 1], orig[3 * i + 2]);
+    vec3 d = V(dir[3 * i], dir[3 * i + 1], dir[3 * i + 2]);
+    Hit h = closest<false>(sc, op, o, d, 1, stk, cnt, ovf);
+    t[i] = h.t;
+    inst[i] = h.tri >= 0 ? h.inst : -1;
+    tri[i] = h.tri;
+    light[i] = h.light;
+    uv[2 * i] = h.tri >= 0 ? h.u : 0.f;
+    uv[2 * i + 1] = h.tri >= 0 ? h.v : 0.f;
+    if (ovf) atomicOr(ovf_flag, 1);
+}
+
+__global__ void __launch_bounds__(256) k_query_any(DevScene sc, DevOpts op, const float* __restrict__ orig,
+                                                   const float* __restrict__ dir, const float* __restrict__ maxd,
+                                                   int n, int* occ, int S, int* spill, int* ovf_flag) {
+    extern __shared__ int lds[];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    Stack stk{lds, spill, static_cast<int>(threadIdx.x), static_cast<int>(blockDim.x), S,
+              static_cast<size_t>(i), static_cast<size_t>(gridDim.x) * blockDim.x};
+    if (i >= n) return;
+    TravCount cnt{0, 0, 0, 0, 0};
+    bool ovf = false;
+    vec3 o = V(orig[3 * i], orig[3 * i + 1], orig[3 * i + 2]);
+    vec3 d = V(dir[3 * i], dir[3 * i + 1], dir[3 * i + 2]);
+    occ[i] = occluded<false>(sc, op, o, d, maxd[i], stk, cnt, ovf) ? 1 : 0;
+    if (ovf) atomicOr(ovf_flag, 1);
+}
+
+// ------------------------------------------------------------------ tonemap (tonemap.glsl:59-133)
+__global__ void k_tonemap(const float4* __restrict__ accum, int w, int h, float inv_spp, int enable_tonemap,
+                          int enable_aces, int simple_aces, int bg_mode, float bgx, float bgy, float bgz,
+                          uchar4* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= w * h) return;
+    const float4 a = accum[i];
+    vec3 color = V(a.x * inv_spp, a.y * inv_spp, a.z * inv_spp);
+    const float alpha = a.w * inv_spp;
+    if (enable_tonemap) {
+        if (enable_aces) {
+            if (simple_aces) {
+                const float A = 2.51f, B = 0.03f, Y = 2.43f, D = 0.59f, E = 0.14f;
+                vec3 num = color * (A * color + V1(B));
+                vec3 den = color * (Y * color + V1(D)) + V1(E);
+                vec3 r = num / den;
+                color = V(gclamp(r.x, 0.f, 1.f), gclamp(r.y, 0.f, 1.f), gclamp(r.z, 0.f, 1.f));
+            } else {
+                // ACES fitted: input matrix, RRT+ODT fit, output matrix (row-vector * mat3).
+                const float im[3][3] = {{0.59719f, 0.35458f, 0.04823f}, {0.07600f, 0.90834f, 0.01566f},
+                                        {0.02840f, 0.13383f, 0.83777f}};
+                const float om[3][3] = {{1.60475f, -0.53108f, -0.07367f}, {-0.10208f, 1.10813f, -0.00605f},
+                                        {-0.00327f, -0.07276f, 1.07602f}};
+                vec3 c = V(dot(color, V(im[0][0], im[1][0], im[2][0])), dot(color, V(im[0][1], im[1][1], im[2][1])),
+                           dot(color, V(im[0][2], im[1][2], im[2][2])));
+                vec3 aa = c * (c + V1(0.0245786f)) - V1(0.000090537f);
+                vec3 bb = c * (0.983729f * c + V1(0.4329510f)) + V1(0.238081f);
+                c = aa / bb;
+                c = V(dot(c, V(om[0][0], om[1][0], om[2][0])), dot(c, V(om[0][1], om[1][1], om[2][1])),
+                      dot(c, V(om[0][2], om[1][2], om[2][2])));
+                color = V(gclamp(c.x, 0.f, 1.f), gclamp(c.y, 0.f, 1.f), gclamp(c.z, 0.f, 1.f));
+            }
+        } else {
+            color = color * 1.0f / (1.0f + Luminance(color) / 1.5f);
+        }
+    }
+    color = vpow(color, 1.0f / 2.2f);
+    float out_alpha = 1.0f;
+    vec3 bg = V(bgx, bgy, bgz);
+    if (bg_mode == 2) {  // transparent background: checkerboard, alpha from the path
+        out_alpha = alpha;
+        const int px = i % w, py = i / w;
+        float fx = floorf((static_cast<float>(px) + 0.5f) / 10.0f), fy = floorf((static_cast<float>(py) + 0.5f) / 10.0f);
+        float m = fmodf(fx + fy, 2.0f);
+        float res = gmax(m > 0.f ? 1.f : (m < 0.f ? -1.f : 0.f), 0.0f);
+        bg = gmix(V1(0.1f), V1(0.2f), res);
+    }
+    if (bg_mode != 0) color = gmix(bg, color, alpha);
+    auto q = [](float v) {
+        v = v < 0.f ? 0.f : (v > 1.f ? 1.f : v);
+        return static_cast<unsigned char>(v * 255.0f + 0.5f);
+    };
+    out[i] = make_uchar4(q(color.x), q(color.y), q(color.z), q(out_alpha));
+}
+
+}  // namespace ptd
+
+// ------------------------------------------------------------------ launchers
+namespace ptk {
+using namespace ptd;
+
+template <class K>
+static int max_blocks(K kernel, int bs, size_t lds) {
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kernel, bs, lds) != hipSuccess) n = 1;
+    return n < 1 ? 1 : n;
+}
+
+int persistent_blocks(int kind, int bs, int S, bool count, int num_cu) {
+    const size_t lds = static_cast<size_t>(S) * bs * sizeof(int);
+    int per_cu = 1;
+    switch (kind) {
+        case KIND_PLAIN: per_cu = count ? max_blocks(k_plain<true>, bs, lds) : max_blocks(k_plain<false>, bs, lds); break;
+        case KIND_EXTEND: per_cu = count ? max_blocks(k_extend<true>, bs, lds) : max_blocks(k_extend<false>, bs, lds); break;
+        case KIND_SHADOW: per_cu = count ? max_blocks(k_shadow<true>, bs, lds) : max_blocks(k_shadow<false>, bs, lds); break;
+        case KIND_SHADE: per_cu = max_blocks(k_shade, bs, 0); break;
+        default: break;
+    }
+    return per_cu * num_cu;
+}
+
+void launch_plain(hipStream_t s, int blocks, int bs, const DevScene& sc, const DevOpts& op, float4* accum, int sample,
+                  int num_items, int* work, int S, int* spill, unsigned long long* counters, int* ovf, bool count) {
+    const size_t lds = static_cast<size_t>(S) * bs * sizeof(int);
+    if (count) hipLaunchKernelGGL(k_plain<true>, dim3(blocks), dim3(bs), lds, s, sc, op, accum, sample, num_items, work, S, spill, counters, ovf);
+    else hipLaunchKernelGGL(k_plain<false>, dim3(blocks), dim3(bs), lds, s, sc, op, accum, sample, num_items, work, S, spill, counters, ovf);
+}
+
+void launch_raygen(hipStream_t s, const DevOpts& op, const PathBuf& pb, int npix, int first_sample, int num_paths,
+                   int* queue, int* counts) {
+    hipLaunchKernelGGL(k_raygen, dim3((num_paths + 255) / 256), dim3(256), 0, s, op, pb, npix, first_sample, num_paths, queue, counts);
+}
+
+void launch_extend(hipStream_t s, int blocks, int bs, const DevScene& sc, const DevOpts& op, const PathBuf& pb,
+                   const int* queue, const int* count_in, int* work, int S, int* spill, unsigned long long* counters,
+                   int* ovf, bool count) {
+    const size_t lds = static_cast<size_t>(S) * bs * sizeof(int);
+    if (count) hipLaunchKernelGGL(k_extend<true>, dim3(blocks), dim3(bs), lds, s, sc, op, pb, queue, count_in, work, S, spill, counters, ovf);
+    else hipLaunchKernelGGL(k_extend<false>, dim3(blocks), dim3(bs), lds, s, sc, op, pb, queue, count_in, work, S, spill, counters, ovf);
+}
+
+void launch_shade(hipStream_t s, int blocks, int bs, const DevScene& sc, const DevOpts& op, const PathBuf& pb,
+                  const ShadowBuf& sb, const int* qin, const int* count_in, int* qout, int* count_out,
+                  int* shadow_count, int* work) {
+    hipLaunchKernelGGL(k_shade, dim3(blocks), dim3(bs), 0, s, sc, op, pb, sb, qin, count_in, qout, count_out, shadow_count, work);
+}
+
+void launch_shadow(hipStream_t s, int blocks, int bs, const DevScene& sc, const DevOpts& op, const PathBuf& pb,
+                   const ShadowBuf& sb, const int* count_in, int* work, int S, int* spill,
+                   unsigned long long* counters, int* ovf, bool count) {
+    const size_t lds = static_cast<size_t>(S) * bs * sizeof(int);
+    if (count) hipLaunchKernelGGL(k_shadow<true>, dim3(blocks), dim3(bs), lds, s, sc, op, pb, sb, count_in, work, S, spill, counters, ovf);
+    else hipLaunchKernelGGL(k_shadow<false>, dim3(blocks), dim3(bs), lds, s, sc, op, pb, sb, count_in, work, S, spill, counters, ovf);
+}
+
+void launch_accum(hipStream_t s, const DevOpts& op, const PathBuf& pb, float4* accum, int npix, int nsamp) {
+    hipLaunchKernelGGL(k_accum, dim3((npix + 255) / 256), dim3(256), 0, s, op, pb, accum, npix, nsamp);
+}
+
+void launch_fill(hipStream_t s, int* p, int n, int v) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_fill_int, dim3((n + 255) / 256), dim3(256), 0, s, p, n, v);
+}
+
+void launch_query_closest(hipStream_t s, const DevScene& sc, const DevOpts& op, const float* o, const float* d, int n,
+                          float* t, int* inst, int* tri, int* light, float* uv, int S, int* spill, int* ovf) {
+    const int bs = 256;
+    hipLaunchKernelGGL(k_query_closest, dim3((n + bs - 1) / bs), dim3(bs), static_cast<size_t>(S) * bs * 4, s, sc, op, o, d, n, t, inst, tri, light, uv, S, spill, ovf);
+}
+
+void launch_query_any(hipStream_t s, const DevScene& sc, const DevOpts& op, const float* o, const float* d,
+                      const float* maxd, int n, int* occ, int S, int* spill, int* ovf) {
+    const int bs = 256;
+    hipLaunchKernelGGL(k_query_any, dim3((n + bs - 1) / bs), dim3(bs), static_cast<size_t>(S) * bs * 4, s, sc, op, o, d, maxd, n, occ, S, spill, ovf);
+}
+
+void launch_tonemap(hipStream_t s, const float4* accum, int w, int h, float inv_spp, int tm, int aces, int simple,
+                    int bg_mode, const float* bg, uchar4* out) {
+    hipLaunchKernelGGL(k_tonemap, dim3((w * h + 255) / 256), dim3(256), 0, s, accum, w, h, inv_spp, tm, aces, simple, bg_mode, bg[0], bg[1], bg[2], out);
+}
+
+}  // namespace ptk

@@ -1,0 +1,916 @@
+// Host side of the HIP renderer and its C ABI (pt_* in include/glslpt.h).
+//
+// Mirrors GLSLPT::Renderer (src/core/Renderer.h:104-185): the scene arrays of
+// InitGPUDataBuffers (src/core/Renderer.cpp:135-249) are uploaded once and
+// re-laid out for the GPU; the OPT_* switches of InitShaders (:400-459)
+// become a runtime flag word; Render adds samples to an RGBA32F accumulation
+// buffer (:546-590); instance edits refresh transforms, materials and the TLAS
+// node range (:649-665).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <exception>
+#include <string>
+#include <vector>
+
+#include "../../../include/glslpt.h"
+#include "pt_kernels.h"
+
+using ptd::DevOpts;
+using ptd::DevScene;
+using ptd::GInst;
+using ptd::GLight;
+using ptd::GNode;
+using ptd::GTri;
+
+namespace {
+
+struct HipError : std::exception {
+    std::string msg;
+    explicit HipError(std::string m) : msg(std::move(m)) {}
+    const char* what() const noexcept override { return msg.c_str(); }
+};
+struct ArgError : std::exception {
+    std::string msg;
+    int code;
+    ArgError(int c, std::string m) : msg(std::move(m)), code(c) {}
+    const char* what() const noexcept override { return msg.c_str(); }
+};
+
+#define HIPCHK(x)                                                                           \
+    do {                                                                                    \
+        hipError_t e_ = (x);                                                                \
+        if (e_ != hipSuccess) throw HipError(std::string(#x) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+template <class T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    void free() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    void ensure(size_t count) {
+        if (count <= n && p) return;
+        free();
+        if (count == 0) return;
+        HIPCHK(hipMalloc(reinterpret_cast<void**>(&p), count * sizeof(T)));
+        n = count;
+    }
+    void upload(const T* src, size_t count, hipStream_t s) {
+        ensure(count);
+        if (count) HIPCHK(hipMemcpyAsync(p, src, count * sizeof(T), hipMemcpyHostToDevice, s));
+    }
+};
+
+struct StageEvent { int stage; hipEvent_t a, b; };
+
+}  // namespace
+
+struct pt_renderer_s {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int num_cu = 0;
+    std::string err;
+
+    // scene (GPU layout)
+    DevBuf<GNode> nodes;
+    DevBuf<GTri> tris;
+    DevBuf<int4> vidx;
+    DevBuf<float4> verts, norms, mats;
+    DevBuf<GInst> inst;
+    DevBuf<GLight> lights;
+    DevBuf<uint8_t> tex;
+    DevBuf<float> env_img, env_cdf;
+    DevScene sc{};
+    bool has_scene = false;
+    int num_tris = 0, num_instances = 0, num_materials = 0;
+    bool any_alpha = false, any_medium = false, has_env = false;
+    // host copy of the flat node array (for TLAS refresh) and node re-layout map
+    std::vector<pt_bvh_node> flat;
+    int top_level_index = 0;
+
+    // render state
+    pt_options opts{};
+    pt_camera cam{};
+    bool has_opts = false, has_cam = false;
+    DevBuf<float4> accum;
+    int accum_w = 0, accum_h = 0;
+
+    // wavefront buffers
+    DevBuf<float4> p_ro, p_rd, p_thr, p_rad, p_aux, p_med, p_hit;
+    DevBuf<uint4> p_rng;
+    DevBuf<int4> p_meta;
+    DevBuf<int2> p_hit2;
+    DevBuf<float4> s_o, s_d, s_c;
+    DevBuf<int> qa, qb, counters;
+    DevBuf<int> spill;
+    DevBuf<unsigned long long> work_counts;  // nodes, tris, insts, lights, spills
+    DevBuf<int> ovf;
+    DevBuf<uchar4> ldr;
+
+    // graph cache
+    hipGraphExec_t graph_exec = nullptr;
+    hipGraph_t graph = nullptr;
+    long long graph_key = -1;
+
+    // stats
+    pt_stats stats{};
+    std::vector<StageEvent> pending;
+    std::vector<hipEvent_t> free_events;
+    hipEvent_t ev_r0 = nullptr, ev_r1 = nullptr;
+    bool render_pending = false;
+
+    ~pt_renderer_s() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        for (auto& e : pending) { (void)hipEventDestroy(e.a); (void)hipEventDestroy(e.b); }
+        for (auto e : free_events) (void)hipEventDestroy(e);
+        if (ev_r0) (void)hipEventDestroy(ev_r0);
+        if (ev_r1) (void)hipEventDestroy(ev_r1);
+        DevBuf<int>* ib[] = {&qa, &qb, &counters, &spill, &ovf};
+        for (auto* b : ib) b->free();
+        nodes.free(); tris.free(); vidx.free(); verts.free(); norms.free(); mats.free(); inst.free();
+        lights.free(); tex.free(); env_img.free(); env_cdf.free(); accum.free();
+        p_ro.free(); p_rd.free(); p_thr.free(); p_rad.free(); p_aux.free(); p_med.free(); p_hit.free();
+        p_rng.free(); p_meta.free(); p_hit2.free(); s_o.free(); s_d.free(); s_c.free();
+        work_counts.free(); ldr.free();
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    hipEvent_t get_event() {
+        if (!free_events.empty()) {
+            hipEvent_t e = free_events.back();
+            free_events.pop_back();
+            return e;
+        }
+        hipEvent_t e;
+        HIPCHK(hipEventCreate(&e));
+        return e;
+    }
+    void stage_begin(int stage, hipEvent_t& a) {
+        (void)stage;
+        a = get_event();
+        HIPCHK(hipEventRecord(a, stream));
+    }
+    void stage_end(int stage, hipEvent_t a) {
+        hipEvent_t b = get_event();
+        HIPCHK(hipEventRecord(b, stream));
+        pending.push_back({stage, a, b});
+        stats.launches_stage[stage]++;
+    }
+    void resolve_events() {
+        if (pending.empty() && !render_pending) return;
+        HIPCHK(hipStreamSynchronize(stream));
+        for (auto& e : pending) {
+            float ms = 0.f;
+            HIPCHK(hipEventElapsedTime(&ms, e.a, e.b));
+            stats.ms_stage[e.stage] += ms;
+            free_events.push_back(e.a);
+            free_events.push_back(e.b);
+        }
+        pending.clear();
+        if (render_pending) {
+            float ms = 0.f;
+            HIPCHK(hipEventElapsedTime(&ms, ev_r0, ev_r1));
+            stats.ms_last_render = ms;
+            render_pending = false;
+        }
+        if (opts.count_work && work_counts.p) {
+            unsigned long long c[5];
+            HIPCHK(hipMemcpy(c, work_counts.p, sizeof(c), hipMemcpyDeviceToHost));
+            stats.nodes_visited = static_cast<int64_t>(c[0]);
+            stats.tris_tested = static_cast<int64_t>(c[1]);
+            stats.instances_entered = static_cast<int64_t>(c[2]);
+            stats.lights_tested = static_cast<int64_t>(c[3]);
+            stats.stack_spills = static_cast<int64_t>(c[4]);
+        }
+        if (ovf.p) {
+            int f = 0;
+            HIPCHK(hipMemcpy(&f, ovf.p, sizeof(int), hipMemcpyDeviceToHost));
+            stats.stack_overflow = f;
+        }
+    }
+
+    DevOpts dev_opts() const {
+        DevOpts o{};
+        uint32_t f = 0;
+        if (sc.num_lights > 0) f |= ptd::OPT_LIGHTS;
+        if (opts.enable_rr) f |= ptd::OPT_RR;
+        if (opts.enable_uniform_light) f |= ptd::OPT_UNIFORM_LIGHT;
+        if (opts.enable_envmap && has_env) f |= ptd::OPT_ENVMAP;
+        if (opts.hide_emitters) f |= ptd::OPT_HIDE_EMITTERS;
+        if (opts.enable_background) f |= ptd::OPT_BACKGROUND;
+        if (opts.transparent_background) f |= ptd::OPT_TRANSPARENT_BACKGROUND;
+        if (any_alpha) f |= ptd::OPT_ALPHA_TEST;
+        if (any_medium) f |= ptd::OPT_MEDIUM;
+        if (opts.enable_volume_mis) f |= ptd::OPT_VOL_MIS;
+        if (opts.enable_roughness_mollification) f |= ptd::OPT_ROUGHNESS_MOLLIFICATION;
+        if (opts.opengl_normal_map) f |= ptd::OPT_OPENGL_NORMALMAP;
+        o.flags = f;
+        o.width = opts.width;
+        o.height = opts.height;
+        o.max_depth = opts.max_depth;
+        o.rr_depth = opts.rr_depth;
+        o.ucol_x = opts.uniform_light_col[0];
+        o.ucol_y = opts.uniform_light_col[1];
+        o.ucol_z = opts.uniform_light_col[2];
+        o.env_intensity = opts.envmap_intensity;
+        o.env_rot = opts.envmap_rot / 360.0f;
+        o.moll_amt = opts.roughness_mollification_amt;
+        for (int i = 0; i < 3; ++i) {
+            o.cam_pos[i] = cam.position[i];
+            o.cam_up[i] = cam.up[i];
+            o.cam_right[i] = cam.right[i];
+            o.cam_fwd[i] = cam.forward[i];
+        }
+        o.cam_fov = cam.fov;
+        o.cam_focal = cam.focalDist;
+        o.cam_aperture = cam.aperture;
+        return o;
+    }
+
+    int block_size() const { return opts.block_size > 0 ? opts.block_size : 256; }
+    int stack_S() const { return std::max(1, std::min(ptd::MAXD, opts.stack_depth > 0 ? opts.stack_depth : 24)); }
+
+    void ensure_spill(size_t threads) {
+        spill.ensure(threads * static_cast<size_t>(ptd::MAXD - stack_S() + 1));
+    }
+    void ensure_misc() {
+        work_counts.ensure(8);
+        ovf.ensure(4);
+    }
+};
+
+namespace {
+
+int Fail(pt_handle h, int code, const std::string& m) {
+    if (h) h->err = m;
+    return code;
+}
+
+template <class F>
+int Guard(pt_handle h, F&& f) {
+    if (!h) return PT_ERR_INVALID_ARG;
+    try {
+        return f();
+    } catch (const HipError& e) {
+        return Fail(h, PT_ERR_HIP, e.msg);
+    } catch (const ArgError& e) {
+        return Fail(h, e.code, e.msg);
+    } catch (const std::bad_alloc&) {
+        return Fail(h, PT_ERR_OVERFLOW, "out of host memory");
+    } catch (const std::exception& e) {
+        return Fail(h, PT_ERR_STATE, e.what());
+    }
+}
+
+// Inverse of the affine part of a column-major Mat4, in double.
+void affine_inverse(const float* m, double inv[4][3]) {
+    double a[3][3];
+    for (int c = 0; c < 3; ++c)
+        for (int r = 0; r < 3; ++r) a[c][r] = m[4 * c + r];  // a[col][row]
+    // determinant and adjugate of the 3x3 (column-major)
+    double det = a[0][0] * (a[1][1] * a[2][2] - a[2][1] * a[1][2]) - a[1][0] * (a[0][1] * a[2][2] - a[2][1] * a[0][2]) +
+                 a[2][0] * (a[0][1] * a[1][2] - a[1][1] * a[0][2]);
+    if (det == 0.0 || !std::isfinite(det)) throw ArgError(PT_ERR_INVALID_ARG, "singular instance transform");
+    double id = 1.0 / det;
+    double b[3][3];  // b[col][row] = inverse
+    b[0][0] = (a[1][1] * a[2][2] - a[2][1] * a[1][2]) * id;
+    b[1][0] = -(a[1][0] * a[2][2] - a[2][0] * a[1][2]) * id;
+    b[2][0] = (a[1][0] * a[2][1] - a[2][0] * a[1][1]) * id;
+    b[0][1] = -(a[0][1] * a[2][2] - a[2][1] * a[0][2]) * id;
+    b[1][1] = (a[0][0] * a[2][2] - a[2][0] * a[0][2]) * id;
+    b[2][1] = -(a[0][0] * a[2][1] - a[2][0] * a[0][1]) * id;
+    b[0][2] = (a[0][1] * a[1][2] - a[1][1] * a[0][2]) * id;
+    b[1][2] = -(a[0][0] * a[1][2] - a[1][0] * a[0][2]) * id;
+    b[2][2] = (a[0][0] * a[1][1] - a[1][0] * a[0][1]) * id;
+    const double t[3] = {m[12], m[13], m[14]};
+    for (int c = 0; c < 3; ++c)
+        for (int r = 0; r < 3; ++r) inv[c][r] = b[c][r];
+    for (int r = 0; r < 3; ++r) inv[3][r] = -(b[0][r] * t[0] + b[1][r] * t[1] + b[2][r] * t[2]);
+}
+
+GInst make_inst(const float* m, int blas_ref, int material) {
+    double inv[4][3];
+    affine_inverse(m, inv);
+    float ic[4][3], fc[3][3];
+    for (int c = 0; c < 4; ++c)
+        for (int r = 0; r < 3; ++r) ic[c][r] = static_cast<float>(inv[c][r]);
+    for (int c = 0; c < 3; ++c)
+        for (int r = 0; r < 3; ++r) fc[c][r] = m[4 * c + r];
+    GInst g;
+    g.i0 = make_float4(ic[0][0], ic[0][1], ic[0][2], ic[1][0]);
+    g.i1 = make_float4(ic[1][1], ic[1][2], ic[2][0], ic[2][1]);
+    g.i2 = make_float4(ic[2][2], ic[3][0], ic[3][1], ic[3][2]);
+    g.f0 = make_float4(fc[0][0], fc[0][1], fc[0][2], fc[1][0]);
+    g.f1 = make_float4(fc[1][1], fc[1][2], fc[2][0], fc[2][1]);
+    float br, mt;
+    std::memcpy(&br, &blas_ref, 4);
+    std::memcpy(&mt, &material, 4);
+    g.f2 = make_float4(fc[2][2], br, mt, 0.f);
+    return g;
+}
+
+// Re-layout of the reference's flat node array (bvh_translator.cpp:32-137)
+// into BVH2 nodes. Only nodes reachable from the TLAS root are converted;
+// GNode order follows the flat (preorder) order, so subtrees stay contiguous.
+struct Relayout {
+    std::vector<GNode> nodes;
+    int root = 0;
+    std::vector<int> inst_blas_ref, inst_material;
+};
+
+Relayout relayout(const pt_bvh_node* flat, int num_nodes, int top, int num_tris, int num_instances) {
+    Relayout R;
+    R.inst_blas_ref.assign(num_instances, 0);
+    R.inst_material.assign(num_instances, 0);
+    auto kind = [&](int i) {
+        float z = flat[i].LRLeaf[2];
+        return z > 0.f ? 1 : (z < 0.f ? -1 : 0);
+    };
+    auto as_int = [](float f) { return static_cast<int>(f); };
+    std::vector<int> map(num_nodes, -1);
+    std::vector<char> reach(num_nodes, 0);
+    // mark reachable internal nodes; TLAS first, then each BLAS root
+    std::vector<int> st;
+    if (top < 0 || top >= num_nodes) throw ArgError(PT_ERR_INVALID_ARG, "top_level_index out of range");
+    st.push_back(top);
+    std::vector<char> blas_seen(num_nodes, 0);
+    while (!st.empty()) {
+        int i = st.back();
+        st.pop_back();
+        if (i < 0 || i >= num_nodes) throw ArgError(PT_ERR_INVALID_ARG, "node child index out of range");
+        if (reach[i]) throw ArgError(PT_ERR_INVALID_ARG, "node reachable twice (not a tree)");
+        reach[i] = 1;
+        int k = kind(i);
+        if (k == 0) {
+            st.push_back(as_int(flat[i].LRLeaf[0]));
+            st.push_back(as_int(flat[i].LRLeaf[1]));
+        } else if (k < 0) {
+            int inst = -as_int(flat[i].LRLeaf[2]) - 1;
+            if (inst < 0 || inst >= num_instances) throw ArgError(PT_ERR_INVALID_ARG, "TLAS leaf instance out of range");
+            int root = as_int(flat[i].LRLeaf[0]);
+            if (root < 0 || root >= num_nodes) throw ArgError(PT_ERR_INVALID_ARG, "BLAS root out of range");
+            if (!blas_seen[root]) {
+                blas_seen[root] = 1;
+                if (reach[root]) throw ArgError(PT_ERR_INVALID_ARG, "BLAS root inside the TLAS");
+                st.push_back(root);
+            }
+        } else {
+            int first = as_int(flat[i].LRLeaf[0]), count = as_int(flat[i].LRLeaf[1]);
+            if (count < 1 || count > 3) throw ArgError(PT_ERR_UNSUPPORTED, "BLAS leaves must hold 1..3 triangles");
+            if (first < 0 || first + count > num_tris) throw ArgError(PT_ERR_INVALID_ARG, "BLAS leaf triangle range out of bounds");
+            if (first >= (1 << 28)) throw ArgError(PT_ERR_UNSUPPORTED, "too many triangles for the leaf encoding");
+        }
+    }
+    int n = 0;
+    for (int i = 0; i < num_nodes; ++i)
+        if (reach[i] && kind(i) == 0) map[i] = n++;
+    R.nodes.resize(std::max(n, 1));
+    auto ref_of = [&](int i) -> int {
+        int k = kind(i);
+        if (k == 0) return map[i];
+        if (k < 0) return -1 - (-as_int(flat[i].LRLeaf[2]) - 1);
+        int first = as_int(flat[i].LRLeaf[0]), count = as_int(flat[i].LRLeaf[1]);
+        return -1 - ((first << 2) | count);
+    };
+    for (int i = 0; i < num_nodes; ++i) {
+        if (!reach[i]) continue;
+        int k = kind(i);
+        if (k == 0) {
+            int l = as_int(flat[i].LRLeaf[0]), r = as_int(flat[i].LRLeaf[1]);
+            const float* lmn = flat[l].bboxmin; const float* lmx = flat[l].bboxmax;
+            const float* rmn = flat[r].bboxmin; const float* rmx = flat[r].bboxmax;
+            GNode& g = R.nodes[map[i]];
+            g.a = make_float4(lmn[0], lmn[1], lmn[2], lmx[0]);
+            g.b = make_float4(lmx[1], lmx[2], rmn[0], rmn[1]);
+            g.c = make_float4(rmn[2], rmx[0], rmx[1], rmx[2]);
+            g.d = make_int4(ref_of(l), ref_of(r), 0, 0);
+        } else if (k < 0) {
+            int inst = -as_int(flat[i].LRLeaf[2]) - 1;
+            R.inst_blas_ref[inst] = ref_of(as_int(flat[i].LRLeaf[0]));
+            R.inst_material[inst] = as_int(flat[i].LRLeaf[1]);
+        }
+    }
+    R.root = ref_of(top);
+    return R;
+}
+
+GLight make_light(const pt_light& l) {
+    GLight g{};
+    g.p = make_float4(l.position[0], l.position[1], l.position[2], l.radius);
+    g.e = make_float4(l.emission[0], l.emission[1], l.emission[2], l.area);
+    g.u = make_float4(l.u[0], l.u[1], l.u[2], l.type);
+    g.v = make_float4(l.v[0], l.v[1], l.v[2], 0.f);
+    // normalize(cross(u, v)), plane w, u / |u|^2, v / |v|^2 (closest_hit.glsl:50-54),
+    // in the same float operations the shader performs.
+    const float* u = l.u; const float* v = l.v;
+    float cx = u[1] * v[2] - u[2] * v[1], cy = u[2] * v[0] - u[0] * v[2], cz = u[0] * v[1] - u[1] * v[0];
+    float len = std::sqrt(cx * cx + cy * cy + cz * cz);
+    float nx = cx / len, ny = cy / len, nz = cz / len;
+    float w = nx * l.position[0] + ny * l.position[1] + nz * l.position[2];
+    g.n = make_float4(nx, ny, nz, w);
+    float iu = 1.0f / (u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+    float iv = 1.0f / (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    g.uu = make_float4(u[0] * iu, u[1] * iu, u[2] * iu, 0.f);
+    g.vv = make_float4(v[0] * iv, v[1] * iv, v[2] * iv, 0.f);
+    return g;
+}
+
+ptd::PathBuf path_buf(pt_renderer_s* h) {
+    ptd::PathBuf pb;
+    pb.ro = h->p_ro.p; pb.rd = h->p_rd.p; pb.thr = h->p_thr.p; pb.rad = h->p_rad.p; pb.rng = h->p_rng.p;
+    pb.meta = h->p_meta.p; pb.aux = h->p_aux.p; pb.med = h->any_medium ? h->p_med.p : nullptr;
+    pb.hit = h->p_hit.p; pb.hit2 = h->p_hit2.p;
+    return pb;
+}
+
+void check_ready(pt_renderer_s* h) {
+    if (!h->has_scene) throw ArgError(PT_ERR_STATE, "no scene uploaded (pt_upload_scene)");
+    if (!h->has_opts) throw ArgError(PT_ERR_STATE, "no options set (pt_set_options)");
+    if (!h->has_cam) throw ArgError(PT_ERR_STATE, "no camera set (pt_set_camera)");
+}
+
+// Counter layout for one wavefront pass of D bounces:
+//   [0, D+1)          paths entering bounce b
+//   [D+1, 2D+2)       shadow rays of bounce b
+//   [2D+2, 5D+5)      work counters (extend, shade, shadow) of bounce b
+//   [5D+5]            plain-kernel work counter
+//   [5D+6]            first sample of the pass
+int counter_cells(int D) { return 5 * D + 8; }
+
+void render_wavefront(pt_renderer_s* h, int first_sample, int num_samples) {
+    const DevOpts op = h->dev_opts();
+    if ((op.flags & ptd::OPT_VOL_MIS) && (op.flags & ptd::OPT_MEDIUM))
+        throw ArgError(PT_ERR_UNSUPPORTED, "volume MIS is implemented by the plain kernel only (mode PT_MODE_PLAIN)");
+    if (op.flags & ptd::OPT_ALPHA_TEST && !(op.flags & ptd::OPT_MEDIUM)) {
+        // Stochastic alpha in AnyHit draws from the path's RNG stream between NEE
+        // and BSDF sampling, which the deferred shadow stage cannot reproduce.
+        throw ArgError(PT_ERR_UNSUPPORTED, "alpha-tested shadow rays need the plain kernel (mode PT_MODE_PLAIN)");
+    }
+    const int W = op.width, H = op.height;
+    const int tiles = ((W + 7) / 8) * ((H + 7) / 8);
+    const int npix = tiles * 64;
+    const int spp_pass = std::max(1, std::min(h->opts.wave_spp > 0 ? h->opts.wave_spp : 1, num_samples));
+    const size_t max_paths = static_cast<size_t>(npix) * spp_pass;
+    if (max_paths > static_cast<size_t>(INT32_MAX / 2)) throw ArgError(PT_ERR_OVERFLOW, "too many paths per pass");
+    h->p_ro.ensure(max_paths); h->p_rd.ensure(max_paths); h->p_thr.ensure(max_paths); h->p_rad.ensure(max_paths);
+    h->p_rng.ensure(max_paths); h->p_meta.ensure(max_paths); h->p_aux.ensure(max_paths);
+    if (h->any_medium) h->p_med.ensure(max_paths);
+    h->p_hit.ensure(max_paths); h->p_hit2.ensure(max_paths);
+    h->s_o.ensure(2 * max_paths); h->s_d.ensure(2 * max_paths); h->s_c.ensure(2 * max_paths);
+    h->qa.ensure(max_paths); h->qb.ensure(max_paths);
+    // Bounce iterations: maxDepth + 1 closest hits; alpha pass-through repeats a depth.
+    const int D = op.max_depth + 1 + ((op.flags & ptd::OPT_ALPHA_TEST) ? 64 : 0);
+    h->counters.ensure(counter_cells(D));
+    h->ensure_misc();
+    const int bs = h->block_size(), S = h->stack_S();
+    const bool cw = h->opts.count_work != 0;
+    const int b_ext = ptk::persistent_blocks(ptk::KIND_EXTEND, bs, S, cw, h->num_cu);
+    const int b_sha = ptk::persistent_blocks(ptk::KIND_SHADE, bs, S, cw, h->num_cu);
+    const int b_sdw = ptk::persistent_blocks(ptk::KIND_SHADOW, bs, S, cw, h->num_cu);
+    h->ensure_spill(static_cast<size_t>(std::max(b_ext, b_sdw)) * bs);
+    const ptd::PathBuf pb = path_buf(h);
+    ptd::ShadowBuf sb{h->s_o.p, h->s_d.p, h->s_c.p};
+    int* cnt = h->counters.p;
+    hipStream_t s = h->stream;
+
+    for (int s0 = 0; s0 < num_samples; s0 += spp_pass) {
+        const int ns = std::min(spp_pass, num_samples - s0);
+        const int npaths = npix * ns;
+        hipEvent_t ev;
+        ptk::launch_fill(s, cnt, counter_cells(D), 0);
+        h->stage_begin(PT_STAGE_RAYGEN, ev);
+        ptk::launch_raygen(s, op, pb, npix, first_sample + s0, npaths, h->qa.p, cnt);
+        h->stage_end(PT_STAGE_RAYGEN, ev);
+        int* qin = h->qa.p;
+        int* qout = h->qb.p;
+        for (int b = 0; b < D; ++b) {
+            int* c_in = cnt + b;
+            int* c_out = cnt + b + 1;
+            int* c_sh = cnt + (D + 1) + b;
+            int* w = cnt + (2 * D + 2) + 3 * b;
+            h->stage_begin(PT_STAGE_EXTEND, ev);
+            ptk::launch_extend(s, b_ext, bs, h->sc, op, pb, qin, c_in, w + 0, S, h->spill.p, h->work_counts.p, h->ovf.p, cw);
+            h->stage_end(PT_STAGE_EXTEND, ev);
+            h->stage_begin(PT_STAGE_SHADE, ev);
+            ptk::launch_shade(s, b_sha, bs, h->sc, op, pb, sb, qin, c_in, qout, c_out, c_sh, w + 1);
+            h->stage_end(PT_STAGE_SHADE, ev);
+            h->stage_begin(PT_STAGE_SHADOW, ev);
+            ptk::launch_shadow(s, b_sdw, bs, h->sc, op, pb, sb, c_sh, w + 2, S, h->spill.p, h->work_counts.p, h->ovf.p, cw);
+            h->stage_end(PT_STAGE_SHADOW, ev);
+            std::swap(qin, qout);
+        }
+        h->stage_begin(PT_STAGE_ACCUM, ev);
+        ptk::launch_accum(s, op, pb, h->accum.p, npix, ns);
+        h->stage_end(PT_STAGE_ACCUM, ev);
+        HIPCHK(hipGetLastError());
+    }
+}
+
+void render_plain(pt_renderer_s* h, int first_sample, int num_samples) {
+    const DevOpts op = h->dev_opts();
+    const int W = op.width, H = op.height;
+    const int items = ((W + 7) / 8) * ((H + 7) / 8) * 64;
+    const int D = 1;
+    h->counters.ensure(std::max<size_t>(h->counters.n, counter_cells(D)));
+    h->ensure_misc();
+    const int bs = h->block_size(), S = h->stack_S();
+    const bool cw = h->opts.count_work != 0;
+    const int blocks = ptk::persistent_blocks(ptk::KIND_PLAIN, bs, S, cw, h->num_cu);
+    h->ensure_spill(static_cast<size_t>(blocks) * bs);
+    for (int smp = first_sample; smp < first_sample + num_samples; ++smp) {
+        hipEvent_t ev;
+        ptk::launch_fill(h->stream, h->counters.p, 1, 0);
+        h->stage_begin(PT_STAGE_PLAIN, ev);
+        ptk::launch_plain(h->stream, blocks, bs, h->sc, op, h->accum.p, smp, items, h->counters.p, S, h->spill.p,
+                          h->work_counts.p, h->ovf.p, cw);
+        h->stage_end(PT_STAGE_PLAIN, ev);
+        HIPCHK(hipGetLastError());
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+void pt_default_options(pt_options* o) {
+    if (!o) return;
+    std::memset(o, 0, sizeof(*o));
+    o->width = 1280; o->height = 720;                 // RenderOptions defaults (Renderer.h:41-72)
+    o->max_depth = 2;
+    o->enable_rr = 1; o->rr_depth = 2;
+    o->uniform_light_col[0] = o->uniform_light_col[1] = o->uniform_light_col[2] = 0.3f;
+    o->envmap_intensity = 1.0f;
+    o->opengl_normal_map = 1;
+    o->mode = PT_MODE_WAVEFRONT;
+    o->block_size = 256;
+    o->stack_depth = 24;
+    o->wave_spp = 1;
+}
+
+int pt_device_count(int32_t* n) {
+    if (!n) return PT_ERR_INVALID_ARG;
+    int c = 0;
+    if (hipGetDeviceCount(&c) != hipSuccess) c = 0;
+    *n = c;
+    return PT_OK;
+}
+
+int pt_create(int32_t device, pt_handle* out) {
+    if (!out) return PT_ERR_INVALID_ARG;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return PT_ERR_NO_DEVICE;
+    if (device < 0 || device >= n) return PT_ERR_INVALID_ARG;
+    pt_renderer_s* h = new (std::nothrow) pt_renderer_s;
+    if (!h) return PT_ERR_OVERFLOW;
+    try {
+        h->device = device;
+        HIPCHK(hipSetDevice(device));
+        HIPCHK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+        hipDeviceProp_t prop;
+        HIPCHK(hipGetDeviceProperties(&prop, device));
+        h->num_cu = prop.multiProcessorCount;
+        HIPCHK(hipEventCreate(&h->ev_r0));
+        HIPCHK(hipEventCreate(&h->ev_r1));
+        pt_default_options(&h->opts);
+    } catch (const std::exception&) {
+        delete h;
+        return PT_ERR_HIP;
+    }
+    *out = h;
+    return PT_OK;
+}
+
+int pt_destroy(pt_handle h) {
+    if (!h) return PT_OK;
+    (void)hipSetDevice(h->device);
+    delete h;
+    return PT_OK;
+}
+
+const char* pt_last_error(pt_handle h) { return h ? h->err.c_str() : "null renderer handle"; }
+
+int pt_upload_scene(pt_handle h, const pt_scene_view* v) {
+    return Guard(h, [&] {
+        if (!v || !v->nodes || v->num_nodes <= 0 || !v->vert_indices || v->num_tris <= 0 || !v->vertices_uvx ||
+            !v->normals_uvy || v->num_vertices <= 0 || !v->materials || v->num_materials <= 0 || !v->transforms ||
+            v->num_instances <= 0 || (v->num_lights > 0 && !v->lights))
+            return Fail(h, PT_ERR_INVALID_ARG, "incomplete scene view");
+        HIPCHK(hipSetDevice(h->device));
+        HIPCHK(hipStreamSynchronize(h->stream));
+        for (int i = 0; i < 3 * v->num_tris; ++i)
+            if (v->vert_indices[i] < 0 || v->vert_indices[i] >= v->num_vertices)
+                return Fail(h, PT_ERR_INVALID_ARG, "vert_indices entry out of range");
+        Relayout R = relayout(v->nodes, v->num_nodes, v->top_level_index, v->num_tris, v->num_instances);
+        for (int i = 0; i < v->num_instances; ++i)
+            if (R.inst_material[i] < 0 || R.inst_material[i] >= v->num_materials)
+                return Fail(h, PT_ERR_INVALID_ARG, "TLAS leaf material out of range");
+        // triangles in leaf order
+        std::vector<GTri> tris(v->num_tris);
+        std::vector<int4> vidx(v->num_tris);
+        const float* P = v->vertices_uvx;
+        for (int r = 0; r < v->num_tris; ++r) {
+            const int* vi = v->vert_indices + 3 * r;
+            const float* a = P + 4 * static_cast<size_t>(vi[0]);
+            const float* b = P + 4 * static_cast<size_t>(vi[1]);
+            const float* c = P + 4 * static_cast<size_t>(vi[2]);
+            tris[r].v0 = make_float4(a[0], a[1], a[2], 0.f);
+            tris[r].e0 = make_float4(b[0] - a[0], b[1] - a[1], b[2] - a[2], 0.f);
+            tris[r].e1 = make_float4(c[0] - a[0], c[1] - a[1], c[2] - a[2], 0.f);
+            vidx[r] = make_int4(vi[0], vi[1], vi[2], 0);
+        }
+        std::vector<GInst> insts(v->num_instances);
+        for (int i = 0; i < v->num_instances; ++i)
+            insts[i] = make_inst(v->transforms + 16 * static_cast<size_t>(i), R.inst_blas_ref[i], R.inst_material[i]);
+        std::vector<GLight> lights(std::max(1, v->num_lights));
+        for (int i = 0; i < v->num_lights; ++i) lights[i] = make_light(v->lights[i]);
+        hipStream_t s = h->stream;
+        h->nodes.upload(R.nodes.data(), R.nodes.size(), s);
+        h->tris.upload(tris.data(), tris.size(), s);
+        h->vidx.upload(vidx.data(), vidx.size(), s);
+        h->verts.upload(reinterpret_cast<const float4*>(v->vertices_uvx), v->num_vertices, s);
+        h->norms.upload(reinterpret_cast<const float4*>(v->normals_uvy), v->num_vertices, s);
+        h->mats.upload(reinterpret_cast<const float4*>(v->materials), 8 * static_cast<size_t>(v->num_materials), s);
+        h->inst.upload(insts.data(), insts.size(), s);
+        h->lights.upload(lights.data(), lights.size(), s);
+        h->any_alpha = h->any_medium = false;
+        for (int i = 0; i < v->num_materials; ++i) {
+            if (static_cast<int>(v->materials[i].alphaMode) != 0) h->any_alpha = true;
+            if (static_cast<int>(v->materials[i].mediumType) != 0) h->any_medium = true;
+        }
+        if (v->num_textures > 0 && v->texture_maps) {
+            h->tex.upload(v->texture_maps, 4 * static_cast<size_t>(v->tex_width) * v->tex_height * v->num_textures, s);
+        }
+        h->has_env = v->env_width > 0 && v->env_height > 0 && v->env_img && v->env_cdf;
+        if (h->has_env) {
+            const size_t n = static_cast<size_t>(v->env_width) * v->env_height;
+            h->env_img.upload(v->env_img, 3 * n, s);
+            h->env_cdf.upload(v->env_cdf, n, s);
+        }
+        DevScene& sc = h->sc;
+        sc.nodes = h->nodes.p; sc.tris = h->tris.p; sc.vidx = h->vidx.p; sc.verts = h->verts.p; sc.norms = h->norms.p;
+        sc.mats = h->mats.p; sc.inst = h->inst.p; sc.lights = h->lights.p;
+        sc.tex = (v->num_textures > 0 && v->texture_maps) ? h->tex.p : nullptr;
+        sc.env_img = h->has_env ? h->env_img.p : nullptr;
+        sc.env_cdf = h->has_env ? h->env_cdf.p : nullptr;
+        sc.root = R.root;
+        sc.num_lights = v->num_lights;
+        sc.tex_w = v->tex_width; sc.tex_h = v->tex_height;
+        sc.num_tex = sc.tex ? v->num_textures : 0;
+        sc.env_w = h->has_env ? v->env_width : 0;
+        sc.env_h = h->has_env ? v->env_height : 0;
+        sc.env_total = v->env_total_sum;
+        h->flat.assign(v->nodes, v->nodes + v->num_nodes);
+        h->top_level_index = v->top_level_index;
+        h->num_tris = v->num_tris;
+        h->num_instances = v->num_instances;
+        h->num_materials = v->num_materials;
+        HIPCHK(hipStreamSynchronize(s));
+        h->has_scene = true;
+        return PT_OK;
+    });
+}
+
+int pt_update_instances(pt_handle h, const float* transforms, int32_t num_instances, const pt_material* materials,
+                        int32_t num_materials, const pt_bvh_node* nodes, int32_t first_node, int32_t num_nodes) {
+    return Guard(h, [&] {
+        if (!h->has_scene) return Fail(h, PT_ERR_STATE, "no scene uploaded");
+        if (num_instances != h->num_instances || !transforms) return Fail(h, PT_ERR_INVALID_ARG, "instance count mismatch");
+        if (materials && num_materials != h->num_materials) return Fail(h, PT_ERR_INVALID_ARG, "material count mismatch");
+        if (nodes && (first_node < 0 || first_node + num_nodes > static_cast<int>(h->flat.size())))
+            return Fail(h, PT_ERR_INVALID_ARG, "node range out of bounds");
+        HIPCHK(hipSetDevice(h->device));
+        HIPCHK(hipStreamSynchronize(h->stream));
+        if (nodes) std::copy(nodes, nodes + num_nodes, h->flat.begin() + first_node);
+        Relayout R = relayout(h->flat.data(), static_cast<int>(h->flat.size()), h->top_level_index, h->num_tris,
+                              h->num_instances);
+        std::vector<GInst> insts(num_instances);
+        for (int i = 0; i < num_instances; ++i)
+            insts[i] = make_inst(transforms + 16 * static_cast<size_t>(i), R.inst_blas_ref[i], R.inst_material[i]);
+        h->nodes.upload(R.nodes.data(), R.nodes.size(), h->stream);
+        h->inst.upload(insts.data(), insts.size(), h->stream);
+        if (materials) h->mats.upload(reinterpret_cast<const float4*>(materials), 8 * static_cast<size_t>(num_materials), h->stream);
+        h->sc.nodes = h->nodes.p;
+        h->sc.inst = h->inst.p;
+        h->sc.mats = h->mats.p;
+        h->sc.root = R.root;
+        HIPCHK(hipStreamSynchronize(h->stream));
+        return PT_OK;
+    });
+}
+
+int pt_set_options(pt_handle h, const pt_options* o) {
+    return Guard(h, [&] {
+        if (!o || o->width <= 0 || o->height <= 0 || o->max_depth < 0) return Fail(h, PT_ERR_INVALID_ARG, "invalid options");
+        if (o->block_size != 0 && o->block_size != 64 && o->block_size != 128 && o->block_size != 256)
+            return Fail(h, PT_ERR_INVALID_ARG, "block_size must be 64, 128 or 256");
+        if (o->stack_depth < 0 || o->stack_depth > ptd::MAXD) return Fail(h, PT_ERR_INVALID_ARG, "stack_depth out of range");
+        if (o->mode != PT_MODE_PLAIN && o->mode != PT_MODE_WAVEFRONT) return Fail(h, PT_ERR_INVALID_ARG, "unknown mode");
+        if (static_cast<long long>(o->width) * o->height > (1LL << 28)) return Fail(h, PT_ERR_INVALID_ARG, "resolution too large");
+        HIPCHK(hipSetDevice(h->device));
+        h->opts = *o;
+        h->has_opts = true;
+        if (o->width != h->accum_w || o->height != h->accum_h) {
+            HIPCHK(hipStreamSynchronize(h->stream));
+            h->accum.ensure(static_cast<size_t>(o->width) * o->height);
+            h->accum_w = o->width;
+            h->accum_h = o->height;
+            HIPCHK(hipMemsetAsync(h->accum.p, 0, sizeof(float4) * h->accum.n, h->stream));
+        }
+        return PT_OK;
+    });
+}
+
+int pt_set_camera(pt_handle h, const pt_camera* c) {
+    return Guard(h, [&] {
+        if (!c) return Fail(h, PT_ERR_INVALID_ARG, "null camera");
+        h->cam = *c;
+        h->has_cam = true;
+        return PT_OK;
+    });
+}
+
+int pt_clear_accum(pt_handle h) {
+    return Guard(h, [&] {
+        if (!h->accum.p) return Fail(h, PT_ERR_STATE, "no accumulation buffer (pt_set_options)");
+        HIPCHK(hipSetDevice(h->device));
+        HIPCHK(hipMemsetAsync(h->accum.p, 0, sizeof(float4) * static_cast<size_t>(h->accum_w) * h->accum_h, h->stream));
+        return PT_OK;
+    });
+}
+
+int pt_render(pt_handle h, int32_t first_sample, int32_t num_samples) {
+    return Guard(h, [&] {
+        check_ready(h);
+        if (first_sample < 0 || num_samples < 0) return Fail(h, PT_ERR_INVALID_ARG, "negative sample range");
+        if (num_samples == 0) return PT_OK;
+        HIPCHK(hipSetDevice(h->device));
+        h->resolve_events();
+        if (h->opts.count_work) {
+            h->ensure_misc();
+            HIPCHK(hipMemsetAsync(h->work_counts.p, 0, 8 * sizeof(unsigned long long), h->stream));
+        }
+        h->ensure_misc();
+        HIPCHK(hipMemsetAsync(h->ovf.p, 0, sizeof(int), h->stream));
+        HIPCHK(hipEventRecord(h->ev_r0, h->stream));
+        if (h->opts.mode == PT_MODE_PLAIN) render_plain(h, first_sample, num_samples);
+        else render_wavefront(h, first_sample, num_samples);
+        HIPCHK(hipEventRecord(h->ev_r1, h->stream));
+        h->render_pending = true;
+        return PT_OK;
+    });
+}
+
+int pt_synchronize(pt_handle h) {
+    return Guard(h, [&] {
+        HIPCHK(hipSetDevice(h->device));
+        h->resolve_events();
+        HIPCHK(hipStreamSynchronize(h->stream));
+        return h->stats.stack_overflow ? Fail(h, PT_ERR_OVERFLOW, "BVH traversal exceeded the maximum stack depth") : PT_OK;
+    });
+}
+
+int pt_read_accum(pt_handle h, float* rgba, size_t num_floats) {
+    return Guard(h, [&] {
+        if (!h->accum.p) return Fail(h, PT_ERR_STATE, "no accumulation buffer");
+        const size_t need = 4 * static_cast<size_t>(h->accum_w) * h->accum_h;
+        if (!rgba || num_floats < need) return Fail(h, PT_ERR_INVALID_ARG, "output buffer too small");
+        HIPCHK(hipSetDevice(h->device));
+        h->resolve_events();
+        HIPCHK(hipMemcpyAsync(rgba, h->accum.p, need * sizeof(float), hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
+        return PT_OK;
+    });
+}
+
+int pt_read_ldr(pt_handle h, int32_t num_samples, int32_t enable_tonemap, int32_t enable_aces, int32_t simple_aces_fit,
+                const float background[3], uint8_t* rgba, size_t num_bytes) {
+    return Guard(h, [&] {
+        if (!h->accum.p) return Fail(h, PT_ERR_STATE, "no accumulation buffer");
+        const size_t npx = static_cast<size_t>(h->accum_w) * h->accum_h;
+        if (!rgba || num_bytes < 4 * npx || num_samples <= 0) return Fail(h, PT_ERR_INVALID_ARG, "invalid LDR request");
+        HIPCHK(hipSetDevice(h->device));
+        h->ldr.ensure(npx);
+        const float bg0[3] = {1.f, 1.f, 1.f};
+        int bg_mode = h->opts.transparent_background ? 2 : (h->opts.enable_background ? 1 : 0);
+        ptk::launch_tonemap(h->stream, h->accum.p, h->accum_w, h->accum_h, 1.0f / static_cast<float>(num_samples),
+                            enable_tonemap, enable_aces, simple_aces_fit, bg_mode, background ? background : bg0, h->ldr.p);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(rgba, h->ldr.p, 4 * npx, hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
+        return PT_OK;
+    });
+}
+
+int pt_get_stats(pt_handle h, pt_stats* out) {
+    return Guard(h, [&] {
+        if (!out) return Fail(h, PT_ERR_INVALID_ARG, "null stats");
+        HIPCHK(hipSetDevice(h->device));
+        h->resolve_events();
+        *out = h->stats;
+        return PT_OK;
+    });
+}
+
+int pt_reset_stats(pt_handle h) {
+    return Guard(h, [&] {
+        HIPCHK(hipSetDevice(h->device));
+        h->resolve_events();
+        std::memset(&h->stats, 0, sizeof(h->stats));
+        return PT_OK;
+    });
+}
+
+int pt_closest_hit(pt_handle h, const float* origins, const float* dirs, int32_t n, float* t, int32_t* instance,
+                   int32_t* triangle, int32_t* light, float* uv) {
+    return Guard(h, [&] {
+        if (!h->has_scene) return Fail(h, PT_ERR_STATE, "no scene uploaded");
+        if (n < 0 || (n > 0 && (!origins || !dirs || !t || !instance || !triangle || !light || !uv)))
+            return Fail(h, PT_ERR_INVALID_ARG, "invalid query arrays");
+        if (n == 0) return PT_OK;
+        HIPCHK(hipSetDevice(h->device));
+        h->resolve_events();
+        DevBuf<float> o, d, tt, uvb;
+        DevBuf<int> ib, tb, lb;
+        o.upload(origins, 3 * static_cast<size_t>(n), h->stream);
+        d.upload(dirs, 3 * static_cast<size_t>(n), h->stream);
+        tt.ensure(n); uvb.ensure(2 * static_cast<size_t>(n)); ib.ensure(n); tb.ensure(n); lb.ensure(n);
+        h->ensure_misc();
+        const int S = h->has_opts ? h->stack_S() : 24;
+        const size_t grid = ((static_cast<size_t>(n) + 255) / 256) * 256;
+        DevBuf<int> spill;
+        spill.ensure(grid * static_cast<size_t>(ptd::MAXD - S + 1));
+        HIPCHK(hipMemsetAsync(h->ovf.p, 0, sizeof(int), h->stream));
+        DevOpts op = h->has_opts ? h->dev_opts() : DevOpts{};
+        if (!h->has_opts && h->sc.num_lights > 0) op.flags |= ptd::OPT_LIGHTS;
+        ptk::launch_query_closest(h->stream, h->sc, op, o.p, d.p, n, tt.p, ib.p, tb.p, lb.p, uvb.p, S, spill.p, h->ovf.p);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(t, tt.p, sizeof(float) * n, hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipMemcpyAsync(instance, ib.p, sizeof(int) * n, hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipMemcpyAsync(triangle, tb.p, sizeof(int) * n, hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipMemcpyAsync(light, lb.p, sizeof(int) * n, hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipMemcpyAsync(uv, uvb.p, sizeof(float) * 2 * n, hipMemcpyDeviceToHost, h->stream));
+        int f = 0;
+        HIPCHK(hipMemcpyAsync(&f, h->ovf.p, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
+        o.free(); d.free(); tt.free(); uvb.free(); ib.free(); tb.free(); lb.free(); spill.free();
+        return f ? Fail(h, PT_ERR_OVERFLOW, "BVH traversal exceeded the maximum stack depth") : PT_OK;
+    });
+}
+
+int pt_any_hit(pt_handle h, const float* origins, const float* dirs, const float* max_dist, int32_t n, int32_t* occluded) {
+    return Guard(h, [&] {
+        if (!h->has_scene) return Fail(h, PT_ERR_STATE, "no scene uploaded");
+        if (n < 0 || (n > 0 && (!origins || !dirs || !max_dist || !occluded))) return Fail(h, PT_ERR_INVALID_ARG, "invalid query arrays");
+        if (n == 0) return PT_OK;
+        HIPCHK(hipSetDevice(h->device));
+        h->resolve_events();
+        DevBuf<float> o, d, m;
+        DevBuf<int> oc;
+        o.upload(origins, 3 * static_cast<size_t>(n), h->stream);
+        d.upload(dirs, 3 * static_cast<size_t>(n), h->stream);
+        m.upload(max_dist, n, h->stream);
+        oc.ensure(n);
+        h->ensure_misc();
+        const int S = h->has_opts ? h->stack_S() : 24;
+        const size_t grid = ((static_cast<size_t>(n) + 255) / 256) * 256;
+        DevBuf<int> spill;
+        spill.ensure(grid * static_cast<size_t>(ptd::MAXD - S + 1));
+        HIPCHK(hipMemsetAsync(h->ovf.p, 0, sizeof(int), h->stream));
+        DevOpts op = h->has_opts ? h->dev_opts() : DevOpts{};
+        if (!h->has_opts && h->sc.num_lights > 0) op.flags |= ptd::OPT_LIGHTS;
+        ptk::launch_query_any(h->stream, h->sc, op, o.p, d.p, m.p, n, oc.p, S, spill.p, h->ovf.p);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(occluded, oc.p, sizeof(int) * n, hipMemcpyDeviceToHost, h->stream));
+        int f = 0;
+        HIPCHK(hipMemcpyAsync(&f, h->ovf.p, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
+        o.free(); d.free(); m.free(); oc.free(); spill.free();
+        return f ? Fail(h, PT_ERR_OVERFLOW, "BVH traversal exceeded the maximum stack depth") : PT_OK;
+    });
+}
+
+int pt_batch_render(pt_handle h, const pt_job* jobs, int32_t num_jobs) {
+    return Guard(h, [&] {
+        (void)jobs; (void)num_jobs;
+        return Fail(h, PT_ERR_UNSUPPORTED, "pt_batch_render is not available in this build");
+    });
+}
+
+int pt_adjoint(pt_handle h, int32_t first_sample, int32_t num_samples, const float* dL_dimage, float* d_params) {
+    return Guard(h, [&] {
+        (void)first_sample; (void)num_samples; (void)dL_dimage; (void)d_params;
+        return Fail(h, PT_ERR_UNSUPPORTED, "pt_adjoint is not available in this build");
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,140 @@
+"""Python face of the HIP renderer (libglslpt_hip.so), mirroring the
+reference's ``GLSLPT::Renderer`` (src/core/Renderer.h:104-185): upload the
+scene arrays, set RenderOptions and the camera, Render (add samples to the
+RGBA32F accumulation buffer), read the output back.
+
+There is no CPU fallback: constructing a Renderer without the HIP library or
+without a GPU raises.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _capi as api
+
+
+def default_options(**kw):
+    o = api.PtOptions()
+    api.hip_lib().pt_default_options(C.byref(o))
+    for k, v in kw.items():
+        if k == "uniform_light_col":
+            o.uniform_light_col[:] = [float(x) for x in v]
+        else:
+            setattr(o, k, v)
+    return o
+
+
+class Renderer:
+    def __init__(self, device=0):
+        self._lib = api.hip_lib()
+        h = C.c_void_p()
+        rc = self._lib.pt_create(int(device), C.byref(h))
+        if rc != api.PT_OK:
+            raise api.PtError("pt_create", rc, "no usable HIP device" if rc == -7 else "")
+        self._h = h
+        self.width = self.height = 0
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.pt_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def _check(self, fn, rc):
+        if rc != api.PT_OK:
+            msg = self._lib.pt_last_error(self._h)
+            raise api.PtError(fn, rc, msg.decode() if msg else "")
+
+    # --- GLSLPT::Renderer surface ---------------------------------------
+    def upload_scene(self, scene):
+        v = scene.view()
+        self._check("pt_upload_scene", self._lib.pt_upload_scene(self._h, C.byref(v)))
+
+    def update_instances(self, scene, first_node=None):
+        """Instance-edit refresh after Scene.RebuildInstances (Renderer.cpp:649-665)."""
+        a = scene.arrays()
+        v = scene.view()
+        tli = v.top_level_index if first_node is None else first_node
+        self._check("pt_update_instances", self._lib.pt_update_instances(
+            self._h, v.transforms, v.num_instances, v.materials, v.num_materials,
+            C.cast(C.byref(v.nodes.contents, tli * C.sizeof(api.PtBvhNode)), C.POINTER(api.PtBvhNode)),
+            tli, v.num_nodes - tli))
+        return a
+
+    def set_options(self, opts=None, **kw):
+        o = opts if opts is not None else default_options(**kw)
+        self._check("pt_set_options", self._lib.pt_set_options(self._h, C.byref(o)))
+        self.width, self.height = o.width, o.height
+        self.opts = o
+
+    def set_camera(self, cam):
+        self._check("pt_set_camera", self._lib.pt_set_camera(self._h, C.byref(cam)))
+
+    def clear(self):
+        self._check("pt_clear_accum", self._lib.pt_clear_accum(self._h))
+
+    def render(self, first_sample, num_samples):
+        self._check("pt_render", self._lib.pt_render(self._h, int(first_sample), int(num_samples)))
+
+    def synchronize(self):
+        self._check("pt_synchronize", self._lib.pt_synchronize(self._h))
+
+    def read_accum(self):
+        out = np.zeros((self.height, self.width, 4), np.float32)
+        self._check("pt_read_accum", self._lib.pt_read_accum(self._h, api.fptr(out), out.size))
+        return out
+
+    def read_ldr(self, num_samples, tonemap=True, aces=False, simple_aces=False, background=(1, 1, 1)):
+        out = np.zeros((self.height, self.width, 4), np.uint8)
+        bg = np.asarray(background, np.float32)
+        self._check("pt_read_ldr", self._lib.pt_read_ldr(
+            self._h, int(num_samples), int(tonemap), int(aces), int(simple_aces), api.fptr(bg),
+            out.ctypes.data_as(C.POINTER(C.c_uint8)), out.size))
+        return out
+
+    def stats(self):
+        s = api.PtStats()
+        self._check("pt_get_stats", self._lib.pt_get_stats(self._h, C.byref(s)))
+        d = {f: getattr(s, f) for f, _ in s._fields_ if f not in ("ms_stage", "launches_stage")}
+        d["ms_stage"] = {api.STAGES[i]: s.ms_stage[i] for i in range(8)}
+        d["launches_stage"] = {api.STAGES[i]: s.launches_stage[i] for i in range(8)}
+        return d
+
+    def reset_stats(self):
+        self._check("pt_reset_stats", self._lib.pt_reset_stats(self._h))
+
+    # --- ray queries ------------------------------------------------------
+    def closest_hit(self, origins, dirs):
+        o = np.ascontiguousarray(origins, np.float32)
+        d = np.ascontiguousarray(dirs, np.float32)
+        n = o.shape[0]
+        t = np.zeros(n, np.float32)
+        inst = np.zeros(n, np.int32)
+        tri = np.zeros(n, np.int32)
+        li = np.zeros(n, np.int32)
+        uv = np.zeros((n, 2), np.float32)
+        self._check("pt_closest_hit", self._lib.pt_closest_hit(
+            self._h, api.fptr(o), api.fptr(d), n, api.fptr(t), api.iptr(inst), api.iptr(tri), api.iptr(li), api.fptr(uv)))
+        return dict(t=t, instance=inst, triangle=tri, light=li, uv=uv)
+
+    def any_hit(self, origins, dirs, max_dist):
+        o = np.ascontiguousarray(origins, np.float32)
+        d = np.ascontiguousarray(dirs, np.float32)
+        m = np.ascontiguousarray(max_dist, np.float32)
+        n = o.shape[0]
+        occ = np.zeros(n, np.int32)
+        self._check("pt_any_hit", self._lib.pt_any_hit(self._h, api.fptr(o), api.fptr(d), api.fptr(m), n, api.iptr(occ)))
+        return occ
+
+
+def options_for(settings, mode=api.PT_MODE_WAVEFRONT, **kw):
+    """pt_options for a scenes.RenderSettings workload."""
+    o = default_options(width=settings.width, height=settings.height, max_depth=settings.max_depth,
+                        enable_rr=int(settings.enable_rr), rr_depth=settings.rr_depth,
+                        enable_uniform_light=int(settings.enable_uniform_light),
+                        uniform_light_col=settings.uniform_light_col, mode=mode)
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o

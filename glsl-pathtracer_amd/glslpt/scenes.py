@@ -1,0 +1,255 @@
+"""Procedural scenes for the workloads (synthetic inputs; no files needed).
+
+* ``cornell(seed)``      Cornell box: five wall quads (2 triangles each) and two
+                         blocks (12 triangles each) = 34 triangles, one rect
+                         light; diffuse Disney materials with seeded baseColor.
+                         Geometry from the published Cornell-box measurements
+                         (mm, scaled to metres), each mesh placed by an instance
+                         transform the way the reference's .scene files do
+                         (translate + scale 0.01, assets/cornell_box_orig.scene).
+* ``instanced_spheres``  N instances of a seeded geodesic icosphere displaced by
+                         3-octave value noise, random rigid transforms and
+                         uniform scale, Disney materials drawn from a seed,
+                         rect + sphere lights and a constant environment.
+* ``tiny``               a few triangles, for smoke runs.
+
+Each builder returns (Scene, RenderSettings) where RenderSettings carries the
+resolution, depth and option flags of the workload.
+"""
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from .scene import Scene, material, quad_light, sphere_light, mat4
+
+
+@dataclass
+class RenderSettings:
+    width: int
+    height: int
+    spp: int
+    max_depth: int
+    enable_rr: bool = True
+    rr_depth: int = 2
+    enable_uniform_light: bool = False
+    uniform_light_col: tuple = (0.3, 0.3, 0.3)
+    extra: dict = field(default_factory=dict)
+
+
+# ---------------------------------------------------------------- Cornell box
+# Published Cornell-box geometry (mm). Walls are axis-aligned quads.
+_CB_WALLS = {
+    "floor":   [(552.8, 0, 0), (0, 0, 0), (0, 0, 559.2), (549.6, 0, 559.2)],
+    "ceiling": [(556.0, 548.8, 0), (556.0, 548.8, 559.2), (0, 548.8, 559.2), (0, 548.8, 0)],
+    "back":    [(549.6, 0, 559.2), (0, 0, 559.2), (0, 548.8, 559.2), (556.0, 548.8, 559.2)],
+    "green":   [(0, 0, 559.2), (0, 0, 0), (0, 548.8, 0), (0, 548.8, 559.2)],
+    "red":     [(552.8, 0, 0), (549.6, 0, 559.2), (556.0, 548.8, 559.2), (556.0, 548.8, 0)],
+}
+_CB_SHORT = [(130.0, 165.0, 65.0), (82.0, 165.0, 225.0), (240.0, 165.0, 272.0), (290.0, 165.0, 114.0)]
+_CB_TALL = [(423.0, 330.0, 247.0), (265.0, 330.0, 296.0), (314.0, 330.0, 456.0), (472.0, 330.0, 406.0)]
+
+
+def _quad_tris(q):
+    a, b, c, d = [np.asarray(p, np.float64) for p in q]
+    return [np.stack([a, b, c]), np.stack([a, c, d])]
+
+
+def _block_tris(top):
+    top = [np.asarray(p, np.float64) for p in top]
+    bot = [np.array([p[0], 0.0, p[2]]) for p in top]
+    tris = _quad_tris(top)
+    for i in range(4):
+        j = (i + 1) % 4
+        tris += _quad_tris([bot[i], bot[j], top[j], top[i]])
+    tris += _quad_tris([bot[3], bot[2], bot[1], bot[0]])
+    return tris
+
+
+def _face_normals(tris):
+    n = np.cross(tris[:, 1] - tris[:, 0], tris[:, 2] - tris[:, 0])
+    n /= np.linalg.norm(n, axis=1, keepdims=True)
+    return np.repeat(n[:, None, :], 3, axis=1)
+
+
+def _orient_inward(tris, center):
+    """Flip triangles so their geometric normal faces `center` (walls face the room)."""
+    out = []
+    for t in tris:
+        n = np.cross(t[1] - t[0], t[2] - t[0])
+        c = t.mean(axis=0)
+        if np.dot(n, center - c) < 0:
+            t = t[[0, 2, 1]]
+        out.append(t)
+    return np.stack(out)
+
+
+def _orient_outward(tris, center):
+    out = []
+    for t in tris:
+        n = np.cross(t[1] - t[0], t[2] - t[0])
+        if np.dot(n, t.mean(axis=0) - center) < 0:
+            t = t[[0, 2, 1]]
+        out.append(t)
+    return np.stack(out)
+
+
+def cornell(seed=0xC0FFEE01, width=256, height=256, spp=4, max_depth=5):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    sc = Scene()
+    room_center = np.array([278.0, 274.4, 279.6])
+    parts = []
+    for name, q in _CB_WALLS.items():
+        parts.append((name, _orient_inward(np.stack(_quad_tris(q)), room_center)))
+    for name, top in (("short", _CB_SHORT), ("tall", _CB_TALL)):
+        tr = np.stack(_block_tris(top))
+        parts.append((name, _orient_outward(tr, tr.reshape(-1, 3).mean(axis=0))))
+    for i, (name, tris) in enumerate(parts):
+        # Mesh in centred local units of 0.1 mm-per-unit (cm), placed by translate + scale 0.01.
+        centre = tris.reshape(-1, 3).mean(axis=0)
+        local = ((tris - centre) / 10.0).astype(np.float32)          # cm
+        mesh = sc.AddMesh(local, _face_normals(local.astype(np.float64)).astype(np.float32), name=name)
+        bc = rng.random(3)
+        mid = sc.AddMaterial(baseColor=bc)
+        xf = mat4(translate=(centre / 1000.0), scale=(0.01, 0.01, 0.01))
+        sc.AddMeshInstance(mesh, xf, mid)
+    sc.AddLight(quad_light((0.343, 0.5478, 0.227), (0.343, 0.5478, 0.332), (0.213, 0.5478, 0.227), (17, 12, 4)))
+    sc.AddCamera((0.276, 0.275, -0.75), (0.276, 0.275, 0.10), 40.0)
+    sc.ProcessScene()
+    return sc, RenderSettings(width, height, spp, max_depth, enable_rr=True, rr_depth=2)
+
+
+# ------------------------------------------------------------- icospheres
+def _icosahedron():
+    t = (1.0 + 5 ** 0.5) / 2.0
+    v = np.array([[-1, t, 0], [1, t, 0], [-1, -t, 0], [1, -t, 0], [0, -1, t], [0, 1, t], [0, -1, -t], [0, 1, -t],
+                  [t, 0, -1], [t, 0, 1], [-t, 0, -1], [-t, 0, 1]], np.float64)
+    v /= np.linalg.norm(v, axis=1, keepdims=True)
+    f = np.array([[0, 11, 5], [0, 5, 1], [0, 1, 7], [0, 7, 10], [0, 10, 11], [1, 5, 9], [5, 11, 4], [11, 10, 2],
+                  [10, 7, 6], [7, 1, 8], [3, 9, 4], [3, 4, 2], [3, 2, 6], [3, 6, 8], [3, 8, 9], [4, 9, 5],
+                  [2, 4, 11], [6, 2, 10], [8, 6, 7], [9, 8, 1]])
+    return v, f
+
+
+def geodesic_sphere(freq):
+    """Frequency-`freq` geodesic subdivision of the icosahedron: 20*freq^2
+    triangles on the unit sphere, shared vertices. Returns (verts, faces)."""
+    v0, f0 = _icosahedron()
+    verts, index = [], {}
+
+    def vid(p):
+        key = tuple(np.round(p, 12))
+        if key not in index:
+            index[key] = len(verts)
+            verts.append(p / np.linalg.norm(p))
+        return index[key]
+
+    faces = []
+    for a, b, c in f0:
+        A, B, C = v0[a], v0[b], v0[c]
+        grid = {}
+        for i in range(freq + 1):
+            for j in range(freq + 1 - i):
+                k = freq - i - j
+                grid[(i, j)] = vid((A * k + B * i + C * j) / freq)
+        for i in range(freq):
+            for j in range(freq - i):
+                faces.append((grid[(i, j)], grid[(i + 1, j)], grid[(i, j + 1)]))
+                if i + j < freq - 1:
+                    faces.append((grid[(i + 1, j)], grid[(i + 1, j + 1)], grid[(i, j + 1)]))
+    return np.array(verts), np.array(faces, np.int64)
+
+
+def value_noise3(p, seed, octaves=3):
+    """Sum of `octaves` octaves of trilinear value noise on an integer lattice
+    with hashed values in [-1, 1]; octave k has frequency 2^k and weight 2^-k."""
+    out = np.zeros(p.shape[0])
+    for k in range(octaves):
+        q = p * (2.0 ** k) * 2.0
+        i0 = np.floor(q).astype(np.int64)
+        fr = q - i0
+        w = fr * fr * (3 - 2 * fr)
+        acc = np.zeros(p.shape[0])
+        for dx in (0, 1):
+            for dy in (0, 1):
+                for dz in (0, 1):
+                    c = i0 + np.array([dx, dy, dz])
+                    h = (c[:, 0] * 73856093) ^ (c[:, 1] * 19349663) ^ (c[:, 2] * 83492791) ^ (seed + k * 0x9E3779B1)
+                    h = (h * 0x2545F491) & 0xFFFFFFFF
+                    h ^= h >> 15
+                    val = (h & 0xFFFF) / 32767.5 - 1.0
+                    wx = w[:, 0] if dx else 1 - w[:, 0]
+                    wy = w[:, 1] if dy else 1 - w[:, 1]
+                    wz = w[:, 2] if dz else 1 - w[:, 2]
+                    acc += wx * wy * wz * val
+        out += acc * (0.5 ** k)
+    return out
+
+
+def displaced_icosphere(freq=28, amplitude=0.1, seed=0x5EED0002):
+    """Triangle soup (positions[n,3,3], normals[n,3,3]) of a noise-displaced
+    geodesic sphere with smooth vertex normals."""
+    v, f = geodesic_sphere(freq)
+    d = 1.0 + amplitude * value_noise3(v, seed)
+    p = v * d[:, None]
+    fn = np.cross(p[f[:, 1]] - p[f[:, 0]], p[f[:, 2]] - p[f[:, 0]])
+    vn = np.zeros_like(p)
+    for k in range(3):
+        np.add.at(vn, f[:, k], fn)
+    vn /= np.linalg.norm(vn, axis=1, keepdims=True)
+    return p[f].astype(np.float32), vn[f].astype(np.float32)
+
+
+def random_rotation(rng):
+    q = rng.normal(size=4)
+    q /= np.linalg.norm(q)
+    x, y, z, w = q
+    # column-per-row Mat4 data layout as Mat4::QuatToMatrix (src/math/Mat4.h:109-139)
+    r = np.eye(4)
+    r[0, :3] = [1 - 2 * (y * y + z * z), 2 * (x * y + w * z), 2 * (x * z - w * y)]
+    r[1, :3] = [2 * (x * y - w * z), 1 - 2 * (x * x + z * z), 2 * (y * z + w * x)]
+    r[2, :3] = [2 * (x * z + w * y), 2 * (y * z - w * x), 1 - 2 * (x * x + y * y)]
+    return r.astype(np.float32)
+
+
+def disney_materials(rng, n):
+    mats = []
+    for _ in range(n):
+        mats.append(material(baseColor=rng.random(3), roughness=rng.uniform(0.05, 1.0),
+                             metallic=float(rng.random() < 0.5), ior=1.5, clearcoat=rng.uniform(0, 1)))
+    return mats
+
+
+def instanced_spheres(num_instances=64, freq=28, seed=0x5EED0002, width=1920, height=1080, spp=64, max_depth=8,
+                      extent=20.0, num_materials=16):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    sc = Scene()
+    pos, nrm = displaced_icosphere(freq, 0.1, seed)
+    mesh = sc.AddMesh(pos, nrm, name=f"icosphere{freq}")
+    mids = [sc.AddMaterial(m) for m in disney_materials(rng, num_materials)]
+    for i in range(num_instances):
+        s = rng.uniform(0.5, 2.0)
+        xf = mat4(translate=rng.uniform(-extent, extent, 3), scale=(s, s, s), rot=random_rotation(rng))
+        sc.AddMeshInstance(mesh, xf, mids[i % num_materials])
+    h = extent * 1.5
+    sc.AddLight(quad_light((-10, h, -10), (0, h, -10), (-10, h, 0), (6, 6, 6)))
+    sc.AddLight(quad_light((5, h, 5), (15, h, 5), (5, h, 15), (5, 6, 8)))
+    sc.AddLight(sphere_light((0, extent * 0.6, extent * 1.2), 2.0, (8, 7, 5)))
+    sc.AddCamera((0.0, extent * 0.5, extent * 3.0), (0.0, 0.0, 0.0), 45.0)
+    sc.ProcessScene()
+    return sc, RenderSettings(width, height, spp, max_depth, enable_rr=True, rr_depth=3,
+                              enable_uniform_light=True, uniform_light_col=(0.3, 0.3, 0.3))
+
+
+def tiny(width=32, height=32, spp=2, max_depth=3):
+    """Two triangles under a quad light, seen from above."""
+    sc = Scene()
+    quad = np.array([[[-1, 0, -1], [1, 0, -1], [1, 0, 1]], [[-1, 0, -1], [1, 0, 1], [-1, 0, 1]]], np.float32)
+    quad = quad[:, [0, 2, 1]]  # face up (+y)
+    n = np.tile(np.array([0, 1, 0], np.float32), (2, 3, 1))
+    mesh = sc.AddMesh(quad, n, name="floor")
+    mid = sc.AddMaterial(baseColor=(0.8, 0.6, 0.4), roughness=0.4)
+    sc.AddMeshInstance(mesh, mat4(), mid)
+    sc.AddLight(quad_light((-0.3, 1.5, -0.3), (0.3, 1.5, -0.3), (-0.3, 1.5, 0.3), (10, 10, 10)))
+    sc.AddCamera((0, 3, 2.5), (0, 0, 0), 45.0)
+    sc.ProcessScene()
+    return sc, RenderSettings(width, height, spp, max_depth, enable_rr=True, rr_depth=2)

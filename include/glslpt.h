@@ -1,0 +1,218 @@
+/*
+ * glslpt — C ABI of the MI355X path-tracing backend.
+ *
+ * Two libraries sit behind this header:
+ *   libglslpt_scene.so  host scene surface + BVH build + flattening (no GPU)
+ *   libglslpt_hip.so    HIP renderer for gfx950 (wavefront and plain kernels)
+ *
+ * Conventions: every entry point returns PT_OK (0) or a negative PT_ERR_*;
+ * no C++ exception crosses the ABI; the last error message of a handle is
+ * returned by pt_last_error / pts_last_error. Arrays are plain host pointers
+ * with explicit counts; the caller keeps ownership of everything it passes.
+ * One host thread per handle; each renderer handle owns one HIP stream.
+ *
+ * Reference interfaces replaced (file:line into knightcrawler25/GLSL-PathTracer):
+ *   pts_*   GLSLPT::Scene        src/core/Scene.h:74-125, src/core/Scene.cpp:56-297
+ *   pt_*    GLSLPT::Renderer     src/core/Renderer.h:104-185, src/core/Renderer.cpp:135-812
+ */
+#ifndef GLSLPT_H
+#define GLSLPT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PT_OK 0
+#define PT_ERR_INVALID_ARG (-1)
+#define PT_ERR_HIP (-2)
+#define PT_ERR_STATE (-3)
+#define PT_ERR_OVERFLOW (-4)
+#define PT_ERR_IO (-5)
+#define PT_ERR_UNSUPPORTED (-6)
+#define PT_ERR_NO_DEVICE (-7)
+
+/* ------------------------------------------------------------------ records */
+
+/* 32 floats = 8 x vec4, the GPU material record (src/core/Material.h:47-120). */
+typedef struct pt_material {
+    float baseColor[3]; float anisotropic;
+    float emission[3]; float padding1;
+    float metallic, roughness, subsurface, specularTint;
+    float sheen, sheenTint, clearcoat, clearcoatGloss;
+    float specTrans, ior, mediumType, mediumDensity;
+    float mediumColor[3]; float mediumAnisotropy;
+    float baseColorTexId, metallicRoughnessTexID, normalmapTexID, emissionmapTexID;
+    float opacity, alphaMode, alphaCutoff, padding2;
+} pt_material;
+
+/* 15 floats = 5 x vec3 (src/core/Scene.h:50-59). type: 0 quad, 1 sphere, 2 distant. */
+typedef struct pt_light {
+    float position[3]; float emission[3]; float u[3]; float v[3];
+    float radius, area, type;
+} pt_light;
+
+/* Camera uniforms read by the ray generator (src/shaders/common/globals.glsl:83-92);
+ * fov is horizontal, in radians. */
+typedef struct pt_camera {
+    float position[3]; float up[3]; float right[3]; float forward[3];
+    float fov, focalDist, aperture;
+} pt_camera;
+
+/* Flattened BVH node (thirdparty/RadeonRays/bvh_translator.h:45-50).
+ * LRLeaf: internal (left, right, 0); BLAS leaf (first vertIndices row, count, 1);
+ * TLAS leaf (BLAS root node, material id, -instance-1). */
+typedef struct pt_bvh_node { float bboxmin[3]; float bboxmax[3]; float LRLeaf[3]; } pt_bvh_node;
+
+/* Exactly the arrays Renderer::InitGPUDataBuffers uploads
+ * (src/core/Renderer.cpp:135-249) plus the uniforms derived from the scene. */
+typedef struct pt_scene_view {
+    const pt_bvh_node* nodes; int32_t num_nodes; int32_t top_level_index;
+    const int32_t* vert_indices; int32_t num_tris;           /* 3 ints per row */
+    const float* vertices_uvx; const float* normals_uvy;       /* 4 floats per vertex */
+    int32_t num_vertices;
+    const pt_material* materials; int32_t num_materials;
+    const float* transforms; int32_t num_instances;            /* Mat4 data[4][4], column per data[i] */
+    const pt_light* lights; int32_t num_lights;
+    const uint8_t* texture_maps; int32_t tex_width, tex_height, num_textures; /* RGBA8 layers */
+    const float* env_img; const float* env_cdf;                /* RGB32F image, R32F running-sum CDF */
+    int32_t env_width, env_height; float env_total_sum;
+} pt_scene_view;
+
+/* ------------------------------------------------------------ scene builder */
+
+typedef struct pts_scene_s* pts_scene;
+
+int pts_create(pts_scene* out);
+int pts_destroy(pts_scene s);
+const char* pts_last_error(pts_scene s);
+/* Scene::AddMesh(filename) — OBJ triangle soup (src/core/Scene.cpp:62-83). */
+int pts_add_mesh_obj(pts_scene s, const char* filename, int32_t* id);
+/* Scene::AddMesh from arrays: positions[9n], normals[9n] or NULL, uvs[6n] or NULL. */
+int pts_add_mesh(pts_scene s, const float* positions, const float* normals, const float* uvs,
+                 int32_t num_tris, const char* name, int32_t* id);
+int pts_add_material(pts_scene s, const pt_material* m, int32_t* id);      /* Scene.cpp:109-114 */
+/* transform: 16 floats, Mat4 data[4][4] (column per 4 floats) (Scene.cpp:134-139). */
+int pts_add_mesh_instance(pts_scene s, int32_t mesh_id, const float* transform, int32_t material_id,
+                          int32_t* id);
+int pts_add_light(pts_scene s, const pt_light* l, int32_t* id);            /* Scene.cpp:141-146 */
+int pts_add_texture(pts_scene s, const uint8_t* rgba, int32_t width, int32_t height, const char* name,
+                    int32_t* id);                                          /* Scene.cpp:85-107 */
+int pts_add_camera(pts_scene s, const float eye[3], const float lookat[3], float fov_degrees); /* :56-60 */
+int pts_add_env_map(pts_scene s, const float* rgb, int32_t width, int32_t height);            /* :116-132 */
+int pts_set_texture_array_size(pts_scene s, int32_t width, int32_t height);
+int pts_process(pts_scene s);                                              /* Scene.cpp:216-297 */
+int pts_rebuild_instances(pts_scene s);                                    /* Scene.cpp:200-214 */
+int pts_set_instance_transform(pts_scene s, int32_t instance, const float* transform);
+int pts_get_view(pts_scene s, pt_scene_view* out);  /* pointers valid until the next mutation */
+int pts_get_camera(pts_scene s, pt_camera* out);
+int pts_camera_offset_orientation(pts_scene s, float dx, float dy);       /* Camera.cpp:152-157 */
+int pts_get_blas_root(pts_scene s, int32_t mesh_id, int32_t* root);
+
+/* ------------------------------------------------------------------ renderer */
+
+#define PT_MODE_WAVEFRONT 0
+#define PT_MODE_PLAIN 1
+
+/* RenderOptions subset read by the path (src/core/Renderer.h:37-100) plus the
+ * backend's launch configuration. */
+typedef struct pt_options {
+    int32_t width, height;                 /* renderResolution */
+    int32_t max_depth;                     /* maxDepth */
+    int32_t enable_rr, rr_depth;           /* enableRR, RRDepth */
+    int32_t enable_uniform_light; float uniform_light_col[3];
+    int32_t enable_envmap; float envmap_intensity; float envmap_rot;   /* rot in degrees */
+    int32_t hide_emitters;
+    int32_t enable_background, transparent_background;
+    int32_t enable_roughness_mollification; float roughness_mollification_amt;
+    int32_t enable_volume_mis;
+    int32_t opengl_normal_map;
+    /* backend launch configuration (no reference counterpart) */
+    int32_t mode;            /* PT_MODE_WAVEFRONT or PT_MODE_PLAIN */
+    int32_t block_size;      /* 64, 128 or 256 */
+    int32_t stack_depth;     /* LDS traversal-stack entries per thread: 16, 24 or 32 */
+    int32_t wave_spp;        /* samples per wavefront pass */
+    int32_t use_graph;       /* capture the bounce stages of a pass in a HIP graph */
+    int32_t count_work;      /* instrumented kernels: count nodes / triangles / instances */
+} pt_options;
+
+typedef struct pt_stats {
+    double ms_last_render;           /* device time of the last pt_render / pt_batch_render */
+    double ms_stage[8];              /* per-stage device time accumulated since pt_reset_stats */
+    int64_t launches_stage[8];
+    int64_t rays_closest, rays_shadow;      /* rays issued (count_work only) */
+    int64_t nodes_visited, tris_tested, instances_entered, lights_tested; /* count_work only */
+    int64_t stack_spills;            /* traversal-stack entries that went to the global backing store */
+    int32_t stack_overflow;          /* 1 if a traversal exceeded the maximum stack depth */
+} pt_stats;
+
+/* stage indices of pt_stats.ms_stage */
+#define PT_STAGE_RAYGEN 0
+#define PT_STAGE_EXTEND 1
+#define PT_STAGE_SHADE 2
+#define PT_STAGE_SHADOW 3
+#define PT_STAGE_ACCUM 4
+#define PT_STAGE_PLAIN 5
+#define PT_STAGE_ADJOINT 6
+
+typedef struct pt_renderer_s* pt_handle;
+
+void pt_default_options(pt_options* o);
+int pt_device_count(int32_t* n);
+int pt_create(int32_t device, pt_handle* out);                 /* Renderer::Renderer (Renderer.cpp:41-84) */
+int pt_destroy(pt_handle h);                                   /* Renderer::~Renderer (:86-133) */
+const char* pt_last_error(pt_handle h);
+int pt_upload_scene(pt_handle h, const pt_scene_view* v);      /* InitGPUDataBuffers (:135-249) */
+/* Instance edit refresh: transforms, materials and the TLAS node range
+ * [first_node, first_node + num_nodes) (Renderer.cpp:649-665). */
+int pt_update_instances(pt_handle h, const float* transforms, int32_t num_instances,
+                        const pt_material* materials, int32_t num_materials,
+                        const pt_bvh_node* nodes, int32_t first_node, int32_t num_nodes);
+int pt_set_options(pt_handle h, const pt_options* o);          /* InitShaders defines + uniforms */
+int pt_set_camera(pt_handle h, const pt_camera* c);            /* camera uniforms (:764-771) */
+int pt_clear_accum(pt_handle h);                               /* accumFBO clear (:733-744) */
+/* Adds samples [first_sample, first_sample + num_samples) of every pixel to the
+ * RGBA32F accumulation buffer; sample s uses RNG seed (x, y, s + 2, x + y). */
+int pt_render(pt_handle h, int32_t first_sample, int32_t num_samples);
+int pt_synchronize(pt_handle h);
+int pt_read_accum(pt_handle h, float* rgba, size_t num_floats); /* GetOutputBuffer analogue */
+/* Tonemapped, gamma-corrected RGBA8, rows bottom-up like the GL texture
+ * (tonemap.glsl:97-133, Renderer.cpp:619-634). */
+int pt_read_ldr(pt_handle h, int32_t num_samples, int32_t enable_tonemap, int32_t enable_aces,
+                int32_t simple_aces_fit, const float background[3], uint8_t* rgba, size_t num_bytes);
+int pt_get_stats(pt_handle h, pt_stats* out);
+int pt_reset_stats(pt_handle h);
+
+/* Ray queries against the uploaded scene (ClosestHit closest_hit.glsl:25-266,
+ * AnyHit anyhit.glsl:25-215), host arrays in and out.
+ * closest: t (INF on miss), instance (-1 if none), triangle = vertIndices row
+ * (-1 if none), light index (-1 if none), barycentrics (u, v). */
+int pt_closest_hit(pt_handle h, const float* origins, const float* dirs, int32_t n,
+                   float* t, int32_t* instance, int32_t* triangle, int32_t* light, float* uv);
+int pt_any_hit(pt_handle h, const float* origins, const float* dirs, const float* max_dist,
+               int32_t n, int32_t* occluded);
+
+/* ---- batched uneven jobs: one launch sequence over many render jobs ---- */
+typedef struct pt_job {
+    pt_camera camera;
+    int32_t width, height, spp, max_depth;
+    float* accum_out;        /* host RGBA32F buffer, width*height*4 floats, written (not added) */
+} pt_job;
+int pt_batch_render(pt_handle h, const pt_job* jobs, int32_t num_jobs);
+
+/* ---- adjoint: gradient of an image loss w.r.t. material parameters ---- */
+/* Parameters per material, in this order: baseColor.r, .g, .b, roughness,
+ * metallic, ior, clearcoat (7 floats). dL_dimage is RGB float per pixel
+ * (width*height*3, rows bottom-up) for the same samples as pt_render.
+ * d_params receives num_materials*7 floats (added to, not overwritten). */
+#define PT_ADJOINT_PARAMS 7
+int pt_adjoint(pt_handle h, int32_t first_sample, int32_t num_samples, const float* dL_dimage,
+               float* d_params);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GLSLPT_H */

@@ -1,0 +1,110 @@
+// Test-infrastructure harness around the reference's own host C++ (Scene,
+// Mesh, Camera, RadeonRays Bvh/SplitBvh/BvhTranslator), compiled from the
+// reference sources by oracle/Makefile into oracle/_ref/libref_scene.so.
+// It is a checker: the tests call it to obtain the reference's flattened
+// arrays for given triangle soups and instances, and compare the product's
+// scene builder against them bit for bit. Nothing in the product links it.
+//
+// Reference entry points exercised:
+//   Scene::AddMaterial/AddMeshInstance/AddCamera/ProcessScene  src/core/Scene.cpp:56-297
+//   Mesh::LoadFromFile (tinyobj)                               src/core/Mesh.cpp:44-119
+//   Camera basis                                               src/core/Camera.cpp:120-188
+#include <cstring>
+#include <vector>
+
+#include "Camera.h"
+#include "Mesh.h"
+#include "Scene.h"
+
+using namespace GLSLPT;
+
+namespace {
+struct Built {
+    Scene* scene = nullptr;
+};
+Built g_last;
+}  // namespace
+
+extern "C" {
+
+// Loads an OBJ with the reference loader and returns its triangle soup.
+// Call with out arrays NULL to get the vertex count, then again to fill.
+int ref_load_obj(const char* filename, int* num_vertices, float* verts_uvx, float* normals_uvy) {
+    Mesh mesh;
+    if (!mesh.LoadFromFile(filename)) return -1;
+    *num_vertices = static_cast<int>(mesh.verticesUVX.size());
+    if (verts_uvx) std::memcpy(verts_uvx, mesh.verticesUVX.data(), mesh.verticesUVX.size() * sizeof(Vec4));
+    if (normals_uvy) std::memcpy(normals_uvy, mesh.normalsUVY.data(), mesh.normalsUVY.size() * sizeof(Vec4));
+    return 0;
+}
+
+// Builds a scene with the reference code. Meshes are given as concatenated
+// Vec4 arrays (tri_counts[i] triangles each); instances by mesh id, material
+// id and a 16-float Mat4. Keeps the scene for ref_get_* until the next call.
+int ref_build_scene(int num_meshes, const int* tri_counts, const float* verts_uvx, const float* normals_uvy,
+                    int num_materials, int num_instances, const int* mesh_ids, const int* mat_ids,
+                    const float* transforms, const float* camera_eye_lookat_fov) {
+    delete g_last.scene;
+    g_last.scene = new Scene();
+    Scene* s = g_last.scene;
+    size_t off = 0;
+    for (int i = 0; i < num_meshes; ++i) {
+        Mesh* m = new Mesh;
+        m->name = "mesh" + std::to_string(i);
+        const size_t nv = 3 * static_cast<size_t>(tri_counts[i]);
+        m->verticesUVX.resize(nv);
+        m->normalsUVY.resize(nv);
+        std::memcpy(m->verticesUVX.data(), verts_uvx + 4 * off, nv * sizeof(Vec4));
+        std::memcpy(m->normalsUVY.data(), normals_uvy + 4 * off, nv * sizeof(Vec4));
+        off += nv;
+        s->meshes.push_back(m);
+    }
+    for (int i = 0; i < num_materials; ++i) s->AddMaterial(Material());
+    for (int i = 0; i < num_instances; ++i) {
+        Mat4 xf;
+        std::memcpy(xf.data, transforms + 16 * i, sizeof(xf.data));
+        s->AddMeshInstance(MeshInstance("inst", mesh_ids[i], xf, mat_ids[i]));
+    }
+    if (camera_eye_lookat_fov) {
+        const float* c = camera_eye_lookat_fov;
+        s->AddCamera(Vec3(c[0], c[1], c[2]), Vec3(c[3], c[4], c[5]), c[6]);
+    }
+    s->ProcessScene();
+    return 0;
+}
+
+// Sizes of the last build: nodes, topLevelIndex, vertIndices rows, vertices.
+int ref_get_sizes(int* num_nodes, int* top_level_index, int* num_tris, int* num_vertices) {
+    Scene* s = g_last.scene;
+    if (!s) return -1;
+    *num_nodes = static_cast<int>(s->bvhTranslator.nodes.size());
+    *top_level_index = s->bvhTranslator.topLevelIndex;
+    *num_tris = static_cast<int>(s->vertIndices.size());
+    *num_vertices = static_cast<int>(s->verticesUVX.size());
+    return 0;
+}
+
+// Copies the flattened arrays and the camera basis of the last build.
+// camera: position, up, right, forward (12 floats) + fov, focalDist, aperture.
+int ref_get_arrays(float* nodes, int* vert_indices, float* verts_uvx, float* normals_uvy, float* camera15) {
+    Scene* s = g_last.scene;
+    if (!s) return -1;
+    std::memcpy(nodes, s->bvhTranslator.nodes.data(), s->bvhTranslator.nodes.size() * 36);
+    std::memcpy(vert_indices, s->vertIndices.data(), s->vertIndices.size() * 12);
+    std::memcpy(verts_uvx, s->verticesUVX.data(), s->verticesUVX.size() * 16);
+    std::memcpy(normals_uvy, s->normalsUVY.data(), s->normalsUVY.size() * 16);
+    if (camera15 && s->camera) {
+        const Camera* c = s->camera;
+        const Vec3* v[4] = {&c->position, &c->up, &c->right, &c->forward};
+        for (int i = 0; i < 4; ++i) { camera15[3 * i] = v[i]->x; camera15[3 * i + 1] = v[i]->y; camera15[3 * i + 2] = v[i]->z; }
+        camera15[12] = c->fov; camera15[13] = c->focalDist; camera15[14] = c->aperture;
+    }
+    return 0;
+}
+
+void ref_free(void) {
+    delete g_last.scene;
+    g_last.scene = nullptr;
+}
+
+}  // extern "C"

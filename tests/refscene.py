@@ -1,0 +1,75 @@
+"""Loader for oracle/_ref/libref_scene.so: the reference's own host C++
+(Scene, Mesh, Camera, RadeonRays BVH + translator) compiled from its sources.
+Test infrastructure only: it produces the expected flattened arrays.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+PATH = os.path.join(ROOT, "oracle", "_ref", "libref_scene.so")
+P = C.POINTER
+
+
+def available():
+    return os.path.exists(PATH)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        l = C.CDLL(PATH)
+        l.ref_load_obj.argtypes = [C.c_char_p, P(C.c_int), P(C.c_float), P(C.c_float)]
+        l.ref_build_scene.argtypes = [C.c_int, P(C.c_int), P(C.c_float), P(C.c_float), C.c_int, C.c_int,
+                                      P(C.c_int), P(C.c_int), P(C.c_float), P(C.c_float)]
+        l.ref_get_sizes.argtypes = [P(C.c_int)] * 4
+        l.ref_get_arrays.argtypes = [P(C.c_float), P(C.c_int), P(C.c_float), P(C.c_float), P(C.c_float)]
+        _lib = l
+    return _lib
+
+
+def fp(a):
+    return a.ctypes.data_as(P(C.c_float))
+
+
+def ip(a):
+    return a.ctypes.data_as(P(C.c_int))
+
+
+def load_obj(filename):
+    """Reference OBJ loader: (verticesUVX[n,4], normalsUVY[n,4])."""
+    n = C.c_int()
+    if lib().ref_load_obj(filename.encode(), C.byref(n), None, None) != 0:
+        raise IOError(filename)
+    v = np.zeros((n.value, 4), np.float32)
+    nn = np.zeros((n.value, 4), np.float32)
+    lib().ref_load_obj(filename.encode(), C.byref(n), fp(v), fp(nn))
+    return v, nn
+
+
+def build(meshes, instances, num_materials, camera=None):
+    """meshes: list of (verticesUVX[3n,4], normalsUVY[3n,4]); instances: list of
+    (mesh_id, mat_id, transform[4,4] as Mat4 data). Returns the reference's arrays."""
+    counts = np.array([m[0].shape[0] // 3 for m in meshes], np.int32)
+    v = np.ascontiguousarray(np.concatenate([m[0] for m in meshes]), np.float32)
+    n = np.ascontiguousarray(np.concatenate([m[1] for m in meshes]), np.float32)
+    mids = np.array([i[0] for i in instances], np.int32)
+    mats = np.array([i[1] for i in instances], np.int32)
+    xf = np.ascontiguousarray(np.stack([np.asarray(i[2], np.float32).reshape(16) for i in instances]), np.float32)
+    cam = None if camera is None else np.asarray(camera, np.float32)
+    lib().ref_build_scene(len(meshes), ip(counts), fp(v), fp(n), num_materials, len(instances), ip(mids),
+                          ip(mats), fp(xf), None if cam is None else fp(cam))
+    s = [C.c_int() for _ in range(4)]
+    lib().ref_get_sizes(*[C.byref(x) for x in s])
+    nn, tli, nt, nv = [x.value for x in s]
+    nodes = np.zeros((nn, 9), np.float32)
+    vi = np.zeros((nt, 3), np.int32)
+    vu = np.zeros((nv, 4), np.float32)
+    nu = np.zeros((nv, 4), np.float32)
+    camo = np.zeros(15, np.float32)
+    lib().ref_get_arrays(fp(nodes), ip(vi), fp(vu), fp(nu), fp(camo))
+    return dict(nodes=nodes, top_level_index=tli, vert_indices=vi, vertices_uvx=vu, normals_uvy=nu, camera=camo)

@@ -1,0 +1,134 @@
+"""GPU parity: HIP kernels (through the C ABI) against the CPU oracle on the
+same seeded inputs.
+
+Tolerances (stated by the workload spec): closest-hit primitive and instance
+ids identical on >= 99.99% of rays (ties aside), hit t within 1e-5 relative;
+per-pixel radiance within 1e-4 relative on almost every pixel (a path whose
+Russian-roulette or lobe draw lands within an ulp of its threshold follows
+another branch; such pixels are counted and bounded), image means within
+1e-4 relative.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from common import camera_rays, oracle_setup, random_dirs, rel_err, small_instanced
+from glslpt import scenes
+from glslpt._capi import PT_MODE_PLAIN, PT_MODE_WAVEFRONT
+from glslpt.renderer import Renderer, options_for
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def renderer():
+    r = Renderer(0)
+    yield r
+    r.close()
+
+
+def _hit_parity(gpu, ref, label):
+    n = ref["t"].shape[0]
+    same = (gpu["triangle"] == ref["triangle"]) & (gpu["instance"] == ref["instance"]) & (gpu["light"] == ref["light"])
+    frac = same.mean()
+    both = same & (ref["t"] < 1e6)
+    terr = rel_err(gpu["t"][both], ref["t"][both]).max() if both.any() else 0.0
+    print(f"{label}: rays={n} ids-identical={frac:.6f} hits={int((ref['t'] < 1e6).sum())} max-rel-t={terr:.3g}")
+    assert frac >= 0.9999, f"{label}: only {frac:.6f} of ids identical"
+    assert terr <= 1e-5, f"{label}: t rel err {terr}"
+
+
+def test_closest_hit_cornell(renderer):
+    sc, rs = scenes.cornell()
+    renderer.upload_scene(sc)
+    sa, cam, o = oracle_setup(sc, rs)
+    rng = np.random.default_rng(11)
+    ro, rd = camera_rays(sc.camera(), rs.width, rs.height, rng, 200000)
+    # plus rays from inside the box in all directions
+    ri = rng.uniform([0.05, 0.05, 0.05], [0.5, 0.5, 0.5], (100000, 3)).astype(np.float32)
+    di = random_dirs(rng, 100000)
+    ro = np.concatenate([ro, ri]); rd = np.concatenate([rd, di])
+    renderer.set_options(options_for(rs))
+    g = renderer.closest_hit(ro, rd)
+    r = oracle.closest_hit(sa, o, ro, rd)
+    _hit_parity(g, r, "cornell")
+
+
+def test_closest_hit_instanced(renderer):
+    sc, rs = small_instanced()
+    renderer.upload_scene(sc)
+    sa, cam, o = oracle_setup(sc, rs)
+    rng = np.random.default_rng(12)
+    ro, rd = camera_rays(sc.camera(), rs.width, rs.height, rng, 200000)
+    ri = rng.uniform(-6, 6, (100000, 3)).astype(np.float32)
+    di = random_dirs(rng, 100000)
+    ro = np.concatenate([ro, ri]); rd = np.concatenate([rd, di])
+    renderer.set_options(options_for(rs))
+    g = renderer.closest_hit(ro, rd)
+    r = oracle.closest_hit(sa, o, ro, rd)
+    _hit_parity(g, r, "instanced")
+
+
+def test_any_hit_instanced(renderer):
+    sc, rs = small_instanced()
+    renderer.upload_scene(sc)
+    sa, cam, o = oracle_setup(sc, rs)
+    rng = np.random.default_rng(13)
+    ro = rng.uniform(-6, 6, (200000, 3)).astype(np.float32)
+    rd = random_dirs(rng, 200000)
+    md = rng.uniform(0.1, 10, 200000).astype(np.float32)
+    renderer.set_options(options_for(rs))
+    g = renderer.any_hit(ro, rd, md)
+    r = oracle.any_hit(sa, o, ro, rd, md)
+    agree = (g == r).mean()
+    print(f"any-hit agreement {agree:.6f}, occluded fraction {r.mean():.3f}")
+    assert agree >= 0.9999
+
+
+def _render_parity(renderer, sc, rs, mode, spp, label, px_tol=0.995):
+    renderer.upload_scene(sc)
+    renderer.set_options(options_for(rs, mode=mode, wave_spp=2))
+    renderer.set_camera(sc.camera())
+    renderer.clear()
+    renderer.render(0, spp)
+    renderer.synchronize()
+    g = renderer.read_accum()
+    sa, cam, o = oracle_setup(sc, rs)
+    r = oracle.render(sa, cam, o, 0, spp)
+    e = rel_err(g[..., :3], r[..., :3], floor=1e-4).max(axis=2)
+    within = (e <= 1e-4).mean()
+    mg, mr = g[..., :3].mean(), r[..., :3].mean()
+    print(f"{label}: pixels within 1e-4 = {within:.5f}, worst {e.max():.3g}, mean gpu {mg:.6f} oracle {mr:.6f}, "
+          f"alpha {g[..., 3].mean():.3f}/{r[..., 3].mean():.3f}")
+    assert np.isfinite(g).all()
+    assert within >= px_tol, f"{label}: {within}"
+    assert abs(mg - mr) / mr < 1e-2
+    return g, r
+
+
+@pytest.mark.parametrize("mode", [PT_MODE_PLAIN, PT_MODE_WAVEFRONT])
+def test_render_cornell(renderer, mode):
+    sc, rs = scenes.cornell()
+    _render_parity(renderer, sc, rs, mode, rs.spp, f"cornell mode={mode}")
+
+
+@pytest.mark.parametrize("mode", [PT_MODE_PLAIN, PT_MODE_WAVEFRONT])
+def test_render_instanced(renderer, mode):
+    sc, rs = small_instanced()
+    _render_parity(renderer, sc, rs, mode, 4, f"instanced mode={mode}")
+
+
+def test_plain_equals_wavefront(renderer):
+    sc, rs = small_instanced()
+    out = []
+    for mode in (PT_MODE_PLAIN, PT_MODE_WAVEFRONT):
+        renderer.upload_scene(sc)
+        renderer.set_options(options_for(rs, mode=mode, wave_spp=3))
+        renderer.set_camera(sc.camera())
+        renderer.clear()
+        renderer.render(0, 6)
+        renderer.synchronize()
+        out.append(renderer.read_accum())
+    d = np.abs(out[0] - out[1]).max()
+    print("plain vs wavefront max abs diff", d)
+    assert rel_err(out[0], out[1], floor=1e-3).max() <= 1e-5
