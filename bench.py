@@ -1,0 +1,200 @@
+"""Benchmark of the path-tracing hot path on one MI355X.
+
+Workload (the headline configuration): 64 instances (random rigid transforms,
+uniform scale 0.5-2) of a seeded geodesic icosphere with 15,680 triangles
+displaced by 3-octave value noise (1,003,520 instanced triangles), 16 Disney
+materials drawn from a seed, 2 rect lights + 1 sphere light + a constant
+environment, 1920x1080, 64 spp, max depth 8, Russian roulette from depth 3.
+
+A step renders all 64 samples of the frame. The reported value is the
+wavefront renderer's throughput in Msamples/s (W*H*spp / step time); the plain
+one-kernel-per-frame renderer is timed on the same frame beside it. The scene
+is resident in HBM before the timed region starts.
+
+roofline: the dominant kernel is the closest-hit (extend) stage. Its
+algorithmic bytes per launch come from an instrumented run of the same frame
+(count of internal nodes visited, triangles tested, instances entered, light
+records read, rays): 64 B per node (two child boxes + references), 36 B per
+triangle (three positions), 52 B per instance (inverse affine + BLAS root),
+60 B per light record, 56 B per ray (queue entry, ray, depth, hit record).
+achieved = bytes per launch / average launch time (HIP events on the
+renderer's stream over the timed region); peak = 8 TB/s HBM.
+
+cpu_baseline: the CPU oracle (C++ restatement of the reference shaders,
+OpenMP) on the host cores, rendering a bounded sample of the same frame
+(every k-th row at 1 spp, k chosen for ~15 s of CPU work).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "glsl-pathtracer_amd"))
+
+METRIC = "Msamples/s at 1080p depth 8 (batched vs plain kernel); backward/forward time ratio"
+HBM_PEAK_GBS = 8000.0
+B_NODE, B_TRI, B_INST, B_LIGHT, B_RAY = 64, 36, 52, 60, 56
+
+
+def build_scene(args):
+    from glslpt import scenes
+    return scenes.instanced_spheres(num_instances=64, freq=28, seed=0x5EED0002, width=args.width,
+                                    height=args.height, spp=args.spp, max_depth=8)
+
+
+def run_mode(r, sc, rs, mode, steps, warmup, spp, opts_kw):
+    from glslpt.renderer import options_for
+    r.set_options(options_for(rs, mode=mode, **opts_kw))
+    r.set_camera(sc.camera())
+    r.clear()
+    s0 = 0
+    for _ in range(warmup):
+        r.render(s0, spp)
+        s0 += spp
+    r.synchronize()
+    r.reset_stats()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        r.render(s0, spp)
+        s0 += spp
+    r.synchronize()
+    t1 = time.perf_counter()
+    st = r.stats()
+    return (t1 - t0) / steps * 1e3, st
+
+
+def count_work(r, sc, rs, spp, opts_kw):
+    from glslpt._capi import PT_MODE_WAVEFRONT
+    from glslpt.renderer import options_for
+    r.set_options(options_for(rs, mode=PT_MODE_WAVEFRONT, count_work=1, **opts_kw))
+    r.set_camera(sc.camera())
+    r.clear()
+    r.reset_stats()
+    r.render(0, spp)
+    r.synchronize()
+    return r.stats()
+
+
+def cpu_baseline(sc, rs, target_s):
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle
+    from common import oracle_setup
+    sa, cam, o = oracle_setup(sc, rs)
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+    # probe: every 8th row at 1 spp; then size the sample (rows x spp) to ~target_s
+    step = 8
+    rows = len(range(0, rs.height, step))
+    t0 = time.perf_counter()
+    oracle.render(sa, cam, o, 0, 1, rows=(0, rs.height, step), threads=threads)
+    per_sample = (time.perf_counter() - t0) / (rows * rs.width)
+    want = target_s / max(per_sample, 1e-9)
+    if want < rows * rs.width:
+        step = max(1, int(rs.height * rs.width / want))
+        spp = 1
+    else:
+        step = 1
+        spp = max(1, int(want / (rs.height * rs.width)))
+    rows = len(range(0, rs.height, step))
+    t0 = time.perf_counter()
+    oracle.render(sa, cam, o, 1, spp, rows=(0, rs.height, step), threads=threads)
+    dt = time.perf_counter() - t0
+    samples = rows * rs.width * spp
+    return {"value": samples / dt / 1e6, "unit": "Msamples/s", "cores": threads, "kind": "port",
+            "sample": f"{rows} rows (every {step}th) x {rs.width} px x {spp} spp of the same frame, {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--spp", type=int, default=64)
+    ap.add_argument("--width", type=int, default=1920)
+    ap.add_argument("--height", type=int, default=1080)
+    ap.add_argument("--plain-steps", type=int, default=None)
+    ap.add_argument("--wave-spp", type=int, default=4)
+    ap.add_argument("--block-size", type=int, default=256)
+    ap.add_argument("--stack-depth", type=int, default=24)
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-count", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported (the path shards by sample range "
+              "across GPUs, which this build does not launch)", file=sys.stderr)
+        sys.exit(2)
+
+    from glslpt._capi import PT_MODE_PLAIN, PT_MODE_WAVEFRONT
+    from glslpt.renderer import Renderer
+
+    sc, rs = build_scene(args)
+    a = sc.arrays()
+    ntri_inst = 0
+    r = Renderer(0)
+    r.upload_scene(sc)
+    opts_kw = dict(block_size=args.block_size, stack_depth=args.stack_depth, wave_spp=args.wave_spp)
+    samples_per_step = rs.width * rs.height * args.spp
+
+    ms_w, st_w = run_mode(r, sc, rs, PT_MODE_WAVEFRONT, args.steps, args.warmup, args.spp, opts_kw)
+    value = samples_per_step / (ms_w / 1e3) / 1e6
+
+    plain = None
+    if not args.no_plain:
+        ps = args.plain_steps if args.plain_steps is not None else max(1, args.steps)
+        ms_p, st_p = run_mode(r, sc, rs, PT_MODE_PLAIN, ps, min(1, args.warmup), args.spp, opts_kw)
+        plain = {"value": samples_per_step / (ms_p / 1e3) / 1e6, "ms_per_step": ms_p,
+                 "kernel_ms_per_step": st_p["ms_stage"]["plain"] / ps}
+
+    roofline = None
+    extend_ms = st_w["ms_stage"]["extend"]
+    extend_launches = max(1, st_w["launches_stage"]["extend"])
+    if not args.no_count:
+        cspp = min(args.spp, 4)
+        cw = count_work(r, sc, rs, cspp, opts_kw)
+        scale = args.spp * args.steps / cspp
+        bytes_total = scale * (B_NODE * cw["nodes_visited"] + B_TRI * cw["tris_tested"] +
+                               B_INST * cw["instances_entered"] + B_LIGHT * cw["lights_tested"] +
+                               B_RAY * cw["rays_closest"])
+        per_launch = bytes_total / extend_launches
+        avg_ms = extend_ms / extend_launches
+        achieved = per_launch / (avg_ms / 1e3) / 1e9
+        roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                    "kernel": "k_extend (closest hit)", "bytes_per_launch": per_launch,
+                    "avg_launch_ms": avg_ms, "launches": extend_launches,
+                    "per_ray": {"nodes": cw["nodes_visited"] / max(1, cw["rays_closest"]),
+                                "tris": cw["tris_tested"] / max(1, cw["rays_closest"]),
+                                "instances": cw["instances_entered"] / max(1, cw["rays_closest"])},
+                    "rays_per_sample": cw["rays_closest"] / (rs.width * rs.height * cspp),
+                    "shadow_rays_per_sample": cw["rays_shadow"] / (rs.width * rs.height * cspp),
+                    "stack_spills": cw["stack_spills"]}
+    r.close()
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(sc, rs, args.cpu_seconds)
+
+    out = {
+        "metric": METRIC, "value": value, "unit": "Msamples/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_w, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "64 instances of a 15680-triangle noise-displaced icosphere (1003520 instanced "
+                               "triangles), 16 Disney materials, 2 rect + 1 sphere light + constant environment, "
+                               f"{rs.width}x{rs.height}, {args.spp} spp, max depth 8, RR from depth 3",
+                   "renderer": "wavefront", "wave_spp": args.wave_spp, "block_size": args.block_size,
+                   "stack_depth": args.stack_depth, "samples_per_step": samples_per_step,
+                   "unique_triangles": int(a["vert_indices"].shape[0]), "instances": int(a["transforms"].shape[0])},
+        "stage_ms_per_step": {k: v / args.steps for k, v in st_w["ms_stage"].items() if v > 0},
+        "plain": plain,
+        "wavefront_over_plain": (value / plain["value"]) if plain else None,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -670,7 +670,7 @@ struct Hit {
     int light;  // light index when a light is the closest hit, else -1
 };
 
-struct TravCount { uint32_t nodes, tris, insts, lights, spills; };
+struct TravCount { uint32_t nodes, tris, insts, lights, spills, rays; };
 
 // Traversal stack: entries [0, S) in LDS (entry i of lane tid at lds[i*bs + tid],
 // conflict-free), deeper entries in a global backing store indexed by the
@@ -722,6 +722,7 @@ __device__ __forceinline__ float slab(float mnx, float mny, float mnz, float mxx
 template <bool ANY, bool COUNT>
 __device__ bool traverse(const DevScene& sc, vec3 O, vec3 D, float tmax, Stack& stk, Hit& h, TravCount& cnt,
                          bool& overflow) {
+    if (COUNT) cnt.rays++;
     const vec3 invD = V(1.0f / D.x, 1.0f / D.y, 1.0f / D.z);
     vec3 o = O, d = D, inv = invD;
     float best = tmax;
@@ -819,8 +820,10 @@ __device__ __forceinline__ void hit_lights(const DevScene& sc, vec3 ro, vec3 rd,
 }
 
 // Light loop of AnyHit (anyhit.glsl:28-63).
-__device__ __forceinline__ bool any_light(const DevScene& sc, vec3 ro, vec3 rd, float maxDist) {
+template <bool COUNT>
+__device__ __forceinline__ bool any_light(const DevScene& sc, vec3 ro, vec3 rd, float maxDist, TravCount& cnt) {
     for (int i = 0; i < sc.num_lights; i++) {
+        if (COUNT) cnt.lights++;
         const GLight& L = sc.lights[i];
         float type = L.u.w;
         if (type == static_cast<float>(QUAD_LIGHT)) {

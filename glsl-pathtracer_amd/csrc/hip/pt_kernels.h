@@ -36,14 +36,13 @@ int persistent_blocks(int kind, int bs, int S, bool count, int num_cu);
 void launch_plain(hipStream_t s, int blocks, int bs, const ptd::DevScene& sc, const ptd::DevOpts& op, float4* accum,
                   int sample, int num_items, int* work, int S, int* spill, unsigned long long* counters, int* ovf,
                   bool count);
-void launch_raygen(hipStream_t s, const ptd::DevOpts& op, const ptd::PathBuf& pb, int npix, int first_sample,
-                   int num_paths, int* queue, int* counts);
 void launch_extend(hipStream_t s, int blocks, int bs, const ptd::DevScene& sc, const ptd::DevOpts& op,
-                   const ptd::PathBuf& pb, const int* queue, const int* count_in, int* work, int S, int* spill,
+                   const ptd::PathBuf& pb, bool primary, const int* qin, const int* count_in, int npaths, int npix,
+                   int first_sample, int* hitq, int* hit_count, int* work, int S, int* spill,
                    unsigned long long* counters, int* ovf, bool count);
 void launch_shade(hipStream_t s, int blocks, int bs, const ptd::DevScene& sc, const ptd::DevOpts& op,
-                  const ptd::PathBuf& pb, const ptd::ShadowBuf& sb, const int* qin, const int* count_in, int* qout,
-                  int* count_out, int* shadow_count, int* work);
+                  const ptd::PathBuf& pb, const ptd::ShadowBuf& sb, bool primary, const int* hitq, const int* hit_count,
+                  int npix, int first_sample, int* qout, int* count_out, int* shadow_count, int* work);
 void launch_shadow(hipStream_t s, int blocks, int bs, const ptd::DevScene& sc, const ptd::DevOpts& op,
                    const ptd::PathBuf& pb, const ptd::ShadowBuf& sb, const int* count_in, int* work, int S,
                    int* spill, unsigned long long* counters, int* ovf, bool count);

@@ -13,11 +13,18 @@ namespace ptd {
 // ------------------------------------------------------------------ helpers
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
 
-// One atomic per wave fetches the next 64 work items.
-__device__ __forceinline__ int fetch_batch(int* counter) {
+// Work distribution of the persistent kernels: every wave's first batch of
+// 64 items is static (global wave index * 64); later batches come from one
+// atomic per wave on `counter`, offset past the static range. Short queues
+// (the tail bounces) therefore cost no atomics at all.
+__device__ __forceinline__ int static_total() { return static_cast<int>(gridDim.x * (blockDim.x >> 6)) * 64; }
+__device__ __forceinline__ int first_batch() {
+    return static_cast<int>(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * 64;
+}
+__device__ __forceinline__ int next_batch(int* counter) {
     int base = 0;
     if (lane_id() == 0) base = atomicAdd(counter, 64);
-    return __builtin_amdgcn_readfirstlane(base);
+    return __builtin_amdgcn_readfirstlane(base) + static_total();
 }
 
 // Wave-level stream compaction: one atomic per wave, lanes write at their
@@ -41,6 +48,7 @@ __device__ __forceinline__ void flush_counts(const TravCount& c, unsigned long l
     atomicAdd(out + 2, static_cast<unsigned long long>(c.insts));
     atomicAdd(out + 3, static_cast<unsigned long long>(c.lights));
     atomicAdd(out + 4, static_cast<unsigned long long>(c.spills));
+    atomicAdd(out + 5, static_cast<unsigned long long>(c.rays));
 }
 
 // Pixel of work item `i`: 8x8 pixel tiles, one tile per wave, rows bottom-up.
@@ -82,7 +90,7 @@ __device__ void camera_ray(const DevOpts& op, int px, int py, Rng& rng, vec3& ro
 template <bool COUNT>
 __device__ __forceinline__ bool occluded(const DevScene& sc, const DevOpts& op, vec3 ro, vec3 rd, float maxDist,
                                          Stack& stk, TravCount& cnt, bool& ovf) {
-    if ((op.flags & OPT_LIGHTS) && any_light(sc, ro, rd, maxDist)) return true;
+    if ((op.flags & OPT_LIGHTS) && any_light<COUNT>(sc, ro, rd, maxDist, cnt)) return true;
     Hit h;
     return traverse<true, COUNT>(sc, ro, rd, maxDist, stk, h, cnt, ovf);
 }
@@ -436,11 +444,9 @@ __global__ void __launch_bounds__(256) k_plain(DevScene sc, DevOpts op, float4* 
     Stack stk{lds, spill, static_cast<int>(threadIdx.x), static_cast<int>(blockDim.x), S,
               blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x,
               static_cast<size_t>(gridDim.x) * blockDim.x};
-    TravCount cnt{0, 0, 0, 0, 0};
+    TravCount cnt{0, 0, 0, 0, 0, 0};
     bool ovf = false;
-    for (;;) {
-        const int base = fetch_batch(work);
-        if (base >= num_items) break;
+    for (int base = first_batch(); base < num_items; base = next_batch(work)) {
         const int i = base + lane_id();
         int px, py;
         item_pixel(i, op.width, px, py);
@@ -461,61 +467,11 @@ __global__ void __launch_bounds__(256) k_plain(DevScene sc, DevOpts op, float4* 
 }
 
 // ------------------------------------------------------------------ wavefront
-// Raygen: slot = s_local * npix + pixel; writes the path state and queue.
-__global__ void __launch_bounds__(256) k_raygen(DevOpts op, PathBuf pb, int npix, int first_sample, int num_paths,
-                                                int* queue, int* counts) {
-    const int slot = blockIdx.x * blockDim.x + threadIdx.x;
-    if (slot == 0) counts[0] = num_paths;
-    if (slot >= num_paths) return;
-    const int s = slot / npix;
-    const int item = slot - s * npix;
-    int px, py;
-    item_pixel(item, op.width, px, py);
-    Rng rng = init_rng(static_cast<uint32_t>(px), static_cast<uint32_t>(py), static_cast<uint32_t>(first_sample + s + 2));
-    vec3 ro, rd;
-    if (px < op.width && py < op.height) camera_ray(op, px, py, rng, ro, rd);
-    else { ro = V1(0.f); rd = V(0.f, 0.f, 1.f); }
-    pb.ro[slot] = F4(ro, 0.0f);
-    pb.rd[slot] = F4(rd, 0.0f);
-    pb.thr[slot] = make_float4(1.f, 1.f, 1.f, 0.f);
-    pb.rad[slot] = make_float4(0.f, 0.f, 0.f, 1.f);
-    pb.rng[slot] = make_uint4(rng.x, rng.y, rng.z, rng.w);
-    pb.meta[slot] = make_int4(0, 0, 0, 0);
-    pb.aux[slot] = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (pb.med) pb.med[slot] = make_float4(0.f, 0.f, 0.f, 0.f);
-    // pixels outside the image (partial tiles) never enter the queue
-    queue[slot] = (px < op.width && py < op.height) ? slot : -1;
-}
-
-// Extend: closest hit for every queued path.
-template <bool COUNT>
-__global__ void __launch_bounds__(256) k_extend(DevScene sc, DevOpts op, PathBuf pb, const int* __restrict__ queue,
-                                                const int* count_in, int* work, int S, int* spill,
-                                                unsigned long long* counters, int* ovf_flag) {
-    extern __shared__ int lds[];
-    Stack stk{lds, spill, static_cast<int>(threadIdx.x), static_cast<int>(blockDim.x), S,
-              blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x,
-              static_cast<size_t>(gridDim.x) * blockDim.x};
-    TravCount cnt{0, 0, 0, 0, 0};
-    bool ovf = false;
-    const int count = *count_in;
-    for (;;) {
-        const int base = fetch_batch(work);
-        if (base >= count) break;
-        const int i = base + lane_id();
-        if (i < count) {
-            const int slot = queue[i];
-            if (slot >= 0) {
-                const float4 o4 = pb.ro[slot], d4 = pb.rd[slot];
-                const int depth = pb.meta[slot].x;
-                Hit h = closest<COUNT>(sc, op, xyz(o4), xyz(d4), depth, stk, cnt, ovf);
-                pb.hit[slot] = make_float4(h.t, h.u, h.v, __int_as_float(h.tri));
-                pb.hit2[slot] = make_int2(h.inst, h.light);
-            }
-        }
-    }
-    flush_counts<COUNT>(cnt, counters);
-    if (ovf) atomicOr(ovf_flag, 1);
+// Path slot -> pixel: slot = sample_in_pass * npix + item.
+__device__ __forceinline__ bool slot_pixel(const DevOpts& op, int slot, int npix, int& s, int& px, int& py) {
+    s = slot / npix;
+    item_pixel(slot - s * npix, op.width, px, py);
+    return px < op.width && py < op.height;
 }
 
 __device__ __forceinline__ void load_path(const PathBuf& pb, int slot, PathRegs& p) {
@@ -547,15 +503,81 @@ __device__ __forceinline__ void store_path(const PathBuf& pb, int slot, const Pa
     if (pb.med) pb.med[slot] = F4(p.med.color, p.med.density);
 }
 
-// Shade: everything of one bounce except the NEE visibility tests, which are
-// appended to the shadow queue; continuing paths are compacted into qout.
+// Camera ray and initial state of a primary path (tile.glsl:41-69); the
+// bounce-0 stages recompute it instead of storing it.
+__device__ __forceinline__ void primary_path(const DevOpts& op, int px, int py, int sample, PathRegs& p) {
+    init_path(p);
+    p.rng = init_rng(static_cast<uint32_t>(px), static_cast<uint32_t>(py), static_cast<uint32_t>(sample + 2));
+    camera_ray(op, px, py, p.rng, p.ro, p.rd);
+}
+
+// Extend: closest hit of every queued path. Misses finish here (environment
+// or uniform light, pathtrace.glsl:305-339) and write only their radiance;
+// hits are compacted into the shade queue. At bounce 0 (PRIMARY) the input is
+// every slot of the pass and the camera ray is generated in place.
+template <bool COUNT, bool PRIMARY>
+__global__ void __launch_bounds__(256) k_extend(DevScene sc, DevOpts op, PathBuf pb, const int* __restrict__ qin,
+                                                const int* count_in, int npaths, int npix, int first_sample,
+                                                int* hitq, int* hit_count, int* work, int S, int* spill,
+                                                unsigned long long* counters, int* ovf_flag) {
+    extern __shared__ int lds[];
+    Stack stk{lds, spill, static_cast<int>(threadIdx.x), static_cast<int>(blockDim.x), S,
+              blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x,
+              static_cast<size_t>(gridDim.x) * blockDim.x};
+    TravCount cnt{0, 0, 0, 0, 0, 0};
+    bool ovf = false;
+    const int count = PRIMARY ? npaths : *count_in;
+    for (int base = first_batch(); base < count; base = next_batch(work)) {
+        const int i = base + lane_id();
+        bool keep = false;
+        int slot = -1;
+        if (i < count) {
+            slot = PRIMARY ? i : qin[i];
+            PathRegs p;
+            bool valid = true;
+            if (PRIMARY) {
+                int s, px, py;
+                valid = slot_pixel(op, slot, npix, s, px, py);
+                if (valid) primary_path(op, px, py, first_sample + s, p);
+            } else {
+                const float4 o4 = pb.ro[slot], d4 = pb.rd[slot];
+                p.ro = xyz(o4); p.rd = xyz(d4);
+                p.depth = pb.meta[slot].x;
+            }
+            if (valid) {
+                Hit h = closest<COUNT>(sc, op, p.ro, p.rd, p.depth, stk, cnt, ovf);
+                if (h.t == INF) {
+                    if (!PRIMARY) {
+                        const float4 o4 = pb.ro[slot], t4 = pb.thr[slot], r4 = pb.rad[slot];
+                        p.prev_pdf = o4.w; p.thr = xyz(t4); p.rad = xyz(r4); p.alpha = r4.w;
+                        p.surface_scatter = (pb.meta[slot].y >> 17) & 1;
+                    }
+                    on_miss(sc, op, p);
+                    pb.rad[slot] = F4(p.rad, p.alpha);
+                } else {
+                    pb.hit[slot] = make_float4(h.t, h.u, h.v, __int_as_float(h.tri));
+                    pb.hit2[slot] = make_int2(h.inst, h.light);
+                    keep = true;
+                }
+            }
+        }
+        const int pos = wave_append(keep, hit_count);
+        if (pos >= 0) hitq[pos] = slot;
+    }
+    flush_counts<COUNT>(cnt, counters);
+    if (ovf) atomicOr(ovf_flag, 1);
+}
+
+// Shade: one bounce of every path that hit something, except the NEE
+// visibility tests, which go to the shadow queue. Continuing paths are
+// compacted into the next extend queue; ending paths write only radiance.
+template <bool PRIMARY>
 __global__ void __launch_bounds__(256) k_shade(DevScene sc, DevOpts op, PathBuf pb, ShadowBuf sb,
-                                               const int* __restrict__ qin, const int* count_in, int* qout,
-                                               int* count_out, int* shadow_count, int* work) {
-    const int count = *count_in;
-    for (;;) {
-        const int base = fetch_batch(work);
-        if (base >= count) break;
+                                               const int* __restrict__ hitq, const int* hit_count, int npix,
+                                               int first_sample, int* qout, int* count_out, int* shadow_count,
+                                               int* work) {
+    const int count = *hit_count;
+    for (int base = first_batch(); base < count; base = next_batch(work)) {
         const int i = base + lane_id();
         int slot = -1;
         bool keep = false;
@@ -563,35 +585,36 @@ __global__ void __launch_bounds__(256) k_shade(DevScene sc, DevOpts op, PathBuf 
         nee[0].valid = nee[1].valid = false;
         vec3 thr_nee = V1(0.f);
         if (i < count) {
-            slot = qin[i];
-            if (slot >= 0) {
-                PathRegs p;
+            slot = hitq[i];
+            PathRegs p;
+            if (PRIMARY) {
+                int s, px, py;
+                slot_pixel(op, slot, npix, s, px, py);
+                primary_path(op, px, py, first_sample + s, p);
+            } else {
                 load_path(pb, slot, p);
-                const float4 h4 = pb.hit[slot];
-                const int2 h2 = pb.hit2[slot];
-                Hit h;
-                h.t = h4.x; h.u = h4.y; h.v = h4.z; h.tri = __float_as_int(h4.w); h.inst = h2.x; h.light = h2.y;
-                bool alive = false;
-                if (h.t == INF) {
-                    on_miss(sc, op, p);
-                } else {
-                    SurfState st;
-                    init_surf(st);
-                    Material m;
-                    float eta;
-                    const int ev = bounce_pre(sc, op, h, p, st, m, eta);
-                    if (ev != EV_END) {
-                        if (ev != EV_PASS) {
-                            nee_rays(sc, op, p, p.rng, st, m, eta, nee);
-                            thr_nee = p.thr;
-                        }
-                        alive = bounce_post(op, p, st, m, eta, ev);
-                        if (alive) p.depth++;
-                    }
-                }
-                store_path(pb, slot, p);
-                keep = alive;
             }
+            const float4 h4 = pb.hit[slot];
+            const int2 h2 = pb.hit2[slot];
+            Hit h;
+            h.t = h4.x; h.u = h4.y; h.v = h4.z; h.tri = __float_as_int(h4.w); h.inst = h2.x; h.light = h2.y;
+            SurfState st;
+            init_surf(st);
+            Material m;
+            float eta;
+            bool alive = false;
+            const int ev = bounce_pre(sc, op, h, p, st, m, eta);
+            if (ev != EV_END) {
+                if (ev != EV_PASS) {
+                    nee_rays(sc, op, p, p.rng, st, m, eta, nee);
+                    thr_nee = p.thr;
+                }
+                alive = bounce_post(op, p, st, m, eta, ev);
+                if (alive) p.depth++;
+            }
+            if (alive) store_path(pb, slot, p);
+            else pb.rad[slot] = F4(p.rad, p.alpha);
+            keep = alive;
         }
         // NEE rays: contribution * throughput at NEE time, added if unoccluded.
         for (int k = 0; k < 2; ++k) {
@@ -618,12 +641,10 @@ __global__ void __launch_bounds__(256) k_shadow(DevScene sc, DevOpts op, PathBuf
     Stack stk{lds, spill, static_cast<int>(threadIdx.x), static_cast<int>(blockDim.x), S,
               blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x,
               static_cast<size_t>(gridDim.x) * blockDim.x};
-    TravCount cnt{0, 0, 0, 0, 0};
+    TravCount cnt{0, 0, 0, 0, 0, 0};
     bool ovf = false;
     const int count = *count_in;
-    for (;;) {
-        const int base = fetch_batch(work);
-        if (base >= count) break;
+    for (int base = first_batch(); base < count; base = next_batch(work)) {
         const int i = base + lane_id();
         if (i < count) {
             const float4 o4 = sb.o[i], d4 = sb.d[i];
@@ -671,7 +692,7 @@ __global__ void __launch_bounds__(256) k_query_closest(DevScene sc, DevOpts op, 
     Stack stk{lds, spill, static_cast<int>(threadIdx.x), static_cast<int>(blockDim.x), S,
               static_cast<size_t>(i), static_cast<size_t>(gridDim.x) * blockDim.x};
     if (i >= n) return;
-    TravCount cnt{0, 0, 0, 0, 0};
+    TravCount cnt{0, 0, 0, 0, 0, 0};
     bool ovf = false;
     vec3 o = V(orig[3 * i], orig[3 * i + 1], orig[3 * i + 2]);
     vec3 d = V(dir[3 * i], dir[3 * i + 1], dir[3 * i + 2]);
@@ -693,7 +714,7 @@ __global__ void __launch_bounds__(256) k_query_any(DevScene sc, DevOpts op, cons
     Stack stk{lds, spill, static_cast<int>(threadIdx.x), static_cast<int>(blockDim.x), S,
               static_cast<size_t>(i), static_cast<size_t>(gridDim.x) * blockDim.x};
     if (i >= n) return;
-    TravCount cnt{0, 0, 0, 0, 0};
+    TravCount cnt{0, 0, 0, 0, 0, 0};
     bool ovf = false;
     vec3 o = V(orig[3 * i], orig[3 * i + 1], orig[3 * i + 2]);
     vec3 d = V(dir[3 * i], dir[3 * i + 1], dir[3 * i + 2]);
@@ -774,9 +795,9 @@ int persistent_blocks(int kind, int bs, int S, bool count, int num_cu) {
     int per_cu = 1;
     switch (kind) {
         case KIND_PLAIN: per_cu = count ? max_blocks(k_plain<true>, bs, lds) : max_blocks(k_plain<false>, bs, lds); break;
-        case KIND_EXTEND: per_cu = count ? max_blocks(k_extend<true>, bs, lds) : max_blocks(k_extend<false>, bs, lds); break;
+        case KIND_EXTEND: per_cu = count ? max_blocks(k_extend<true, false>, bs, lds) : max_blocks(k_extend<false, false>, bs, lds); break;
         case KIND_SHADOW: per_cu = count ? max_blocks(k_shadow<true>, bs, lds) : max_blocks(k_shadow<false>, bs, lds); break;
-        case KIND_SHADE: per_cu = max_blocks(k_shade, bs, 0); break;
+        case KIND_SHADE: per_cu = max_blocks(k_shade<false>, bs, 0); break;
         default: break;
     }
     return per_cu * num_cu;
@@ -789,23 +810,22 @@ void launch_plain(hipStream_t s, int blocks, int bs, const DevScene& sc, const D
     else hipLaunchKernelGGL(k_plain<false>, dim3(blocks), dim3(bs), lds, s, sc, op, accum, sample, num_items, work, S, spill, counters, ovf);
 }
 
-void launch_raygen(hipStream_t s, const DevOpts& op, const PathBuf& pb, int npix, int first_sample, int num_paths,
-                   int* queue, int* counts) {
-    hipLaunchKernelGGL(k_raygen, dim3((num_paths + 255) / 256), dim3(256), 0, s, op, pb, npix, first_sample, num_paths, queue, counts);
-}
-
 void launch_extend(hipStream_t s, int blocks, int bs, const DevScene& sc, const DevOpts& op, const PathBuf& pb,
-                   const int* queue, const int* count_in, int* work, int S, int* spill, unsigned long long* counters,
-                   int* ovf, bool count) {
+                   bool primary, const int* qin, const int* count_in, int npaths, int npix, int first_sample,
+                   int* hitq, int* hit_count, int* work, int S, int* spill, unsigned long long* counters, int* ovf,
+                   bool count) {
     const size_t lds = static_cast<size_t>(S) * bs * sizeof(int);
-    if (count) hipLaunchKernelGGL(k_extend<true>, dim3(blocks), dim3(bs), lds, s, sc, op, pb, queue, count_in, work, S, spill, counters, ovf);
-    else hipLaunchKernelGGL(k_extend<false>, dim3(blocks), dim3(bs), lds, s, sc, op, pb, queue, count_in, work, S, spill, counters, ovf);
+#define PT_EXT(C, P) hipLaunchKernelGGL((k_extend<C, P>), dim3(blocks), dim3(bs), lds, s, sc, op, pb, qin, count_in, npaths, npix, first_sample, hitq, hit_count, work, S, spill, counters, ovf)
+    if (count) { if (primary) PT_EXT(true, true); else PT_EXT(true, false); }
+    else { if (primary) PT_EXT(false, true); else PT_EXT(false, false); }
+#undef PT_EXT
 }
 
 void launch_shade(hipStream_t s, int blocks, int bs, const DevScene& sc, const DevOpts& op, const PathBuf& pb,
-                  const ShadowBuf& sb, const int* qin, const int* count_in, int* qout, int* count_out,
-                  int* shadow_count, int* work) {
-    hipLaunchKernelGGL(k_shade, dim3(blocks), dim3(bs), 0, s, sc, op, pb, sb, qin, count_in, qout, count_out, shadow_count, work);
+                  const ShadowBuf& sb, bool primary, const int* hitq, const int* hit_count, int npix, int first_sample,
+                  int* qout, int* count_out, int* shadow_count, int* work) {
+    if (primary) hipLaunchKernelGGL(k_shade<true>, dim3(blocks), dim3(bs), 0, s, sc, op, pb, sb, hitq, hit_count, npix, first_sample, qout, count_out, shadow_count, work);
+    else hipLaunchKernelGGL(k_shade<false>, dim3(blocks), dim3(bs), 0, s, sc, op, pb, sb, hitq, hit_count, npix, first_sample, qout, count_out, shadow_count, work);
 }
 
 void launch_shadow(hipStream_t s, int blocks, int bs, const DevScene& sc, const DevOpts& op, const PathBuf& pb,

@@ -108,9 +108,9 @@ struct pt_renderer_s {
     DevBuf<int4> p_meta;
     DevBuf<int2> p_hit2;
     DevBuf<float4> s_o, s_d, s_c;
-    DevBuf<int> qa, qb, counters;
+    DevBuf<int> qa, qb, hq, counters;
     DevBuf<int> spill;
-    DevBuf<unsigned long long> work_counts;  // nodes, tris, insts, lights, spills
+    DevBuf<unsigned long long> work_counts;  // [0,6) closest-hit work, [8,14) shadow work: nodes, tris, insts, lights, spills, rays
     DevBuf<int> ovf;
     DevBuf<uchar4> ldr;
 
@@ -134,7 +134,7 @@ struct pt_renderer_s {
         for (auto e : free_events) (void)hipEventDestroy(e);
         if (ev_r0) (void)hipEventDestroy(ev_r0);
         if (ev_r1) (void)hipEventDestroy(ev_r1);
-        DevBuf<int>* ib[] = {&qa, &qb, &counters, &spill, &ovf};
+        DevBuf<int>* ib[] = {&qa, &qb, &hq, &counters, &spill, &ovf};
         for (auto* b : ib) b->free();
         nodes.free(); tris.free(); vidx.free(); verts.free(); norms.free(); mats.free(); inst.free();
         lights.free(); tex.free(); env_img.free(); env_cdf.free(); accum.free();
@@ -183,13 +183,19 @@ struct pt_renderer_s {
             render_pending = false;
         }
         if (opts.count_work && work_counts.p) {
-            unsigned long long c[5];
+            unsigned long long c[16];
             HIPCHK(hipMemcpy(c, work_counts.p, sizeof(c), hipMemcpyDeviceToHost));
             stats.nodes_visited = static_cast<int64_t>(c[0]);
             stats.tris_tested = static_cast<int64_t>(c[1]);
             stats.instances_entered = static_cast<int64_t>(c[2]);
             stats.lights_tested = static_cast<int64_t>(c[3]);
-            stats.stack_spills = static_cast<int64_t>(c[4]);
+            stats.rays_closest = static_cast<int64_t>(c[5]);
+            stats.shadow_nodes = static_cast<int64_t>(c[8]);
+            stats.shadow_tris = static_cast<int64_t>(c[9]);
+            stats.shadow_instances = static_cast<int64_t>(c[10]);
+            stats.shadow_lights = static_cast<int64_t>(c[11]);
+            stats.rays_shadow = static_cast<int64_t>(c[13]);
+            stats.stack_spills = static_cast<int64_t>(c[4] + c[12]);
         }
         if (ovf.p) {
             int f = 0;
@@ -243,7 +249,7 @@ struct pt_renderer_s {
         spill.ensure(threads * static_cast<size_t>(ptd::MAXD - stack_S() + 1));
     }
     void ensure_misc() {
-        work_counts.ensure(8);
+        work_counts.ensure(16);
         ovf.ensure(4);
     }
 };
@@ -438,19 +444,18 @@ void check_ready(pt_renderer_s* h) {
     if (!h->has_cam) throw ArgError(PT_ERR_STATE, "no camera set (pt_set_camera)");
 }
 
-// Counter layout for one wavefront pass of D bounces:
-//   [0, D+1)          paths entering bounce b
-//   [D+1, 2D+2)       shadow rays of bounce b
-//   [2D+2, 5D+5)      work counters (extend, shade, shadow) of bounce b
-//   [5D+5]            plain-kernel work counter
-//   [5D+6]            first sample of the pass
-int counter_cells(int D) { return 5 * D + 8; }
+// Counter layout for one wavefront pass of D bounces (all zeroed per pass):
+//   [0, D+1)            paths entering extend at bounce b
+//   [D+1, 2D+2)         hits entering shade at bounce b
+//   [2D+2, 3D+3)        shadow rays of bounce b
+//   [3D+3, 6D+6)        work counters (extend, shade, shadow) of bounce b
+int counter_cells(int D) { return 6 * D + 6; }
 
 void render_wavefront(pt_renderer_s* h, int first_sample, int num_samples) {
     const DevOpts op = h->dev_opts();
     if ((op.flags & ptd::OPT_VOL_MIS) && (op.flags & ptd::OPT_MEDIUM))
         throw ArgError(PT_ERR_UNSUPPORTED, "volume MIS is implemented by the plain kernel only (mode PT_MODE_PLAIN)");
-    if (op.flags & ptd::OPT_ALPHA_TEST && !(op.flags & ptd::OPT_MEDIUM)) {
+    if ((op.flags & ptd::OPT_ALPHA_TEST) && !(op.flags & ptd::OPT_MEDIUM)) {
         // Stochastic alpha in AnyHit draws from the path's RNG stream between NEE
         // and BSDF sampling, which the deferred shadow stage cannot reproduce.
         throw ArgError(PT_ERR_UNSUPPORTED, "alpha-tested shadow rays need the plain kernel (mode PT_MODE_PLAIN)");
@@ -466,7 +471,7 @@ void render_wavefront(pt_renderer_s* h, int first_sample, int num_samples) {
     if (h->any_medium) h->p_med.ensure(max_paths);
     h->p_hit.ensure(max_paths); h->p_hit2.ensure(max_paths);
     h->s_o.ensure(2 * max_paths); h->s_d.ensure(2 * max_paths); h->s_c.ensure(2 * max_paths);
-    h->qa.ensure(max_paths); h->qb.ensure(max_paths);
+    h->qa.ensure(max_paths); h->qb.ensure(max_paths); h->hq.ensure(max_paths);
     // Bounce iterations: maxDepth + 1 closest hits; alpha pass-through repeats a depth.
     const int D = op.max_depth + 1 + ((op.flags & ptd::OPT_ALPHA_TEST) ? 64 : 0);
     h->counters.ensure(counter_cells(D));
@@ -485,26 +490,27 @@ void render_wavefront(pt_renderer_s* h, int first_sample, int num_samples) {
     for (int s0 = 0; s0 < num_samples; s0 += spp_pass) {
         const int ns = std::min(spp_pass, num_samples - s0);
         const int npaths = npix * ns;
+        const int fs = first_sample + s0;
         hipEvent_t ev;
         ptk::launch_fill(s, cnt, counter_cells(D), 0);
-        h->stage_begin(PT_STAGE_RAYGEN, ev);
-        ptk::launch_raygen(s, op, pb, npix, first_sample + s0, npaths, h->qa.p, cnt);
-        h->stage_end(PT_STAGE_RAYGEN, ev);
         int* qin = h->qa.p;
         int* qout = h->qb.p;
         for (int b = 0; b < D; ++b) {
-            int* c_in = cnt + b;
-            int* c_out = cnt + b + 1;
-            int* c_sh = cnt + (D + 1) + b;
-            int* w = cnt + (2 * D + 2) + 3 * b;
+            int* c_ext = cnt + b;
+            int* c_next = cnt + b + 1;
+            int* c_hit = cnt + (D + 1) + b;
+            int* c_sh = cnt + 2 * (D + 1) + b;
+            int* w = cnt + 3 * (D + 1) + 3 * b;
+            const bool primary = b == 0;
             h->stage_begin(PT_STAGE_EXTEND, ev);
-            ptk::launch_extend(s, b_ext, bs, h->sc, op, pb, qin, c_in, w + 0, S, h->spill.p, h->work_counts.p, h->ovf.p, cw);
+            ptk::launch_extend(s, b_ext, bs, h->sc, op, pb, primary, qin, c_ext, npaths, npix, fs, h->hq.p, c_hit, w + 0,
+                               S, h->spill.p, h->work_counts.p, h->ovf.p, cw);
             h->stage_end(PT_STAGE_EXTEND, ev);
             h->stage_begin(PT_STAGE_SHADE, ev);
-            ptk::launch_shade(s, b_sha, bs, h->sc, op, pb, sb, qin, c_in, qout, c_out, c_sh, w + 1);
+            ptk::launch_shade(s, b_sha, bs, h->sc, op, pb, sb, primary, h->hq.p, c_hit, npix, fs, qout, c_next, c_sh, w + 1);
             h->stage_end(PT_STAGE_SHADE, ev);
             h->stage_begin(PT_STAGE_SHADOW, ev);
-            ptk::launch_shadow(s, b_sdw, bs, h->sc, op, pb, sb, c_sh, w + 2, S, h->spill.p, h->work_counts.p, h->ovf.p, cw);
+            ptk::launch_shadow(s, b_sdw, bs, h->sc, op, pb, sb, c_sh, w + 2, S, h->spill.p, h->work_counts.p + 8, h->ovf.p, cw);
             h->stage_end(PT_STAGE_SHADOW, ev);
             std::swap(qin, qout);
         }
@@ -519,8 +525,7 @@ void render_plain(pt_renderer_s* h, int first_sample, int num_samples) {
     const DevOpts op = h->dev_opts();
     const int W = op.width, H = op.height;
     const int items = ((W + 7) / 8) * ((H + 7) / 8) * 64;
-    const int D = 1;
-    h->counters.ensure(std::max<size_t>(h->counters.n, counter_cells(D)));
+    h->counters.ensure(std::max<size_t>(h->counters.n, 8));
     h->ensure_misc();
     const int bs = h->block_size(), S = h->stack_S();
     const bool cw = h->opts.count_work != 0;
@@ -757,7 +762,7 @@ int pt_render(pt_handle h, int32_t first_sample, int32_t num_samples) {
         h->resolve_events();
         if (h->opts.count_work) {
             h->ensure_misc();
-            HIPCHK(hipMemsetAsync(h->work_counts.p, 0, 8 * sizeof(unsigned long long), h->stream));
+            HIPCHK(hipMemsetAsync(h->work_counts.p, 0, 16 * sizeof(unsigned long long), h->stream));
         }
         h->ensure_misc();
         HIPCHK(hipMemsetAsync(h->ovf.p, 0, sizeof(int), h->stream));

@@ -75,9 +75,10 @@ class PtOptions(C.Structure):
 class PtStats(C.Structure):
     _fields_ = [("ms_last_render", C.c_double), ("ms_stage", C.c_double * 8),
                 ("launches_stage", C.c_int64 * 8),
-                ("rays_closest", C.c_int64), ("rays_shadow", C.c_int64),
-                ("nodes_visited", C.c_int64), ("tris_tested", C.c_int64),
+                ("rays_closest", C.c_int64), ("nodes_visited", C.c_int64), ("tris_tested", C.c_int64),
                 ("instances_entered", C.c_int64), ("lights_tested", C.c_int64),
+                ("rays_shadow", C.c_int64), ("shadow_nodes", C.c_int64), ("shadow_tris", C.c_int64),
+                ("shadow_instances", C.c_int64), ("shadow_lights", C.c_int64),
                 ("stack_spills", C.c_int64), ("stack_overflow", i32)]
 
 

@@ -142,8 +142,10 @@ typedef struct pt_stats {
     double ms_last_render;           /* device time of the last pt_render / pt_batch_render */
     double ms_stage[8];              /* per-stage device time accumulated since pt_reset_stats */
     int64_t launches_stage[8];
-    int64_t rays_closest, rays_shadow;      /* rays issued (count_work only) */
-    int64_t nodes_visited, tris_tested, instances_entered, lights_tested; /* count_work only */
+    /* count_work only: traversal work of closest-hit queries (extend stage, or
+     * everything in the plain kernel) and of shadow queries (shadow stage) */
+    int64_t rays_closest, nodes_visited, tris_tested, instances_entered, lights_tested;
+    int64_t rays_shadow, shadow_nodes, shadow_tris, shadow_instances, shadow_lights;
     int64_t stack_spills;            /* traversal-stack entries that went to the global backing store */
     int32_t stack_overflow;          /* 1 if a traversal exceeded the maximum stack depth */
 } pt_stats;

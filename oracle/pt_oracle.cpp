@@ -1381,6 +1381,51 @@ extern "C" {
 
 void oracle_set_inverse_mode(int32_t mode) { g_inverse_mode = mode; }
 
+int oracle_eval_fn(int32_t fn, int32_t n, const float* in, float* out) {
+    static const int IN[] = {2, 5, 2, 5, 2, 1, 2, 7, 3, 2, 2, 6, 2, 10, 19, 12};
+    static const int OUT[] = {1, 1, 1, 1, 1, 1, 1, 3, 3, 3, 3, 3, 1, 1, 1, 1};
+    if (fn < 0 || fn > 15) return -1;
+    const int ni = IN[fn], no = OUT[fn];
+    for (int i = 0; i < n; ++i) {
+        const float* a = in + static_cast<size_t>(i) * ni;
+        float* o = out + static_cast<size_t>(i) * no;
+        V3<float> v;
+        switch (fn) {
+            case 0: o[0] = GTR1(a[0], a[1]); break;
+            case 1: o[0] = GTR2Aniso(a[0], a[1], a[2], a[3], a[4]); break;
+            case 2: o[0] = SmithG(a[0], a[1]); break;
+            case 3: o[0] = SmithGAniso(a[0], a[1], a[2], a[3], a[4]); break;
+            case 4: o[0] = DielectricFresnel(a[0], a[1]); break;
+            case 5: o[0] = SchlickWeight(a[0]); break;
+            case 6: o[0] = PowerHeuristic(a[0], a[1]); break;
+            case 7: v = SampleGGXVNDF(fetch3<float>(a), a[3], a[4], a[5], a[6]); break;
+            case 8: v = SampleGTR1(a[0], a[1], a[2]); break;
+            case 9: v = CosineSampleHemisphere(a[0], a[1]); break;
+            case 10: v = UniformSampleHemisphere(a[0], a[1]); break;
+            case 11: v = SampleHG(fetch3<float>(a), a[3], a[4], a[5]); break;
+            case 12: o[0] = PhaseHG(a[0], a[1]); break;
+            case 13: {
+                Ray<float> r{fetch3<float>(a + 4), fetch3<float>(a + 7)};
+                o[0] = SphereIntersect(a[0], fetch3<float>(a + 1), r);
+                break;
+            }
+            case 14: {
+                Ray<float> r{fetch3<float>(a + 13), fetch3<float>(a + 16)};
+                V4<float> pl{a[9], a[10], a[11], a[12]};
+                o[0] = RectIntersect(fetch3<float>(a), fetch3<float>(a + 3), fetch3<float>(a + 6), pl, r);
+                break;
+            }
+            case 15: {
+                Ray<float> r{fetch3<float>(a + 6), fetch3<float>(a + 9)};
+                o[0] = AABBIntersect(fetch3<float>(a), fetch3<float>(a + 3), r);
+                break;
+            }
+        }
+        if (no == 3) { o[0] = v.x; o[1] = v.y; o[2] = v.z; }
+    }
+    return 0;
+}
+
 int oracle_render(int32_t precision, const oracle_scene* s, const oracle_camera* cam, const oracle_options* o,
                   int32_t first_sample, int32_t num_samples, int32_t row_begin, int32_t row_end, int32_t row_step,
                   float* accum_f32, double* accum_f64, int32_t num_threads, oracle_counters* counters) {

@@ -77,6 +77,16 @@ int oracle_any_hit(const oracle_scene* s, const oracle_options* o, const float* 
  * scalar type (the shader's inverse(mat4)), 1 = double inverse rounded. */
 void oracle_set_inverse_mode(int32_t mode);
 
+/* Sampling and intersection primitives, n calls with packed arguments:
+ *  0 GTR1(NdotH, a)  1 GTR2Aniso(NdotH, HdotX, HdotY, ax, ay)  2 SmithG(NdotV, a)
+ *  3 SmithGAniso(NdotV, VdotX, VdotY, ax, ay)  4 DielectricFresnel(cos, eta)
+ *  5 SchlickWeight(u)  6 PowerHeuristic(a, b)  7 SampleGGXVNDF(V3, ax, ay, r1, r2)->3
+ *  8 SampleGTR1(rgh, r1, r2)->3  9 CosineSampleHemisphere(r1, r2)->3
+ * 10 UniformSampleHemisphere(r1, r2)->3  11 SampleHG(V3, g, r1, r2)->3  12 PhaseHG(cos, g)
+ * 13 SphereIntersect(rad, pos3, o3, d3)  14 RectIntersect(pos3, u3, v3, plane4, o3, d3)
+ * 15 AABBIntersect(min3, max3, o3, d3) */
+int oracle_eval_fn(int32_t fn, int32_t n, const float* in, float* out);
+
 /* RNG known-answer access: n successive pcg4d states from seed. */
 void oracle_pcg4d(const uint32_t seed[4], int32_t n, uint32_t* states_out, float* rand_out);
 

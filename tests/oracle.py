@@ -73,8 +73,42 @@ def lib():
         l.oracle_sample_light.argtypes = [P(f32), i32, P(f32), P(C.c_uint32), P(f32)]
         l.oracle_eval_envmap.argtypes = [P(OracleScene), P(OracleOptions), P(f32), P(f32)]
         l.oracle_sample_envmap.argtypes = [P(OracleScene), P(OracleOptions), P(C.c_uint32), P(f32), P(f32)]
+        l.oracle_eval_fn.argtypes = [i32, i32, P(f32), P(f32)]
+        l.oracle_set_inverse_mode.argtypes = [i32]
+        l.oracle_set_inverse_mode.restype = None
         _lib = l
     return _lib
+
+
+FN = dict(GTR1=0, GTR2Aniso=1, SmithG=2, SmithGAniso=3, DielectricFresnel=4, SchlickWeight=5, PowerHeuristic=6,
+          SampleGGXVNDF=7, SampleGTR1=8, CosineSampleHemisphere=9, UniformSampleHemisphere=10, SampleHG=11,
+          PhaseHG=12, SphereIntersect=13, RectIntersect=14, AABBIntersect=15)
+_FN_OUT = {7: 3, 8: 3, 9: 3, 10: 3, 11: 3}
+
+
+def fn(name, *cols):
+    """Evaluate an oracle primitive; each argument is a scalar, a length-n
+    vector or an (n, k) array of k packed values."""
+    arrs = [np.asarray(c, np.float64) for c in cols]
+    n = max([a.shape[0] for a in arrs if a.ndim >= 1] or [1])
+    flat = []
+    for a in arrs:
+        if a.ndim == 0:
+            flat.append(np.full((n, 1), float(a)))
+        elif a.ndim == 1:
+            flat.append(a.reshape(n, 1))
+        else:
+            flat.append(a.reshape(n, -1))
+    inp = np.ascontiguousarray(np.concatenate(flat, axis=1), np.float32)
+    k = FN[name]
+    out = np.zeros((n, _FN_OUT.get(k, 1)), np.float32)
+    rc = lib().oracle_eval_fn(k, n, fp(inp), fp(out))
+    assert rc == 0
+    return out if out.shape[1] > 1 else out[:, 0]
+
+
+def set_inverse_mode(mode):
+    lib().oracle_set_inverse_mode(int(mode))
 
 
 def fp(a):
@@ -196,3 +230,29 @@ def trace_samples(sa, cam, opts, px, py, sample):
     lib().oracle_trace_samples(C.byref(sa.s), C.byref(cam), C.byref(opts), n, ip(np.asarray(px, np.int32)),
                                ip(np.asarray(py, np.int32)), ip(np.asarray(sample, np.int32)), fp(out))
     return out
+
+
+def disney_eval(mat, V, N, L, eta=-1.0):
+    f = np.zeros(3, np.float32)
+    pdf = np.zeros(1, np.float32)
+    lib().oracle_disney_eval(fp(np.asarray(mat, np.float32)), float(eta), fp(np.asarray(V, np.float32)),
+                             fp(np.asarray(N, np.float32)), fp(np.asarray(L, np.float32)), fp(f), fp(pdf))
+    return f, float(pdf[0])
+
+
+def disney_sample(mat, V, N, seed, eta=-1.0):
+    s = np.asarray(seed, np.uint32).copy()
+    L = np.zeros(3, np.float32)
+    f = np.zeros(3, np.float32)
+    pdf = np.zeros(1, np.float32)
+    lib().oracle_disney_sample(fp(np.asarray(mat, np.float32)), float(eta), fp(np.asarray(V, np.float32)),
+                               fp(np.asarray(N, np.float32)), up(s), fp(L), fp(f), fp(pdf))
+    return L, f, float(pdf[0]), s
+
+
+def sample_light(light, num_lights, pos, seed):
+    s = np.asarray(seed, np.uint32).copy()
+    rec = np.zeros(13, np.float32)
+    lib().oracle_sample_light(fp(np.asarray(light, np.float32)), int(num_lights), fp(np.asarray(pos, np.float32)),
+                              up(s), fp(rec))
+    return dict(direction=rec[0:3], normal=rec[3:6], emission=rec[6:9], dist=rec[9], pdf=rec[10]), s

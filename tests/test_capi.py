@@ -1,0 +1,93 @@
+"""The C-ABI libraries load and export every function include/glslpt.h
+declares; error conventions hold without a GPU (no compute calls here)."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+from glslpt import _capi as api
+from glslpt.scene import Scene
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "glslpt.h")
+
+
+def declared():
+    txt = open(HEADER).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return set(re.findall(r"\b(pts?_[a-z_0-9]+)\s*\(", txt))
+
+
+def test_header_declares_both_surfaces():
+    names = declared()
+    assert {"pt_create", "pt_render", "pt_upload_scene", "pt_batch_render", "pt_adjoint"} <= names
+    assert {"pts_create", "pts_add_mesh", "pts_process"} <= names
+
+
+def test_scene_library_exports_declared_symbols():
+    lib = C.CDLL(os.path.join(api.LIB_DIR, "libglslpt_scene.so"))
+    for n in sorted(x for x in declared() if x.startswith("pts_")):
+        assert hasattr(lib, n), n
+    assert set(api.SCENE_SYMBOLS) == {x for x in declared() if x.startswith("pts_")}
+
+
+def test_hip_library_exports_declared_symbols():
+    path = os.path.join(api.LIB_DIR, "libglslpt_hip.so")
+    assert os.path.exists(path), "HIP library not built"
+    lib = C.CDLL(path)
+    for n in sorted(x for x in declared() if x.startswith("pt_")):
+        assert hasattr(lib, n), n
+    assert set(api.HIP_SYMBOLS) == {x for x in declared() if x.startswith("pt_")}
+
+
+def test_struct_sizes_match_header_layouts():
+    assert C.sizeof(api.PtMaterial) == 128   # 8 x vec4 (src/core/Material.h)
+    assert C.sizeof(api.PtLight) == 60       # 5 x vec3 (src/core/Scene.h:50-59)
+    assert C.sizeof(api.PtBvhNode) == 36     # 3 x vec3 (bvh_translator.h:45-50)
+    assert C.sizeof(api.PtCamera) == 60
+
+
+def test_scene_error_codes():
+    lib = api.scene_lib()
+    h = C.c_void_p()
+    assert lib.pts_create(C.byref(h)) == 0
+    out = C.c_int32()
+    # no positions
+    assert lib.pts_add_mesh(h, None, None, None, 3, b"x", C.byref(out)) == api.PT_OK - 1
+    assert b"positions" in lib.pts_last_error(h)
+    # process before anything exists
+    assert lib.pts_process(h) < 0
+    v = api.PtSceneView()
+    assert lib.pts_get_view(h, C.byref(v)) < 0
+    assert lib.pts_destroy(h) == 0
+
+
+def test_renderer_without_gpu_fails_loudly():
+    lib = api.hip_lib()
+    n = C.c_int32(-1)
+    assert lib.pt_device_count(C.byref(n)) == 0
+    if n.value == 0:
+        h = C.c_void_p()
+        assert lib.pt_create(0, C.byref(h)) == -7  # PT_ERR_NO_DEVICE
+        from glslpt.renderer import Renderer
+        with pytest.raises(api.PtError):
+            Renderer(0)
+
+
+def test_default_options_match_render_options():
+    o = api.PtOptions()
+    api.hip_lib().pt_default_options(C.byref(o))
+    # RenderOptions() defaults (src/core/Renderer.h:41-72)
+    assert (o.width, o.height, o.max_depth, o.enable_rr, o.rr_depth) == (1280, 720, 2, 1, 2)
+    assert np.allclose(list(o.uniform_light_col), [0.3, 0.3, 0.3])
+    assert o.envmap_intensity == 1.0 and o.opengl_normal_map == 1
+
+
+def test_scene_view_counts():
+    from glslpt import scenes
+    sc, rs = scenes.cornell()
+    v = sc.view()
+    assert v.num_tris == 34 and v.num_instances == 7 and v.num_lights == 1 and v.num_materials == 7
+    assert v.num_vertices == 102

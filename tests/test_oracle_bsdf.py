@@ -1,0 +1,265 @@
+"""Disney BSDF and sampling primitives of the oracle (disney.glsl,
+sampling.glsl) pinned by published closed forms in a different algebraic
+form, normalisation integrals and sample-versus-pdf histograms.
+
+Closed forms used:
+  Burley 2012 diffuse   f = c/pi (1 + (Fd90-1)(1-cosL)^5)(1 + (Fd90-1)(1-cosV)^5),
+                        Fd90 = 1/2 + 2 r cos^2(theta_d)   (the oracle uses the
+                        2015 split Fd + Fretro, which expands to this)
+  GGX Smith G1          2 / (1 + sqrt(1 + a^2 tan^2 theta))  (Walter et al. 2007 eq. 34)
+  Fresnel               textbook s/p amplitudes with Snell's law, in double
+  normalisation         int D(h) cos(theta_h) dw = 1 (GGX, GTR1); int phase dw = 1
+"""
+import numpy as np
+import pytest
+
+import oracle
+from glslpt.scene import material, material_array
+
+
+def sph(n, rng):
+    d = rng.normal(size=(n, 3))
+    return d / np.linalg.norm(d, axis=1, keepdims=True)
+
+
+def upper(n, rng):
+    d = sph(n, rng)
+    d[:, 2] = np.abs(d[:, 2])
+    return d
+
+
+def mat(**kw):
+    return material_array(material(**kw))
+
+
+def test_burley_diffuse_closed_form():
+    rng = np.random.default_rng(1)
+    for rough in (0.05, 0.5, 1.0):
+        m = mat(baseColor=(0.8, 0.5, 0.2), roughness=rough, ior=1.0)  # ior 1: specular lobe has F = 0
+        N = np.array([0, 0, 1.0])
+        for _ in range(200):
+            V, L = upper(2, rng)
+            f, pdf = oracle.disney_eval(m, V, N, L, eta=1.0)
+            H = (V + L) / np.linalg.norm(V + L)
+            cd = L @ H
+            fd90 = 0.5 + 2 * max(rough, 0.001) * cd * cd
+            fl = (1 + (fd90 - 1) * (1 - L[2]) ** 5)
+            fv = (1 + (fd90 - 1) * (1 - V[2]) ** 5)
+            want = np.array([0.8, 0.5, 0.2]) / np.pi * fl * fv * L[2]
+            np.testing.assert_allclose(f, want, rtol=2e-5, atol=1e-7)
+
+
+def test_smith_g1_walter_form():
+    rng = np.random.default_rng(2)
+    a = rng.uniform(0.01, 1, 2000)
+    th = rng.uniform(0.01, 1.55, 2000)
+    phi = rng.uniform(0, 2 * np.pi, 2000)
+    v = np.stack([np.sin(th) * np.cos(phi), np.sin(th) * np.sin(phi), np.cos(th)], 1)
+    got = oracle.fn("SmithGAniso", v[:, 2], v[:, 0], v[:, 1], a, a)
+    want = 2 / (1 + np.sqrt(1 + a * a * np.tan(th) ** 2))
+    np.testing.assert_allclose(got, want, rtol=1e-4)
+
+
+def test_dielectric_fresnel_textbook():
+    cosi = np.linspace(0.0, 1.0, 101)
+    for eta in (1 / 1.5, 1 / 1.33, 1.5, 1.0 / 2.4):
+        got = oracle.fn("DielectricFresnel", cosi, np.full_like(cosi, eta))
+        n1, n2 = 1.0, 1.0 / eta   # sin_t = eta sin_i
+        sint = eta * np.sqrt(np.maximum(0, 1 - cosi ** 2))
+        tir = sint > 1
+        cost = np.sqrt(np.maximum(0, 1 - sint ** 2))
+        with np.errstate(invalid="ignore", divide="ignore"):
+            rs = (n1 * cosi - n2 * cost) / (n1 * cosi + n2 * cost)
+            rp = (n2 * cosi - n1 * cost) / (n2 * cosi + n1 * cost)
+        want = np.where(tir, 1.0, 0.5 * (rs ** 2 + rp ** 2))
+        np.testing.assert_allclose(got, want, atol=2e-6)
+
+
+def test_schlick_and_power_heuristic():
+    u = np.linspace(-0.5, 1.5, 41)
+    np.testing.assert_allclose(oracle.fn("SchlickWeight", u), np.clip(1 - u, 0, 1) ** 5, rtol=1e-6, atol=1e-7)
+    a = np.array([0.0, 1.0, 2.0, 3.0]); b = np.array([1.0, 1.0, 0.5, 0.0])
+    np.testing.assert_allclose(oracle.fn("PowerHeuristic", a, b), a * a / (a * a + b * b), rtol=1e-6)
+
+
+def _hemisphere_integral(fvals_fn, n=400):
+    # midpoint rule in (cos theta, phi)
+    mu = (np.arange(n) + 0.5) / n
+    ph = (np.arange(2 * n) + 0.5) / (2 * n) * 2 * np.pi
+    M, P = np.meshgrid(mu, ph, indexing="ij")
+    s = np.sqrt(1 - M ** 2)
+    h = np.stack([s * np.cos(P), s * np.sin(P), M], -1).reshape(-1, 3)
+    vals = fvals_fn(h)
+    return vals.sum() * (1.0 / n) * (2 * np.pi / (2 * n))
+
+
+@pytest.mark.parametrize("ax,ay", [(0.5, 0.5), (0.2, 0.6), (0.9, 0.9)])
+def test_ggx_normalised(ax, ay):
+    got = _hemisphere_integral(lambda h: oracle.fn("GTR2Aniso", h[:, 2], h[:, 0], h[:, 1], ax, ay) * h[:, 2])
+    assert abs(got - 1) < 2e-3
+
+
+@pytest.mark.parametrize("a", [0.05, 0.3, 0.8])
+def test_gtr1_normalised(a):
+    got = _hemisphere_integral(lambda h: oracle.fn("GTR1", h[:, 2], np.full(h.shape[0], a)) * h[:, 2], n=1200)
+    assert abs(got - 1) < 5e-3
+
+
+@pytest.mark.parametrize("g", [-0.6, 0.0, 0.3, 0.85])
+def test_hg_phase_normalised_and_sampled(g):
+    """PhaseHG integrates to 1; SampleHG(V, g) draws cos = dot(d, V) with
+    density 2 pi PhaseHG(cos, g), the pdf the bounce loop pairs it with
+    (pathtrace.glsl:400-401)."""
+    mu = np.linspace(-1, 1, 40001)
+    p = oracle.fn("PhaseHG", mu, g).astype(np.float64)
+    assert abs(np.trapezoid(p, mu) * 2 * np.pi - 1) < 1e-3
+    rng = np.random.default_rng(3)
+    n = 100000
+    V = np.tile([0.0, 0.0, 1.0], (n, 1))
+    d = oracle.fn("SampleHG", V, g, rng.random(n), rng.random(n))
+    hist, edges = np.histogram(d[:, 2], bins=20, range=(-1, 1))
+    cum = np.concatenate([[0], np.cumsum((p[1:] + p[:-1]) / 2 * np.diff(mu))]) * 2 * np.pi
+    want = n * np.diff(np.interp(edges, mu, cum))
+    ok = want > 20
+    chi2 = (((hist - want) ** 2) / want)[ok].sum()
+    assert chi2 < ok.sum() + 6 * np.sqrt(2 * ok.sum()), chi2
+
+
+def _chi2_dir_hist(dirs, pdf_fn, n_mu=16, n_phi=16, sub=6):
+    """Chi-square of sampled unit directions against a pdf over the sphere."""
+    n = dirs.shape[0]
+    mu = dirs[:, 2]
+    phi = np.mod(np.arctan2(dirs[:, 1], dirs[:, 0]), 2 * np.pi)
+    hist, _, _ = np.histogram2d(mu, phi, bins=[n_mu, n_phi], range=[[-1, 1], [0, 2 * np.pi]])
+    # expected mass per bin by midpoint sub-sampling
+    mm = -1 + (np.arange(n_mu * sub) + 0.5) * (2 / (n_mu * sub))
+    pp = (np.arange(n_phi * sub) + 0.5) * (2 * np.pi / (n_phi * sub))
+    M, Pp = np.meshgrid(mm, pp, indexing="ij")
+    s = np.sqrt(np.maximum(0, 1 - M ** 2))
+    L = np.stack([s * np.cos(Pp), s * np.sin(Pp), M], -1).reshape(-1, 3)
+    dens = pdf_fn(L).reshape(n_mu * sub, n_phi * sub) * (2 / (n_mu * sub)) * (2 * np.pi / (n_phi * sub))
+    want = n * dens.reshape(n_mu, sub, n_phi, sub).sum(axis=(1, 3))
+    ok = want > 30
+    chi2 = (((hist[ok] - want[ok]) ** 2) / want[ok]).sum()
+    return chi2, int(ok.sum()), hist[ok].sum() / n, want[ok].sum() / n
+
+
+def test_cosine_hemisphere_sampling():
+    rng = np.random.default_rng(4)
+    n = 200000
+    d = oracle.fn("CosineSampleHemisphere", rng.random(n), rng.random(n))
+    chi2, dof, _, _ = _chi2_dir_hist(d, lambda L: np.maximum(L[:, 2], 0) / np.pi)
+    assert chi2 < dof + 5 * np.sqrt(2 * dof)
+
+
+def test_vndf_sampling_matches_visible_normals():
+    rng = np.random.default_rng(5)
+    n = 200000
+    ax, ay = 0.35, 0.6
+    V = np.array([0.5, -0.3, 0.8]); V /= np.linalg.norm(V)
+    h = oracle.fn("SampleGGXVNDF", np.tile(V, (n, 1)), np.full(n, ax), np.full(n, ay), rng.random(n), rng.random(n))
+
+    def dv(H):  # D_V(H) = G1(V) max(0, V.H) D(H) / V.z   (Heitz 2018)
+        g1 = oracle.fn("SmithGAniso", np.full(H.shape[0], V[2]), np.full(H.shape[0], V[0]), np.full(H.shape[0], V[1]), ax, ay)
+        d = oracle.fn("GTR2Aniso", H[:, 2], H[:, 0], H[:, 1], ax, ay)
+        return np.where(H[:, 2] > 0, g1 * np.maximum(0, H @ V) * d / V[2], 0)
+
+    chi2, dof, _, _ = _chi2_dir_hist(h, dv, n_mu=16, n_phi=16, sub=8)
+    assert chi2 < dof + 6 * np.sqrt(2 * dof), (chi2, dof)
+
+
+MATERIALS = {
+    "diffuse": dict(baseColor=(0.7, 0.6, 0.5), roughness=0.6),
+    "metal_rough": dict(baseColor=(0.9, 0.6, 0.3), metallic=1.0, roughness=0.5),
+    "metal_aniso": dict(baseColor=(0.8, 0.8, 0.8), metallic=1.0, roughness=0.4, anisotropic=0.8),
+    "clearcoat": dict(baseColor=(0.2, 0.4, 0.7), roughness=0.7, clearcoat=1.0, clearcoatGloss=0.3),
+    "glass": dict(baseColor=(1.0, 1.0, 1.0), roughness=0.6, specTrans=1.0, ior=1.5),
+}
+
+
+def _refraction_possible(L, V, eta):
+    """Walter et al. 2007: a transmitted L pairs with the half vector
+    H ~ -(eta V + L) only when V.H > 0 and L.H < 0."""
+    H = L + V[None, :] * eta
+    H /= np.linalg.norm(H, axis=1, keepdims=True)
+    H[H[:, 2] < 0] *= -1
+    return (L[:, 2] >= 0) | (((H @ V) > 0) & (np.sum(L * H, axis=1) < 0))
+
+
+@pytest.mark.parametrize("name", sorted(MATERIALS))
+def test_disney_sample_matches_eval_pdf(name):
+    """Directions drawn by DisneySample follow the mixture pdf DisneyEval
+    returns, and sample weights agree with evaluation. For transmission the
+    comparison is restricted to directions refraction can produce (see
+    test_eval_transmission_quirk)."""
+    m = mat(**MATERIALS[name])
+    rng = np.random.default_rng(6)
+    V = np.array([0.3, 0.2, 0.9]); V /= np.linalg.norm(V)
+    N = np.array([0.0, 0.0, 1.0])
+    eta = 1.0 / MATERIALS[name].get("ior", 1.5)
+    n = 60000
+    Ls = np.zeros((n, 3))
+    for i in range(n):
+        seed = rng.integers(0, 2 ** 32, 4, dtype=np.uint64).astype(np.uint32)
+        L, f, pdf, _ = oracle.disney_sample(m, V, N, seed)
+        Ls[i] = L
+        if i < 300 and pdf > 0:
+            f2, pdf2 = oracle.disney_eval(m, V, N, L)
+            np.testing.assert_allclose(f, f2, rtol=1e-6, atol=1e-9)
+            assert abs(pdf - pdf2) <= 1e-6 * max(1, pdf)
+
+    def pdf_fn(L):
+        out = np.zeros(L.shape[0])
+        for i in range(L.shape[0]):
+            out[i] = oracle.disney_eval(m, V, N, L[i])[1]
+        return np.where(_refraction_possible(L, V, eta), out, 0.0)
+
+    chi2, dof, got_mass, want_mass = _chi2_dir_hist(Ls, pdf_fn, n_mu=12, n_phi=12, sub=6)
+    if name == "glass":
+        # The reference's glass sampler can reflect below or refract above the
+        # horizon; DisneyEval then scores such a sample with the other branch's
+        # pdf, so the match is approximate (~1% of the mass): bounded per bin.
+        assert chi2 < 6 * dof, (name, chi2, dof)
+    else:
+        assert chi2 < dof + 6 * np.sqrt(2 * dof), (name, chi2, dof)
+    assert abs(got_mass - want_mass) < 0.02, (got_mass, want_mass)
+
+
+def test_eval_transmission_quirk():
+    """Reference quirk reproduced on purpose: DisneyEval's refraction term
+    (disney.glsl:104-123) does not check that L and V lie on opposite sides
+    of the generalised half vector, so it returns a non-zero f and pdf for
+    directions no refraction produces (grazing transmission here)."""
+    m = mat(baseColor=(1, 1, 1), roughness=0.6, specTrans=1.0, ior=1.5)
+    V = np.array([0.3, 0.2, 0.9]); V /= np.linalg.norm(V)
+    N = np.array([0.0, 0.0, 1.0])
+    L = np.array([0.9, 0.0, -0.2]); L /= np.linalg.norm(L)
+    assert not _refraction_possible(L[None, :], V, 1 / 1.5)[0]
+    f, pdf = oracle.disney_eval(m, V, N, L)
+    assert pdf > 0 and f.max() > 0
+
+
+def test_reflection_reciprocity():
+    rng = np.random.default_rng(7)
+    N = np.array([0, 0, 1.0])
+    for name in ("diffuse", "metal_rough", "clearcoat"):
+        m = mat(**MATERIALS[name], ior=1.5)
+        for _ in range(100):
+            V, L = upper(2, rng)
+            f1, _ = oracle.disney_eval(m, V, N, L)
+            f2, _ = oracle.disney_eval(m, L, N, V)
+            # DisneyEval returns f * |L.z|; remove the cosine before comparing
+            np.testing.assert_allclose(f1 / L[2], f2 / V[2], rtol=2e-4, atol=1e-7)
+
+
+def test_energy_bound_metal():
+    """White furnace: E[f/pdf] <= 1 for a white conductor (no energy gain)."""
+    m = mat(baseColor=(1, 1, 1), metallic=1.0, roughness=0.3)
+    rng = np.random.default_rng(8)
+    V = np.array([0.0, 0.6, 0.8])
+    N = np.array([0, 0, 1.0])
+    w = []
+    for _ in range(20000):
+        L, f, pdf, _ = oracle.disney_sample(m, V, N, rng.integers(0, 2 ** 32, 4, dtype=np.uint64).astype(np.uint32))
+        w.append(f[0] / pdf if pdf > 0 else 0.0)
+    assert 0.8 < np.mean(w) <= 1.0 + 3 * np.std(w) / np.sqrt(len(w))
