@@ -114,9 +114,10 @@ def main():
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--plain-steps", type=int, default=None)
-    ap.add_argument("--wave-spp", type=int, default=4)
-    ap.add_argument("--block-size", type=int, default=256)
-    ap.add_argument("--stack-depth", type=int, default=24)
+    ap.add_argument("--wave-spp", type=int, default=None)
+    ap.add_argument("--block-size", type=int, default=None)
+    ap.add_argument("--stack-depth", type=int, default=None)
+    ap.add_argument("--compact-every", type=int, default=None)
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-count", action="store_true")
@@ -135,7 +136,14 @@ def main():
     ntri_inst = 0
     r = Renderer(0)
     r.upload_scene(sc)
-    opts_kw = dict(block_size=args.block_size, stack_depth=args.stack_depth, wave_spp=args.wave_spp)
+    # launch configuration: explicit flags, else the autotuner's stored choice
+    from glslpt.autotune import tuned
+    cfg = dict(block_size=256, stack_depth=24, wave_spp=4, compact_every=1)
+    cfg.update(tuned("c2") or {})
+    for k in cfg:
+        if getattr(args, k) is not None:
+            cfg[k] = getattr(args, k)
+    opts_kw = dict(cfg)
     samples_per_step = rs.width * rs.height * args.spp
 
     ms_w, st_w = run_mode(r, sc, rs, PT_MODE_WAVEFRONT, args.steps, args.warmup, args.spp, opts_kw)
@@ -169,6 +177,9 @@ def main():
                                 "tris": cw["tris_tested"] / max(1, cw["rays_closest"]),
                                 "instances": cw["instances_entered"] / max(1, cw["rays_closest"])},
                     "rays_per_sample": cw["rays_closest"] / (rs.width * rs.height * cspp),
+                    "shadow_per_ray": {"nodes": cw["shadow_nodes"] / max(1, cw["rays_shadow"]),
+                                       "tris": cw["shadow_tris"] / max(1, cw["rays_shadow"]),
+                                       "instances": cw["shadow_instances"] / max(1, cw["rays_shadow"])},
                     "shadow_rays_per_sample": cw["rays_shadow"] / (rs.width * rs.height * cspp),
                     "stack_spills": cw["stack_spills"]}
     r.close()
@@ -184,8 +195,7 @@ def main():
         "config": {"workload": "64 instances of a 15680-triangle noise-displaced icosphere (1003520 instanced "
                                "triangles), 16 Disney materials, 2 rect + 1 sphere light + constant environment, "
                                f"{rs.width}x{rs.height}, {args.spp} spp, max depth 8, RR from depth 3",
-                   "renderer": "wavefront", "wave_spp": args.wave_spp, "block_size": args.block_size,
-                   "stack_depth": args.stack_depth, "samples_per_step": samples_per_step,
+                   "renderer": "wavefront", "launch": cfg, "samples_per_step": samples_per_step,
                    "unique_triangles": int(a["vert_indices"].shape[0]), "instances": int(a["transforms"].shape[0])},
         "stage_ms_per_step": {k: v / args.steps for k, v in st_w["ms_stage"].items() if v > 0},
         "plain": plain,

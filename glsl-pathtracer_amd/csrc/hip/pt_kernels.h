@@ -42,7 +42,7 @@ void launch_extend(hipStream_t s, int blocks, int bs, const ptd::DevScene& sc, c
                    unsigned long long* counters, int* ovf, bool count);
 void launch_shade(hipStream_t s, int blocks, int bs, const ptd::DevScene& sc, const ptd::DevOpts& op,
                   const ptd::PathBuf& pb, const ptd::ShadowBuf& sb, bool primary, const int* hitq, const int* hit_count,
-                  int npix, int first_sample, int* qout, int* count_out, int* shadow_count, int* work);
+                  int npix, int first_sample, int* qout, int* count_out, int* shadow_count, int* work, bool compact);
 void launch_shadow(hipStream_t s, int blocks, int bs, const ptd::DevScene& sc, const ptd::DevOpts& op,
                    const ptd::PathBuf& pb, const ptd::ShadowBuf& sb, const int* count_in, int* work, int S,
                    int* spill, unsigned long long* counters, int* ovf, bool count);

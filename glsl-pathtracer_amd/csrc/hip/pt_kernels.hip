@@ -503,6 +503,9 @@ __device__ __forceinline__ void store_path(const PathBuf& pb, int slot, const Pa
     if (pb.med) pb.med[slot] = F4(p.med.color, p.med.density);
 }
 
+// meta.y flag of a path that ended in a bounce whose queue was not compacted.
+constexpr int PATH_DEAD = 1 << 18;
+
 // Camera ray and initial state of a primary path (tile.glsl:41-69); the
 // bounce-0 stages recompute it instead of storing it.
 __device__ __forceinline__ void primary_path(const DevOpts& op, int px, int py, int sample, PathRegs& p) {
@@ -540,9 +543,13 @@ __global__ void __launch_bounds__(256) k_extend(DevScene sc, DevOpts op, PathBuf
                 valid = slot_pixel(op, slot, npix, s, px, py);
                 if (valid) primary_path(op, px, py, first_sample + s, p);
             } else {
-                const float4 o4 = pb.ro[slot], d4 = pb.rd[slot];
-                p.ro = xyz(o4); p.rd = xyz(d4);
-                p.depth = pb.meta[slot].x;
+                const int2 mt = *reinterpret_cast<const int2*>(&pb.meta[slot]);
+                valid = !(mt.y & PATH_DEAD);  // ended path left in an uncompacted queue
+                if (valid) {
+                    const float4 o4 = pb.ro[slot], d4 = pb.rd[slot];
+                    p.ro = xyz(o4); p.rd = xyz(d4);
+                    p.depth = mt.x;
+                }
             }
             if (valid) {
                 Hit h = closest<COUNT>(sc, op, p.ro, p.rd, p.depth, stk, cnt, ovf);
@@ -554,6 +561,7 @@ __global__ void __launch_bounds__(256) k_extend(DevScene sc, DevOpts op, PathBuf
                     }
                     on_miss(sc, op, p);
                     pb.rad[slot] = F4(p.rad, p.alpha);
+                    if (!PRIMARY) pb.meta[slot].y = PATH_DEAD;
                 } else {
                     pb.hit[slot] = make_float4(h.t, h.u, h.v, __int_as_float(h.tri));
                     pb.hit2[slot] = make_int2(h.inst, h.light);
@@ -575,7 +583,7 @@ template <bool PRIMARY>
 __global__ void __launch_bounds__(256) k_shade(DevScene sc, DevOpts op, PathBuf pb, ShadowBuf sb,
                                                const int* __restrict__ hitq, const int* hit_count, int npix,
                                                int first_sample, int* qout, int* count_out, int* shadow_count,
-                                               int* work) {
+                                               int* work, int compact) {
     const int count = *hit_count;
     for (int base = first_batch(); base < count; base = next_batch(work)) {
         const int i = base + lane_id();
@@ -612,8 +620,13 @@ __global__ void __launch_bounds__(256) k_shade(DevScene sc, DevOpts op, PathBuf 
                 alive = bounce_post(op, p, st, m, eta, ev);
                 if (alive) p.depth++;
             }
-            if (alive) store_path(pb, slot, p);
-            else pb.rad[slot] = F4(p.rad, p.alpha);
+            if (alive) {
+                store_path(pb, slot, p);
+            } else {
+                pb.rad[slot] = F4(p.rad, p.alpha);
+                // without compaction the path stays in the queue: mark it ended
+                if (!compact) pb.meta[slot].y = PATH_DEAD;
+            }
             keep = alive;
         }
         // NEE rays: contribution * throughput at NEE time, added if unoccluded.
@@ -626,8 +639,10 @@ __global__ void __launch_bounds__(256) k_shade(DevScene sc, DevOpts op, PathBuf 
                 sb.c[pos] = F4(c, 0.f);
             }
         }
-        const int pos = wave_append(keep, count_out);
-        if (pos >= 0) qout[pos] = slot;
+        if (compact) {
+            const int pos = wave_append(keep, count_out);
+            if (pos >= 0) qout[pos] = slot;
+        }
     }
 }
 
@@ -823,9 +838,10 @@ void launch_extend(hipStream_t s, int blocks, int bs, const DevScene& sc, const 
 
 void launch_shade(hipStream_t s, int blocks, int bs, const DevScene& sc, const DevOpts& op, const PathBuf& pb,
                   const ShadowBuf& sb, bool primary, const int* hitq, const int* hit_count, int npix, int first_sample,
-                  int* qout, int* count_out, int* shadow_count, int* work) {
-    if (primary) hipLaunchKernelGGL(k_shade<true>, dim3(blocks), dim3(bs), 0, s, sc, op, pb, sb, hitq, hit_count, npix, first_sample, qout, count_out, shadow_count, work);
-    else hipLaunchKernelGGL(k_shade<false>, dim3(blocks), dim3(bs), 0, s, sc, op, pb, sb, hitq, hit_count, npix, first_sample, qout, count_out, shadow_count, work);
+                  int* qout, int* count_out, int* shadow_count, int* work, bool compact) {
+    const int c = compact ? 1 : 0;
+    if (primary) hipLaunchKernelGGL(k_shade<true>, dim3(blocks), dim3(bs), 0, s, sc, op, pb, sb, hitq, hit_count, npix, first_sample, qout, count_out, shadow_count, work, c);
+    else hipLaunchKernelGGL(k_shade<false>, dim3(blocks), dim3(bs), 0, s, sc, op, pb, sb, hitq, hit_count, npix, first_sample, qout, count_out, shadow_count, work, c);
 }
 
 void launch_shadow(hipStream_t s, int blocks, int bs, const DevScene& sc, const DevOpts& op, const PathBuf& pb,

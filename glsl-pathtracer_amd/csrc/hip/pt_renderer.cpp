@@ -495,24 +495,33 @@ void render_wavefront(pt_renderer_s* h, int first_sample, int num_samples) {
         ptk::launch_fill(s, cnt, counter_cells(D), 0);
         int* qin = h->qa.p;
         int* qout = h->qb.p;
+        int* c_ext = cnt;  // count of the current extend queue
+        const int cadence = std::max(1, h->opts.compact_every);
         for (int b = 0; b < D; ++b) {
-            int* c_ext = cnt + b;
-            int* c_next = cnt + b + 1;
             int* c_hit = cnt + (D + 1) + b;
             int* c_sh = cnt + 2 * (D + 1) + b;
             int* w = cnt + 3 * (D + 1) + 3 * b;
             const bool primary = b == 0;
+            // bounce 0 always compacts (its input is the implicit slot range);
+            // later bounces compact every `cadence`-th time, otherwise the next
+            // extend re-reads this queue and skips ended paths.
+            const bool compact = primary || (b % cadence) == 0;
+            int* c_next = compact ? cnt + b + 1 : c_ext;
             h->stage_begin(PT_STAGE_EXTEND, ev);
             ptk::launch_extend(s, b_ext, bs, h->sc, op, pb, primary, qin, c_ext, npaths, npix, fs, h->hq.p, c_hit, w + 0,
                                S, h->spill.p, h->work_counts.p, h->ovf.p, cw);
             h->stage_end(PT_STAGE_EXTEND, ev);
             h->stage_begin(PT_STAGE_SHADE, ev);
-            ptk::launch_shade(s, b_sha, bs, h->sc, op, pb, sb, primary, h->hq.p, c_hit, npix, fs, qout, c_next, c_sh, w + 1);
+            ptk::launch_shade(s, b_sha, bs, h->sc, op, pb, sb, primary, h->hq.p, c_hit, npix, fs, qout, c_next, c_sh, w + 1,
+                              compact);
             h->stage_end(PT_STAGE_SHADE, ev);
             h->stage_begin(PT_STAGE_SHADOW, ev);
             ptk::launch_shadow(s, b_sdw, bs, h->sc, op, pb, sb, c_sh, w + 2, S, h->spill.p, h->work_counts.p + 8, h->ovf.p, cw);
             h->stage_end(PT_STAGE_SHADOW, ev);
-            std::swap(qin, qout);
+            if (compact) {
+                std::swap(qin, qout);
+                c_ext = c_next;
+            }
         }
         h->stage_begin(PT_STAGE_ACCUM, ev);
         ptk::launch_accum(s, op, pb, h->accum.p, npix, ns);
@@ -559,6 +568,7 @@ void pt_default_options(pt_options* o) {
     o->block_size = 256;
     o->stack_depth = 24;
     o->wave_spp = 1;
+    o->compact_every = 1;
 }
 
 int pt_device_count(int32_t* n) {
@@ -720,6 +730,7 @@ int pt_set_options(pt_handle h, const pt_options* o) {
             return Fail(h, PT_ERR_INVALID_ARG, "block_size must be 64, 128 or 256");
         if (o->stack_depth < 0 || o->stack_depth > ptd::MAXD) return Fail(h, PT_ERR_INVALID_ARG, "stack_depth out of range");
         if (o->mode != PT_MODE_PLAIN && o->mode != PT_MODE_WAVEFRONT) return Fail(h, PT_ERR_INVALID_ARG, "unknown mode");
+        if (o->compact_every < 0 || o->compact_every > 64) return Fail(h, PT_ERR_INVALID_ARG, "compact_every out of range");
         if (static_cast<long long>(o->width) * o->height > (1LL << 28)) return Fail(h, PT_ERR_INVALID_ARG, "resolution too large");
         HIPCHK(hipSetDevice(h->device));
         h->opts = *o;

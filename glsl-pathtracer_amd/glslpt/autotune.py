@@ -1,0 +1,84 @@
+"""Launch-configuration autotuner for the wavefront renderer.
+
+Sweeps workgroup size {64, 128, 256}, LDS traversal-stack depth {16, 24, 32},
+samples per wavefront pass and queue-compaction cadence on the actual scene,
+timing short renders on the GPU, and stores the fastest configuration as JSON
+in glslpt/tuning/<key>.json, where bench.py and callers look it up.
+
+    python -m glslpt.autotune c2 [spp] [--quick]   # the 1080p instanced-spheres workload
+"""
+import itertools
+import json
+import os
+import sys
+import time
+
+from . import _capi as api
+from .renderer import Renderer, options_for
+
+TUNING_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning")
+
+BLOCK_SIZES = (64, 128, 256)
+STACK_DEPTHS = (16, 24, 32)
+QUICK = dict(block_size=(64, 128), stack_depth=(16, 24), wave_spp=(16, 32, 64), compact_every=(1, 2))
+WAVE_SPP = (8, 16, 32, 64)
+CADENCE = (1, 2)
+
+
+def tuned(key):
+    """The stored configuration for `key`, or None."""
+    path = os.path.join(TUNING_DIR, f"{key}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f)["best"]
+
+
+def _time(r, sc, rs, spp, reps, **kw):
+    r.set_options(options_for(rs, mode=api.PT_MODE_WAVEFRONT, **kw))
+    r.set_camera(sc.camera())
+    r.clear()
+    r.render(0, spp)  # warm-up (allocations, first touch)
+    r.synchronize()
+    t0 = time.perf_counter()
+    for i in range(reps):
+        r.render((i + 1) * spp, spp)
+    r.synchronize()
+    return (time.perf_counter() - t0) / reps
+
+
+def sweep(r, sc, rs, spp=16, reps=2, grid=None, log=print):
+    grid = grid or dict(block_size=BLOCK_SIZES, stack_depth=STACK_DEPTHS, wave_spp=WAVE_SPP, compact_every=CADENCE)
+    keys = list(grid)
+    results = []
+    for vals in itertools.product(*[grid[k] for k in keys]):
+        cfg = dict(zip(keys, vals))
+        if cfg.get("wave_spp", 1) > spp:
+            continue
+        dt = _time(r, sc, rs, spp, reps, **cfg)
+        msps = rs.width * rs.height * spp / dt / 1e6
+        results.append({"config": cfg, "ms": dt * 1e3, "msamples_per_s": msps})
+        log(json.dumps(results[-1]))
+    results.sort(key=lambda x: -x["msamples_per_s"])
+    return results
+
+
+def main(argv):
+    if len(argv) < 1 or argv[0] not in ("c2",):
+        sys.exit("usage: python -m glslpt.autotune c2 [spp]")
+    spp = int(argv[1]) if len(argv) > 1 and argv[1].isdigit() else 64
+    grid = QUICK if "--quick" in argv else None
+    from .scenes import instanced_spheres
+    sc, rs = instanced_spheres(num_instances=64, freq=28, seed=0x5EED0002, width=1920, height=1080, spp=64, max_depth=8)
+    r = Renderer(0)
+    r.upload_scene(sc)
+    res = sweep(r, sc, rs, spp=spp, reps=1, grid=grid)
+    os.makedirs(TUNING_DIR, exist_ok=True)
+    out = {"key": "c2", "spp_timed": spp, "best": res[0]["config"], "results": res}
+    with open(os.path.join(TUNING_DIR, "c2.json"), "w") as f:
+        json.dump(out, f, indent=1)
+    print("best", json.dumps(res[0]))
+
+
+if __name__ == "__main__":
+    main(sys.argv[1:])

@@ -136,6 +136,7 @@ typedef struct pt_options {
     int32_t wave_spp;        /* samples per wavefront pass */
     int32_t use_graph;       /* capture the bounce stages of a pass in a HIP graph */
     int32_t count_work;      /* instrumented kernels: count nodes / triangles / instances */
+    int32_t compact_every;   /* compact the path queue every k-th bounce (1 = every bounce) */
 } pt_options;
 
 typedef struct pt_stats {

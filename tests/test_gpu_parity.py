@@ -118,12 +118,16 @@ def test_render_instanced(renderer, mode):
     _render_parity(renderer, sc, rs, mode, 4, f"instanced mode={mode}")
 
 
-def test_plain_equals_wavefront(renderer):
+@pytest.mark.parametrize("cadence,bs,stack", [(1, 256, 24), (2, 128, 16), (3, 64, 32), (1, 256, 2)])
+def test_plain_equals_wavefront(renderer, cadence, bs, stack):
+    """Launch configuration (workgroup size, LDS stack depth with spills to
+    the global backing store, compaction cadence) never changes the image."""
     sc, rs = small_instanced()
     out = []
     for mode in (PT_MODE_PLAIN, PT_MODE_WAVEFRONT):
         renderer.upload_scene(sc)
-        renderer.set_options(options_for(rs, mode=mode, wave_spp=3))
+        renderer.set_options(options_for(rs, mode=mode, wave_spp=3, compact_every=cadence, block_size=bs,
+                                         stack_depth=stack))
         renderer.set_camera(sc.camera())
         renderer.clear()
         renderer.render(0, 6)
