@@ -105,6 +105,18 @@ def cpu_baseline(sc, rs, target_s):
             "sample": f"{rows} rows (every {step}th) x {rs.width} px x {spp} spp of the same frame, {dt:.1f} s"}
 
 
+def measured_traffic(cfg):
+    """HBM bytes per k_extend dispatch from the committed rocprofv3 counter
+    runs (profiles/extend_c2_1080p_traffic.*), when they were taken with the
+    same launch configuration; else None."""
+    path = os.path.join(ROOT, "profiles", "extend_c2_1080p_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        t = json.load(f)
+    return t["extend_hbm_bytes_per_dispatch"] if t.get("launch") == cfg else None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -170,7 +182,7 @@ def main():
         avg_ms = extend_ms / extend_launches
         achieved = per_launch / (avg_ms / 1e3) / 1e9
         roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                    "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(cfg),
                     "kernel": "k_extend (closest hit)", "bytes_per_launch": per_launch,
                     "avg_launch_ms": avg_ms, "launches": extend_launches,
                     "per_ray": {"nodes": cw["nodes_visited"] / max(1, cw["rays_closest"]),

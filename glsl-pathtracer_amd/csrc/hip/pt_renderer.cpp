@@ -579,14 +579,29 @@ int pt_device_count(int32_t* n) {
     return PT_OK;
 }
 
+// Message of the last failure that had no handle to hold it (pt_create).
+static thread_local std::string g_create_err = "no error";
+
 int pt_create(int32_t device, pt_handle* out) {
-    if (!out) return PT_ERR_INVALID_ARG;
+    if (!out) {
+        g_create_err = "pt_create: null output pointer";
+        return PT_ERR_INVALID_ARG;
+    }
     *out = nullptr;
     int n = 0;
-    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return PT_ERR_NO_DEVICE;
-    if (device < 0 || device >= n) return PT_ERR_INVALID_ARG;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
+        g_create_err = "pt_create: no HIP device visible";
+        return PT_ERR_NO_DEVICE;
+    }
+    if (device < 0 || device >= n) {
+        g_create_err = "pt_create: device " + std::to_string(device) + " out of range (" + std::to_string(n) + " visible)";
+        return PT_ERR_INVALID_ARG;
+    }
     pt_renderer_s* h = new (std::nothrow) pt_renderer_s;
-    if (!h) return PT_ERR_OVERFLOW;
+    if (!h) {
+        g_create_err = "pt_create: out of host memory";
+        return PT_ERR_OVERFLOW;
+    }
     try {
         h->device = device;
         HIPCHK(hipSetDevice(device));
@@ -597,7 +612,8 @@ int pt_create(int32_t device, pt_handle* out) {
         HIPCHK(hipEventCreate(&h->ev_r0));
         HIPCHK(hipEventCreate(&h->ev_r1));
         pt_default_options(&h->opts);
-    } catch (const std::exception&) {
+    } catch (const std::exception& e) {
+        g_create_err = std::string("pt_create: ") + e.what();
         delete h;
         return PT_ERR_HIP;
     }
@@ -612,7 +628,8 @@ int pt_destroy(pt_handle h) {
     return PT_OK;
 }
 
-const char* pt_last_error(pt_handle h) { return h ? h->err.c_str() : "null renderer handle"; }
+// A null handle reports the last pt_create failure of this thread.
+const char* pt_last_error(pt_handle h) { return h ? h->err.c_str() : g_create_err.c_str(); }
 
 int pt_upload_scene(pt_handle h, const pt_scene_view* v) {
     return Guard(h, [&] {
