@@ -1,7 +1,7 @@
 // Launch interface between the renderer (host) and the kernels.
 #pragma once
 
-#include "pt_device.h"
+#include "pt_adjoint.h"
 
 namespace ptd {
 
@@ -27,12 +27,25 @@ struct ShadowBuf {
     float4* c;  // contribution (already times throughput)
 };
 
+// Adjoint (replay pass) inputs and outputs.
+struct AdjBuf {
+    const float4* ltot;   // primal radiance of every path slot of the pass
+    const float* dimg;    // dLoss / d(mean image), RGB per pixel, rows bottom-up
+    float inv_n;          // 1 / number of samples in the mean image
+    float* grad;          // [num_mats * 7] gradient sums
+    float4* sg;           // per shadow ray: 2 float4 = 7 gradient terms, material id (int bits)
+    int num_mats;
+    int lds;              // 1: accumulate per block in LDS
+};
+
 }  // namespace ptd
 
 namespace ptk {
-enum { KIND_PLAIN = 0, KIND_EXTEND = 1, KIND_SHADE = 2, KIND_SHADOW = 3 };
+enum { KIND_PLAIN = 0, KIND_EXTEND = 1, KIND_SHADE = 2, KIND_SHADOW = 3, KIND_SHADE_ADJ = 4, KIND_SHADOW_ADJ = 5 };
 
-int persistent_blocks(int kind, int bs, int S, bool count, int num_cu);
+// Resident blocks for a persistent launch (occupancy x CUs); extra_lds is the
+// adjoint's per-block gradient table.
+int persistent_blocks(int kind, int bs, int S, bool count, int num_cu, size_t extra_lds = 0);
 void launch_plain(hipStream_t s, int blocks, int bs, const ptd::DevScene& sc, const ptd::DevOpts& op, float4* accum,
                   int sample, int num_items, int* work, int S, int* spill, unsigned long long* counters, int* ovf,
                   bool count);
@@ -42,10 +55,11 @@ void launch_extend(hipStream_t s, int blocks, int bs, const ptd::DevScene& sc, c
                    unsigned long long* counters, int* ovf, bool count);
 void launch_shade(hipStream_t s, int blocks, int bs, const ptd::DevScene& sc, const ptd::DevOpts& op,
                   const ptd::PathBuf& pb, const ptd::ShadowBuf& sb, bool primary, const int* hitq, const int* hit_count,
-                  int npix, int first_sample, int* qout, int* count_out, int* shadow_count, int* work, bool compact);
+                  int npix, int first_sample, int* qout, int* count_out, int* shadow_count, int* work, bool compact,
+                  const ptd::AdjBuf* ad = nullptr);
 void launch_shadow(hipStream_t s, int blocks, int bs, const ptd::DevScene& sc, const ptd::DevOpts& op,
                    const ptd::PathBuf& pb, const ptd::ShadowBuf& sb, const int* count_in, int* work, int S,
-                   int* spill, unsigned long long* counters, int* ovf, bool count);
+                   int* spill, unsigned long long* counters, int* ovf, bool count, const ptd::AdjBuf* ad = nullptr);
 void launch_accum(hipStream_t s, const ptd::DevOpts& op, const ptd::PathBuf& pb, float4* accum, int npix, int nsamp);
 void launch_fill(hipStream_t s, int* p, int n, int v);
 void launch_query_closest(hipStream_t s, const ptd::DevScene& sc, const ptd::DevOpts& op, const float* o,

@@ -149,16 +149,18 @@ struct PathRegs {
 
 // Result of DirectLight's sampling step; the visibility test is separate so
 // the wavefront form can defer it to the shadow stage.
-struct NeeRay { vec3 o, d, contrib; float maxd; bool valid; };
+struct NeeRay { vec3 o, d, contrib, w; float maxd; bool valid; };
 
 // The sampling and BSDF part of DirectLight for one light kind. kind 0 =
 // environment (SampleEnvMap), 1 = analytic light. RNG draws happen in the
-// reference order; contrib excludes the final `* throughput`.
+// reference order; contrib excludes the final `* throughput`; w is contrib
+// without the BSDF value (the factor the adjoint multiplies df/dparam by).
 __device__ NeeRay nee_setup(const DevScene& sc, const DevOpts& op, int kind, vec3 rd_in, const SurfState& st,
                             const Material& m, float eta, Rng& rng) {
     NeeRay r;
     r.valid = false;
     r.contrib = V1(0.0f);
+    r.w = V1(0.0f);
     vec3 scatterPos = st.fhp + st.normal * EPS;
     r.o = scatterPos;
     if (kind == 0) {
@@ -174,6 +176,7 @@ __device__ NeeRay nee_setup(const DevScene& sc, const DevOpts& op, int kind, vec
             float mis = PowerHeuristic(lightPdf, pdf);
             if (mis > 0.0f) {
                 r.contrib = mis * Li * f * op.env_intensity / lightPdf;
+                r.w = mis * Li * op.env_intensity / lightPdf;
                 r.valid = true;
             }
         }
@@ -194,6 +197,7 @@ __device__ NeeRay nee_setup(const DevScene& sc, const DevOpts& op, int kind, vec
         if (L.e.w > 0.0f) mis = PowerHeuristic(ls.pdf, pdf);
         if (pdf > 0.0f) {
             r.contrib = mis * Li * f / ls.pdf;
+            r.w = mis * Li / ls.pdf;
             r.valid = true;
         }
     }
@@ -341,9 +345,13 @@ __device__ int bounce_pre(const DevScene& sc, const DevOpts& op, const Hit& h, P
     return EV_SURFACE;
 }
 
+// BSDF sample of a SURFACE event, reported to the adjoint.
+struct BsdfOut { vec3 f, L; float pdf; };
+
 // Bounce body after NEE: BSDF or phase sampling, medium bookkeeping, Russian
 // roulette (pathtrace.glsl:392-471). Returns false when the path ends.
-__device__ bool bounce_post(const DevOpts& op, PathRegs& p, const SurfState& st, const Material& m, float eta, int ev) {
+__device__ bool bounce_post(const DevOpts& op, PathRegs& p, const SurfState& st, const Material& m, float eta, int ev,
+                            BsdfOut* bo = nullptr) {
     if (ev == EV_MEDIUM) {
         float ra = rnd(p.rng);
         float rb = rnd(p.rng);
@@ -358,6 +366,7 @@ __device__ bool bounce_post(const DevOpts& op, PathRegs& p, const SurfState& st,
         } else {
             float pdf;
             vec3 f = DisneySample(m, eta, -p.rd, st.ffnormal, p.rng, L, pdf);
+            if (bo) { bo->f = f; bo->L = L; bo->pdf = pdf; }
             p.prev_pdf = pdf;
             if (pdf > 0.0f) p.thr = p.thr * (f / pdf);
             else return false;
@@ -576,14 +585,92 @@ __global__ void __launch_bounds__(256) k_extend(DevScene sc, DevOpts op, PathBuf
     if (ovf) atomicOr(ovf_flag, 1);
 }
 
+// Gradient accumulators of one block: per-material parameter sums in LDS
+// (flushed with one global atomic per non-zero entry at the end), or direct
+// global atomics when the material table does not fit.
+struct GradAcc {
+    float* lds;   // null: global atomics
+    float* glob;
+    int n;        // num_mats * NPARAM
+    __device__ __forceinline__ void init() {
+        if (!lds) return;
+        for (int i = threadIdx.x; i < n; i += blockDim.x) lds[i] = 0.0f;
+        __syncthreads();
+    }
+    __device__ __forceinline__ void add(int mat, const float g[NPARAM]) {
+        if (lds) {
+#pragma unroll
+            for (int k = 0; k < NPARAM; ++k)
+                if (g[k] != 0.0f)
+                    __hip_atomic_fetch_add(lds + mat * NPARAM + k, g[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        } else {
+#pragma unroll
+            for (int k = 0; k < NPARAM; ++k)
+                if (g[k] != 0.0f) atomicAdd(glob + mat * NPARAM + k, g[k]);
+        }
+    }
+    __device__ __forceinline__ void flush() {
+        if (!lds) return;
+        __syncthreads();
+        for (int i = threadIdx.x; i < n; i += blockDim.x)
+            if (lds[i] != 0.0f) atomicAdd(glob + i, lds[i]);
+    }
+};
+
+// Path-replay adjoint of one SURFACE vertex (detached sampling):
+//   BSDF-sample term  sum_c delta_c * Lrem_c / f_c * df_c/dparam, where Lrem is
+//     the primal radiance still to come after this vertex's emission,
+//     assuming the NEE rays are occluded;
+//   per NEE ray (applied by the shadow stage if unoccluded)
+//     sum_c delta_c * thr_c * w_c * df_c(light dir)/dparam
+//     - sum_c delta_c * C_c / f_c * df_c/dparam   (C: the ray's radiance),
+// so that together the terms use the exact remaining radiance.
+__device__ __forceinline__ void adjoint_vertex(const DevScene& sc, const DevOpts& op, const AdjBuf& ad, int slot,
+                                               int npix, const SurfState& st, const Material& m, vec3 rd_in,
+                                               vec3 rad_pre, const BsdfOut& bo, const NeeRay nee[2], vec3 thr_nee,
+                                               float g[NPARAM], float gn[2][NPARAM]) {
+    int s, px, py;
+    slot_pixel(op, slot, npix, s, px, py);
+    const float* dp = ad.dimg + (static_cast<size_t>(py) * op.width + px) * 3;
+    const vec3 delta = V(dp[0], dp[1], dp[2]) * ad.inv_n;
+    const MatD md = seed_material(sc, st, m, rd_in);
+    const vec3 Vw = -rd_in;
+    const bool bsdf_ok = bo.pdf > 0.0f;
+    vec3 r = V1(0.0f);
+    dvec3 df = D3s(dc(0.0f));
+    if (bsdf_ok) {
+        r = V(bo.f.x != 0.0f ? delta.x / bo.f.x : 0.0f, bo.f.y != 0.0f ? delta.y / bo.f.y : 0.0f,
+              bo.f.z != 0.0f ? delta.z / bo.f.z : 0.0f);
+        const vec3 lrem = xyz(ad.ltot[slot]) - rad_pre;
+        df = DisneyEvalD(m, md, Vw, st.ffnormal, bo.L);
+        grad_dot(r * lrem, df, g);
+    }
+    for (int k = 0; k < 2; ++k) {
+        if (!nee[k].valid) continue;
+        const dvec3 dl = DisneyEvalD(m, md, Vw, st.ffnormal, nee[k].d);
+        grad_dot(delta * thr_nee * nee[k].w, dl, gn[k]);
+        if (bsdf_ok) {
+            float t[NPARAM];
+            grad_dot(r * (nee[k].contrib * thr_nee), df, t);
+#pragma unroll
+            for (int j = 0; j < NPARAM; ++j) gn[k][j] = gn[k][j] - t[j];
+        }
+    }
+}
+
 // Shade: one bounce of every path that hit something, except the NEE
 // visibility tests, which go to the shadow queue. Continuing paths are
 // compacted into the next extend queue; ending paths write only radiance.
-template <bool PRIMARY>
+// ADJ: the replay pass of the adjoint; same path decisions, plus parameter
+// gradients of every surface vertex.
+template <bool PRIMARY, bool ADJ>
 __global__ void __launch_bounds__(256) k_shade(DevScene sc, DevOpts op, PathBuf pb, ShadowBuf sb,
                                                const int* __restrict__ hitq, const int* hit_count, int npix,
                                                int first_sample, int* qout, int* count_out, int* shadow_count,
-                                               int* work, int compact) {
+                                               int* work, int compact, AdjBuf ad) {
+    extern __shared__ int lds[];
+    GradAcc acc{ADJ && ad.lds ? reinterpret_cast<float*>(lds) : nullptr, ad.grad, ad.num_mats * NPARAM};
+    if (ADJ) acc.init();
     const int count = *hit_count;
     for (int base = first_batch(); base < count; base = next_batch(work)) {
         const int i = base + lane_id();
@@ -592,6 +679,12 @@ __global__ void __launch_bounds__(256) k_shade(DevScene sc, DevOpts op, PathBuf 
         NeeRay nee[2];
         nee[0].valid = nee[1].valid = false;
         vec3 thr_nee = V1(0.f);
+        int gmat = -1;
+        float gn[2][NPARAM];
+        if (ADJ) {
+#pragma unroll
+            for (int k = 0; k < NPARAM; ++k) gn[0][k] = gn[1][k] = 0.0f;
+        }
         if (i < count) {
             slot = hitq[i];
             PathRegs p;
@@ -617,7 +710,18 @@ __global__ void __launch_bounds__(256) k_shade(DevScene sc, DevOpts op, PathBuf 
                     nee_rays(sc, op, p, p.rng, st, m, eta, nee);
                     thr_nee = p.thr;
                 }
-                alive = bounce_post(op, p, st, m, eta, ev);
+                const vec3 rd_in = p.rd;
+                BsdfOut bo;
+                bo.pdf = 0.0f;
+                alive = bounce_post(op, p, st, m, eta, ev, ADJ ? &bo : nullptr);
+                if (ADJ && ev == EV_SURFACE) {
+                    float g[NPARAM];
+#pragma unroll
+                    for (int k = 0; k < NPARAM; ++k) g[k] = 0.0f;
+                    adjoint_vertex(sc, op, ad, slot, npix, st, m, rd_in, p.rad, bo, nee, thr_nee, g, gn);
+                    gmat = st.matID;
+                    acc.add(gmat, g);
+                }
                 if (alive) p.depth++;
             }
             if (alive) {
@@ -637,6 +741,10 @@ __global__ void __launch_bounds__(256) k_shade(DevScene sc, DevOpts op, PathBuf 
                 sb.o[pos] = F4(nee[k].o, nee[k].maxd);
                 sb.d[pos] = F4(nee[k].d, __int_as_float(slot));
                 sb.c[pos] = F4(c, 0.f);
+                if (ADJ) {
+                    ad.sg[2 * pos] = make_float4(gn[k][0], gn[k][1], gn[k][2], gn[k][3]);
+                    ad.sg[2 * pos + 1] = make_float4(gn[k][4], gn[k][5], gn[k][6], __int_as_float(gmat));
+                }
             }
         }
         if (compact) {
@@ -644,18 +752,22 @@ __global__ void __launch_bounds__(256) k_shade(DevScene sc, DevOpts op, PathBuf 
             if (pos >= 0) qout[pos] = slot;
         }
     }
+    if (ADJ) acc.flush();
 }
 
 // Shadow: any-hit test of queued NEE rays; unoccluded ones add their
-// contribution to the path's radiance.
-template <bool COUNT>
+// contribution to the path's radiance (ADJ: and their parameter gradients).
+template <bool COUNT, bool ADJ>
 __global__ void __launch_bounds__(256) k_shadow(DevScene sc, DevOpts op, PathBuf pb, ShadowBuf sb, const int* count_in,
                                                 int* work, int S, int* spill, unsigned long long* counters,
-                                                int* ovf_flag) {
+                                                int* ovf_flag, AdjBuf ad) {
     extern __shared__ int lds[];
     Stack stk{lds, spill, static_cast<int>(threadIdx.x), static_cast<int>(blockDim.x), S,
               blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x,
               static_cast<size_t>(gridDim.x) * blockDim.x};
+    GradAcc acc{ADJ && ad.lds ? reinterpret_cast<float*>(lds + S * blockDim.x) : nullptr, ad.grad,
+                ad.num_mats * NPARAM};
+    if (ADJ) acc.init();
     TravCount cnt{0, 0, 0, 0, 0, 0};
     bool ovf = false;
     const int count = *count_in;
@@ -669,11 +781,20 @@ __global__ void __launch_bounds__(256) k_shadow(DevScene sc, DevOpts op, PathBuf
                 float4 r = pb.rad[slot];
                 r.x += c.x; r.y += c.y; r.z += c.z;
                 pb.rad[slot] = r;
+                if (ADJ) {
+                    const float4 g0 = ad.sg[2 * i], g1 = ad.sg[2 * i + 1];
+                    const int mat = __float_as_int(g1.w);
+                    if (mat >= 0) {
+                        const float g[NPARAM] = {g0.x, g0.y, g0.z, g0.w, g1.x, g1.y, g1.z};
+                        acc.add(mat, g);
+                    }
+                }
             }
         }
     }
     flush_counts<COUNT>(cnt, counters);
     if (ovf) atomicOr(ovf_flag, 1);
+    if (ADJ) acc.flush();
 }
 
 // Accumulate: accum[pixel] += radiance of samples s = 0..nsamp-1 in order.
@@ -805,14 +926,16 @@ static int max_blocks(K kernel, int bs, size_t lds) {
     return n < 1 ? 1 : n;
 }
 
-int persistent_blocks(int kind, int bs, int S, bool count, int num_cu) {
+int persistent_blocks(int kind, int bs, int S, bool count, int num_cu, size_t extra_lds) {
     const size_t lds = static_cast<size_t>(S) * bs * sizeof(int);
     int per_cu = 1;
     switch (kind) {
         case KIND_PLAIN: per_cu = count ? max_blocks(k_plain<true>, bs, lds) : max_blocks(k_plain<false>, bs, lds); break;
         case KIND_EXTEND: per_cu = count ? max_blocks(k_extend<true, false>, bs, lds) : max_blocks(k_extend<false, false>, bs, lds); break;
-        case KIND_SHADOW: per_cu = count ? max_blocks(k_shadow<true>, bs, lds) : max_blocks(k_shadow<false>, bs, lds); break;
-        case KIND_SHADE: per_cu = max_blocks(k_shade<false>, bs, 0); break;
+        case KIND_SHADOW: per_cu = count ? max_blocks(k_shadow<true, false>, bs, lds) : max_blocks(k_shadow<false, false>, bs, lds); break;
+        case KIND_SHADE: per_cu = max_blocks(k_shade<false, false>, bs, 0); break;
+        case KIND_SHADE_ADJ: per_cu = max_blocks(k_shade<false, true>, bs, extra_lds); break;
+        case KIND_SHADOW_ADJ: per_cu = max_blocks(k_shadow<false, true>, bs, lds + extra_lds); break;
         default: break;
     }
     return per_cu * num_cu;
@@ -838,18 +961,32 @@ void launch_extend(hipStream_t s, int blocks, int bs, const DevScene& sc, const 
 
 void launch_shade(hipStream_t s, int blocks, int bs, const DevScene& sc, const DevOpts& op, const PathBuf& pb,
                   const ShadowBuf& sb, bool primary, const int* hitq, const int* hit_count, int npix, int first_sample,
-                  int* qout, int* count_out, int* shadow_count, int* work, bool compact) {
+                  int* qout, int* count_out, int* shadow_count, int* work, bool compact, const AdjBuf* ad) {
     const int c = compact ? 1 : 0;
-    if (primary) hipLaunchKernelGGL(k_shade<true>, dim3(blocks), dim3(bs), 0, s, sc, op, pb, sb, hitq, hit_count, npix, first_sample, qout, count_out, shadow_count, work, c);
-    else hipLaunchKernelGGL(k_shade<false>, dim3(blocks), dim3(bs), 0, s, sc, op, pb, sb, hitq, hit_count, npix, first_sample, qout, count_out, shadow_count, work, c);
+    if (ad) {
+        const size_t lds = ad->lds ? static_cast<size_t>(ad->num_mats) * NPARAM * sizeof(float) : 0;
+        if (primary) hipLaunchKernelGGL((k_shade<true, true>), dim3(blocks), dim3(bs), lds, s, sc, op, pb, sb, hitq, hit_count, npix, first_sample, qout, count_out, shadow_count, work, c, *ad);
+        else hipLaunchKernelGGL((k_shade<false, true>), dim3(blocks), dim3(bs), lds, s, sc, op, pb, sb, hitq, hit_count, npix, first_sample, qout, count_out, shadow_count, work, c, *ad);
+        return;
+    }
+    const AdjBuf none{};
+    if (primary) hipLaunchKernelGGL((k_shade<true, false>), dim3(blocks), dim3(bs), 0, s, sc, op, pb, sb, hitq, hit_count, npix, first_sample, qout, count_out, shadow_count, work, c, none);
+    else hipLaunchKernelGGL((k_shade<false, false>), dim3(blocks), dim3(bs), 0, s, sc, op, pb, sb, hitq, hit_count, npix, first_sample, qout, count_out, shadow_count, work, c, none);
 }
 
 void launch_shadow(hipStream_t s, int blocks, int bs, const DevScene& sc, const DevOpts& op, const PathBuf& pb,
                    const ShadowBuf& sb, const int* count_in, int* work, int S, int* spill,
-                   unsigned long long* counters, int* ovf, bool count) {
+                   unsigned long long* counters, int* ovf, bool count, const AdjBuf* ad) {
     const size_t lds = static_cast<size_t>(S) * bs * sizeof(int);
-    if (count) hipLaunchKernelGGL(k_shadow<true>, dim3(blocks), dim3(bs), lds, s, sc, op, pb, sb, count_in, work, S, spill, counters, ovf);
-    else hipLaunchKernelGGL(k_shadow<false>, dim3(blocks), dim3(bs), lds, s, sc, op, pb, sb, count_in, work, S, spill, counters, ovf);
+    if (ad) {
+        const size_t extra = ad->lds ? static_cast<size_t>(ad->num_mats) * NPARAM * sizeof(float) : 0;
+        if (count) hipLaunchKernelGGL((k_shadow<true, true>), dim3(blocks), dim3(bs), lds + extra, s, sc, op, pb, sb, count_in, work, S, spill, counters, ovf, *ad);
+        else hipLaunchKernelGGL((k_shadow<false, true>), dim3(blocks), dim3(bs), lds + extra, s, sc, op, pb, sb, count_in, work, S, spill, counters, ovf, *ad);
+        return;
+    }
+    const AdjBuf none{};
+    if (count) hipLaunchKernelGGL((k_shadow<true, false>), dim3(blocks), dim3(bs), lds, s, sc, op, pb, sb, count_in, work, S, spill, counters, ovf, none);
+    else hipLaunchKernelGGL((k_shadow<false, false>), dim3(blocks), dim3(bs), lds, s, sc, op, pb, sb, count_in, work, S, spill, counters, ovf, none);
 }
 
 void launch_accum(hipStream_t s, const DevOpts& op, const PathBuf& pb, float4* accum, int npix, int nsamp) {

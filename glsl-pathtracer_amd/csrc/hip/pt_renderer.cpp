@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdint>
 #include <cstring>
 #include <exception>
 #include <string>
@@ -108,6 +109,8 @@ struct pt_renderer_s {
     DevBuf<int4> p_meta;
     DevBuf<int2> p_hit2;
     DevBuf<float4> s_o, s_d, s_c;
+    DevBuf<float4> a_rad, a_sg;   // adjoint: replay radiance, shadow-ray gradient records
+    DevBuf<float> a_dimg, a_grad; // adjoint: dLoss/dimage, gradient sums
     DevBuf<int> qa, qb, hq, counters;
     DevBuf<int> spill;
     DevBuf<unsigned long long> work_counts;  // [0,6) closest-hit work, [8,14) shadow work: nodes, tris, insts, lights, spills, rays
@@ -140,6 +143,7 @@ struct pt_renderer_s {
         lights.free(); tex.free(); env_img.free(); env_cdf.free(); accum.free();
         p_ro.free(); p_rd.free(); p_thr.free(); p_rad.free(); p_aux.free(); p_med.free(); p_hit.free();
         p_rng.free(); p_meta.free(); p_hit2.free(); s_o.free(); s_d.free(); s_c.free();
+        a_rad.free(); a_sg.free(); a_dimg.free(); a_grad.free();
         work_counts.free(); ldr.free();
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -451,8 +455,14 @@ void check_ready(pt_renderer_s* h) {
 //   [3D+3, 6D+6)        work counters (extend, shade, shadow) of bounce b
 int counter_cells(int D) { return 6 * D + 6; }
 
-void render_wavefront(pt_renderer_s* h, int first_sample, int num_samples) {
-    const DevOpts op = h->dev_opts();
+// Launch shape and buffers of the wavefront form for passes of up to
+// spp_pass samples of every pixel.
+struct Wavefront {
+    int npix, spp_pass, D, bs, S, b_ext, b_sha, b_sdw;
+    bool cw;
+};
+
+Wavefront wavefront_setup(pt_renderer_s* h, const DevOpts& op, int num_samples, size_t adj_lds) {
     if ((op.flags & ptd::OPT_VOL_MIS) && (op.flags & ptd::OPT_MEDIUM))
         throw ArgError(PT_ERR_UNSUPPORTED, "volume MIS is implemented by the plain kernel only (mode PT_MODE_PLAIN)");
     if ((op.flags & ptd::OPT_ALPHA_TEST) && !(op.flags & ptd::OPT_MEDIUM)) {
@@ -460,11 +470,12 @@ void render_wavefront(pt_renderer_s* h, int first_sample, int num_samples) {
         // and BSDF sampling, which the deferred shadow stage cannot reproduce.
         throw ArgError(PT_ERR_UNSUPPORTED, "alpha-tested shadow rays need the plain kernel (mode PT_MODE_PLAIN)");
     }
+    Wavefront w;
     const int W = op.width, H = op.height;
     const int tiles = ((W + 7) / 8) * ((H + 7) / 8);
-    const int npix = tiles * 64;
-    const int spp_pass = std::max(1, std::min(h->opts.wave_spp > 0 ? h->opts.wave_spp : 1, num_samples));
-    const size_t max_paths = static_cast<size_t>(npix) * spp_pass;
+    w.npix = tiles * 64;
+    w.spp_pass = std::max(1, std::min(h->opts.wave_spp > 0 ? h->opts.wave_spp : 1, num_samples));
+    const size_t max_paths = static_cast<size_t>(w.npix) * w.spp_pass;
     if (max_paths > static_cast<size_t>(INT32_MAX / 2)) throw ArgError(PT_ERR_OVERFLOW, "too many paths per pass");
     h->p_ro.ensure(max_paths); h->p_rd.ensure(max_paths); h->p_thr.ensure(max_paths); h->p_rad.ensure(max_paths);
     h->p_rng.ensure(max_paths); h->p_meta.ensure(max_paths); h->p_aux.ensure(max_paths);
@@ -473,59 +484,111 @@ void render_wavefront(pt_renderer_s* h, int first_sample, int num_samples) {
     h->s_o.ensure(2 * max_paths); h->s_d.ensure(2 * max_paths); h->s_c.ensure(2 * max_paths);
     h->qa.ensure(max_paths); h->qb.ensure(max_paths); h->hq.ensure(max_paths);
     // Bounce iterations: maxDepth + 1 closest hits; alpha pass-through repeats a depth.
-    const int D = op.max_depth + 1 + ((op.flags & ptd::OPT_ALPHA_TEST) ? 64 : 0);
-    h->counters.ensure(counter_cells(D));
+    w.D = op.max_depth + 1 + ((op.flags & ptd::OPT_ALPHA_TEST) ? 64 : 0);
+    h->counters.ensure(counter_cells(w.D));
     h->ensure_misc();
-    const int bs = h->block_size(), S = h->stack_S();
-    const bool cw = h->opts.count_work != 0;
-    const int b_ext = ptk::persistent_blocks(ptk::KIND_EXTEND, bs, S, cw, h->num_cu);
-    const int b_sha = ptk::persistent_blocks(ptk::KIND_SHADE, bs, S, cw, h->num_cu);
-    const int b_sdw = ptk::persistent_blocks(ptk::KIND_SHADOW, bs, S, cw, h->num_cu);
-    h->ensure_spill(static_cast<size_t>(std::max(b_ext, b_sdw)) * bs);
-    const ptd::PathBuf pb = path_buf(h);
-    ptd::ShadowBuf sb{h->s_o.p, h->s_d.p, h->s_c.p};
+    w.bs = h->block_size();
+    w.S = h->stack_S();
+    w.cw = h->opts.count_work != 0;
+    w.b_ext = ptk::persistent_blocks(ptk::KIND_EXTEND, w.bs, w.S, w.cw, h->num_cu);
+    w.b_sha = adj_lds != SIZE_MAX ? ptk::persistent_blocks(ptk::KIND_SHADE_ADJ, w.bs, w.S, false, h->num_cu, adj_lds)
+                                  : ptk::persistent_blocks(ptk::KIND_SHADE, w.bs, w.S, w.cw, h->num_cu);
+    w.b_sdw = adj_lds != SIZE_MAX ? ptk::persistent_blocks(ptk::KIND_SHADOW_ADJ, w.bs, w.S, false, h->num_cu, adj_lds)
+                                  : ptk::persistent_blocks(ptk::KIND_SHADOW, w.bs, w.S, w.cw, h->num_cu);
+    h->ensure_spill(static_cast<size_t>(std::max(w.b_ext, w.b_sdw)) * w.bs);
+    return w;
+}
+
+// Every bounce of one pass (paths [0, npaths), samples fs...): extend, shade,
+// shadow per bounce. With `ad` the shade and shadow stages are the adjoint's
+// replay variants.
+void run_bounces(pt_renderer_s* h, const DevOpts& op, const Wavefront& w, const ptd::PathBuf& pb,
+                 const ptd::ShadowBuf& sb, int npaths, int fs, const ptd::AdjBuf* ad) {
     int* cnt = h->counters.p;
     hipStream_t s = h->stream;
-
-    for (int s0 = 0; s0 < num_samples; s0 += spp_pass) {
-        const int ns = std::min(spp_pass, num_samples - s0);
-        const int npaths = npix * ns;
-        const int fs = first_sample + s0;
-        hipEvent_t ev;
-        ptk::launch_fill(s, cnt, counter_cells(D), 0);
-        int* qin = h->qa.p;
-        int* qout = h->qb.p;
-        int* c_ext = cnt;  // count of the current extend queue
-        const int cadence = std::max(1, h->opts.compact_every);
-        for (int b = 0; b < D; ++b) {
-            int* c_hit = cnt + (D + 1) + b;
-            int* c_sh = cnt + 2 * (D + 1) + b;
-            int* w = cnt + 3 * (D + 1) + 3 * b;
-            const bool primary = b == 0;
-            // bounce 0 always compacts (its input is the implicit slot range);
-            // later bounces compact every `cadence`-th time, otherwise the next
-            // extend re-reads this queue and skips ended paths.
-            const bool compact = primary || (b % cadence) == 0;
-            int* c_next = compact ? cnt + b + 1 : c_ext;
-            h->stage_begin(PT_STAGE_EXTEND, ev);
-            ptk::launch_extend(s, b_ext, bs, h->sc, op, pb, primary, qin, c_ext, npaths, npix, fs, h->hq.p, c_hit, w + 0,
-                               S, h->spill.p, h->work_counts.p, h->ovf.p, cw);
-            h->stage_end(PT_STAGE_EXTEND, ev);
-            h->stage_begin(PT_STAGE_SHADE, ev);
-            ptk::launch_shade(s, b_sha, bs, h->sc, op, pb, sb, primary, h->hq.p, c_hit, npix, fs, qout, c_next, c_sh, w + 1,
-                              compact);
-            h->stage_end(PT_STAGE_SHADE, ev);
-            h->stage_begin(PT_STAGE_SHADOW, ev);
-            ptk::launch_shadow(s, b_sdw, bs, h->sc, op, pb, sb, c_sh, w + 2, S, h->spill.p, h->work_counts.p + 8, h->ovf.p, cw);
-            h->stage_end(PT_STAGE_SHADOW, ev);
-            if (compact) {
-                std::swap(qin, qout);
-                c_ext = c_next;
-            }
+    const int D = w.D;
+    hipEvent_t ev;
+    ptk::launch_fill(s, cnt, counter_cells(D), 0);
+    int* qin = h->qa.p;
+    int* qout = h->qb.p;
+    int* c_ext = cnt;  // count of the current extend queue
+    const int cadence = std::max(1, h->opts.compact_every);
+    const bool cw = w.cw && !ad;
+    for (int b = 0; b < D; ++b) {
+        int* c_hit = cnt + (D + 1) + b;
+        int* c_sh = cnt + 2 * (D + 1) + b;
+        int* wk = cnt + 3 * (D + 1) + 3 * b;
+        const bool primary = b == 0;
+        // bounce 0 always compacts (its input is the implicit slot range);
+        // later bounces compact every `cadence`-th time, otherwise the next
+        // extend re-reads this queue and skips ended paths.
+        const bool compact = primary || (b % cadence) == 0;
+        int* c_next = compact ? cnt + b + 1 : c_ext;
+        h->stage_begin(PT_STAGE_EXTEND, ev);
+        ptk::launch_extend(s, w.b_ext, w.bs, h->sc, op, pb, primary, qin, c_ext, npaths, w.npix, fs, h->hq.p, c_hit,
+                           wk + 0, w.S, h->spill.p, h->work_counts.p, h->ovf.p, cw);
+        h->stage_end(PT_STAGE_EXTEND, ev);
+        const int shade_stage = ad ? PT_STAGE_ADJOINT : PT_STAGE_SHADE;
+        h->stage_begin(shade_stage, ev);
+        ptk::launch_shade(s, w.b_sha, w.bs, h->sc, op, pb, sb, primary, h->hq.p, c_hit, w.npix, fs, qout, c_next, c_sh,
+                          wk + 1, compact, ad);
+        h->stage_end(shade_stage, ev);
+        h->stage_begin(PT_STAGE_SHADOW, ev);
+        ptk::launch_shadow(s, w.b_sdw, w.bs, h->sc, op, pb, sb, c_sh, wk + 2, w.S, h->spill.p, h->work_counts.p + 8,
+                           h->ovf.p, cw, ad);
+        h->stage_end(PT_STAGE_SHADOW, ev);
+        if (compact) {
+            std::swap(qin, qout);
+            c_ext = c_next;
         }
+    }
+}
+
+void render_wavefront(pt_renderer_s* h, int first_sample, int num_samples) {
+    const DevOpts op = h->dev_opts();
+    const Wavefront w = wavefront_setup(h, op, num_samples, SIZE_MAX);
+    const ptd::PathBuf pb = path_buf(h);
+    const ptd::ShadowBuf sb{h->s_o.p, h->s_d.p, h->s_c.p};
+    for (int s0 = 0; s0 < num_samples; s0 += w.spp_pass) {
+        const int ns = std::min(w.spp_pass, num_samples - s0);
+        run_bounces(h, op, w, pb, sb, w.npix * ns, first_sample + s0, nullptr);
+        hipEvent_t ev;
         h->stage_begin(PT_STAGE_ACCUM, ev);
-        ptk::launch_accum(s, op, pb, h->accum.p, npix, ns);
+        ptk::launch_accum(h->stream, op, pb, h->accum.p, w.npix, ns);
         h->stage_end(PT_STAGE_ACCUM, ev);
+        HIPCHK(hipGetLastError());
+    }
+}
+
+// Path-replay adjoint: per pass, a primal pass leaves every path's total
+// radiance in the radiance buffer; the replay pass runs the same bounces on a
+// second radiance buffer and accumulates parameter gradients.
+void adjoint_wavefront(pt_renderer_s* h, int first_sample, int num_samples, float inv_n) {
+    const DevOpts op = h->dev_opts();
+    if (op.flags & ptd::OPT_ROUGHNESS_MOLLIFICATION)
+        throw ArgError(PT_ERR_UNSUPPORTED, "the adjoint does not differentiate roughness mollification");
+    const size_t grad_bytes = static_cast<size_t>(h->num_materials) * ptd::NPARAM * sizeof(float);
+    const bool lds = grad_bytes <= 32768;
+    const Wavefront w = wavefront_setup(h, op, num_samples, lds ? grad_bytes : 0);
+    const size_t max_paths = static_cast<size_t>(w.npix) * w.spp_pass;
+    h->a_rad.ensure(max_paths);
+    h->a_sg.ensure(4 * max_paths);
+    const ptd::PathBuf pb = path_buf(h);
+    ptd::PathBuf pb2 = pb;
+    pb2.rad = h->a_rad.p;
+    const ptd::ShadowBuf sb{h->s_o.p, h->s_d.p, h->s_c.p};
+    ptd::AdjBuf ad{};
+    ad.ltot = pb.rad;
+    ad.dimg = h->a_dimg.p;
+    ad.inv_n = inv_n;
+    ad.grad = h->a_grad.p;
+    ad.sg = h->a_sg.p;
+    ad.num_mats = h->num_materials;
+    ad.lds = lds ? 1 : 0;
+    for (int s0 = 0; s0 < num_samples; s0 += w.spp_pass) {
+        const int ns = std::min(w.spp_pass, num_samples - s0);
+        run_bounces(h, op, w, pb, sb, w.npix * ns, first_sample + s0, nullptr);
+        run_bounces(h, op, w, pb2, sb, w.npix * ns, first_sample + s0, &ad);
         HIPCHK(hipGetLastError());
     }
 }
@@ -941,8 +1004,28 @@ int pt_batch_render(pt_handle h, const pt_job* jobs, int32_t num_jobs) {
 
 int pt_adjoint(pt_handle h, int32_t first_sample, int32_t num_samples, const float* dL_dimage, float* d_params) {
     return Guard(h, [&] {
-        (void)first_sample; (void)num_samples; (void)dL_dimage; (void)d_params;
-        return Fail(h, PT_ERR_UNSUPPORTED, "pt_adjoint is not available in this build");
+        check_ready(h);
+        if (first_sample < 0 || num_samples <= 0) return Fail(h, PT_ERR_INVALID_ARG, "invalid sample range");
+        if (!dL_dimage || !d_params) return Fail(h, PT_ERR_INVALID_ARG, "null gradient buffers");
+        if (h->num_materials <= 0) return Fail(h, PT_ERR_STATE, "scene has no materials");
+        HIPCHK(hipSetDevice(h->device));
+        h->resolve_events();
+        const size_t npx = static_cast<size_t>(h->opts.width) * h->opts.height;
+        const size_t ng = static_cast<size_t>(h->num_materials) * ptd::NPARAM;
+        h->a_dimg.upload(dL_dimage, 3 * npx, h->stream);
+        h->a_grad.ensure(ng);
+        HIPCHK(hipMemsetAsync(h->a_grad.p, 0, ng * sizeof(float), h->stream));
+        h->ensure_misc();
+        HIPCHK(hipMemsetAsync(h->ovf.p, 0, sizeof(int), h->stream));
+        HIPCHK(hipEventRecord(h->ev_r0, h->stream));
+        adjoint_wavefront(h, first_sample, num_samples, 1.0f / static_cast<float>(num_samples));
+        HIPCHK(hipEventRecord(h->ev_r1, h->stream));
+        h->render_pending = true;
+        std::vector<float> g(ng);
+        HIPCHK(hipMemcpyAsync(g.data(), h->a_grad.p, ng * sizeof(float), hipMemcpyDeviceToHost, h->stream));
+        h->resolve_events();
+        for (size_t i = 0; i < ng; ++i) d_params[i] += g[i];
+        return h->stats.stack_overflow ? Fail(h, PT_ERR_OVERFLOW, "BVH traversal exceeded the maximum stack depth") : PT_OK;
     });
 }
 

@@ -13,7 +13,12 @@ LIB_DIR = os.path.join(_HERE, "lib")
 PT_OK = 0
 PT_MODE_WAVEFRONT = 0
 PT_MODE_PLAIN = 1
+
+# pt_adjoint parameter order and the material-record float each one is
+# (baseColor r, g, b, roughness, metallic, ior, clearcoat).
 PT_ADJOINT_PARAMS = 7
+ADJOINT_FIELDS = (0, 1, 2, 9, 8, 17, 14)
+ADJOINT_NAMES = ("baseColor.r", "baseColor.g", "baseColor.b", "roughness", "metallic", "ior", "clearcoat")
 STAGES = ("raygen", "extend", "shade", "shadow", "accum", "plain", "adjoint", "batch")
 
 f3 = C.c_float * 3

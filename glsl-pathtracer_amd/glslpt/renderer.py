@@ -48,9 +48,17 @@ class Renderer:
             raise api.PtError(fn, rc, msg.decode() if msg else "")
 
     # --- GLSLPT::Renderer surface ---------------------------------------
-    def upload_scene(self, scene):
+    def upload_scene(self, scene, materials=None):
+        """Upload the scene's arrays; `materials` ((num_materials, 32) float32)
+        replaces the scene's material table in the upload."""
         v = scene.view()
+        if materials is not None:
+            m = np.ascontiguousarray(materials, np.float32)
+            if m.shape != (v.num_materials, 32):
+                raise ValueError(f"materials must be ({v.num_materials}, 32)")
+            v.materials = C.cast(m.ctypes.data, C.POINTER(api.PtMaterial))
         self._check("pt_upload_scene", self._lib.pt_upload_scene(self._h, C.byref(v)))
+        self.num_materials = v.num_materials
 
     def update_instances(self, scene, first_node=None):
         """Instance-edit refresh after Scene.RebuildInstances (Renderer.cpp:649-665)."""
@@ -104,6 +112,20 @@ class Renderer:
 
     def reset_stats(self):
         self._check("pt_reset_stats", self._lib.pt_reset_stats(self._h))
+
+    # --- adjoint ---------------------------------------------------------
+    def adjoint(self, first_sample, num_samples, dl_dimage):
+        """Gradient of a loss with respect to the material parameters
+        (baseColor r, g, b, roughness, metallic, ior, clearcoat per material)
+        for the samples [first_sample, first_sample + num_samples). dl_dimage is
+        dLoss/d(mean image), (height, width, 3), rows bottom-up like read_accum."""
+        g = np.ascontiguousarray(dl_dimage, np.float32)
+        if g.shape != (self.height, self.width, 3):
+            raise ValueError(f"dl_dimage must be ({self.height}, {self.width}, 3)")
+        out = np.zeros((self.num_materials, api.PT_ADJOINT_PARAMS), np.float32)
+        self._check("pt_adjoint", self._lib.pt_adjoint(self._h, int(first_sample), int(num_samples), api.fptr(g),
+                                                       api.fptr(out)))
+        return out
 
     # --- ray queries ------------------------------------------------------
     def closest_hit(self, origins, dirs):

@@ -11,6 +11,9 @@
                          3-octave value noise, random rigid transforms and
                          uniform scale, Disney materials drawn from a seed,
                          rect + sphere lights and a constant environment.
+* ``material_boxes``     differentiable-materials workload: 256 boxes and 8
+                         noise-displaced spheres (~200k triangles), 64
+                         materials with seeded optimisable parameters.
 * ``tiny``               a few triangles, for smoke runs.
 
 Each builder returns (Scene, RenderSettings) where RenderSettings carries the
@@ -237,6 +240,101 @@ def instanced_spheres(num_instances=64, freq=28, seed=0x5EED0002, width=1920, he
     sc.AddCamera((0.0, extent * 0.5, extent * 3.0), (0.0, 0.0, 0.0), 45.0)
     sc.ProcessScene()
     return sc, RenderSettings(width, height, spp, max_depth, enable_rr=True, rr_depth=3,
+                              enable_uniform_light=True, uniform_light_col=(0.3, 0.3, 0.3))
+
+
+# --------------------------------------------------- differentiable materials
+def _unit_box():
+    """12 outward-facing triangles of the cube [-1, 1]^3 with flat normals."""
+    c = np.array([[x, y, z] for x in (-1.0, 1.0) for y in (-1.0, 1.0) for z in (-1.0, 1.0)])
+    faces = [(0, 1, 3, 2), (4, 6, 7, 5), (0, 4, 5, 1), (2, 3, 7, 6), (0, 2, 6, 4), (1, 5, 7, 3)]
+    tris = []
+    for f in faces:
+        tris += _quad_tris([c[i] for i in f])
+    tris = _orient_outward(np.stack(tris), np.zeros(3))
+    return tris.astype(np.float32), _face_normals(tris).astype(np.float32)
+
+
+def ior_from_specular(specular):
+    """Disney `specular` to index of refraction: F0 = 0.08 * specular =
+    ((ior - 1) / (ior + 1))^2 (specular 0.5 -> ior 1.5)."""
+    r = np.sqrt(0.08 * np.asarray(specular, np.float64))
+    return (1.0 + r) / (1.0 - r)
+
+
+def diff_material_params(seed, n=64):
+    """Seeded optimisable parameters of n Disney materials, inside the open
+    ranges (no parameter at a lobe's on/off boundary): baseColor U[0.05, 0.95]^3,
+    roughness U[0.1, 0.9], metallic U[0.05, 0.95], specular U[0.2, 0.8] mapped
+    to ior, clearcoat U[0.05, 0.95]. Returns (n, 7) in pt_adjoint order."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    p = np.zeros((n, 7), np.float32)
+    p[:, 0:3] = rng.uniform(0.05, 0.95, (n, 3))
+    p[:, 3] = rng.uniform(0.1, 0.9, n)
+    p[:, 4] = rng.uniform(0.05, 0.95, n)
+    p[:, 5] = ior_from_specular(rng.uniform(0.2, 0.8, n))
+    p[:, 6] = rng.uniform(0.05, 0.95, n)
+    return p
+
+
+def apply_params(materials, params):
+    """Material table (n, 32) with the pt_adjoint parameters replaced."""
+    from ._capi import ADJOINT_FIELDS
+    m = np.array(materials, np.float32, copy=True)
+    for k, f in enumerate(ADJOINT_FIELDS):
+        m[:, f] = params[:, k]
+    return m
+
+
+def material_boxes(num_boxes=256, sphere_freq=35, num_spheres=8, num_materials=64, seed=0xD1FF0004,
+                   param_seed=0xD1FF0004, width=512, height=512, spp=16, max_depth=4, extent=10.0):
+    """Differentiable-materials workload: `num_boxes` boxes (12 triangles
+    each, random rotation and size, one of them a ground slab) and
+    `num_spheres` distinct noise-displaced geodesic spheres of frequency
+    `sphere_freq` (20 f^2 triangles each; 8 x 24,500 + 256 x 12 = 199,072
+    triangles by default), `num_materials` Disney materials from
+    diff_material_params(param_seed), two rect lights and a constant
+    environment."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    sc = Scene()
+    bpos, bnrm = _unit_box()
+    box = sc.AddMesh(bpos, bnrm, name="box")
+    spheres = []
+    v, f = geodesic_sphere(sphere_freq)
+    for k in range(num_spheres):
+        d = 1.0 + 0.1 * value_noise3(v, seed + 17 * k)
+        pv = v * d[:, None]
+        fn = np.cross(pv[f[:, 1]] - pv[f[:, 0]], pv[f[:, 2]] - pv[f[:, 0]])
+        vn = np.zeros_like(pv)
+        for j in range(3):
+            np.add.at(vn, f[:, j], fn)
+        vn /= np.linalg.norm(vn, axis=1, keepdims=True)
+        spheres.append(sc.AddMesh(pv[f].astype(np.float32), vn[f].astype(np.float32), name=f"sphere{k}"))
+    params = diff_material_params(param_seed, num_materials)
+    mids = []
+    for i in range(num_materials):
+        p = params[i]
+        mids.append(sc.AddMaterial(material(baseColor=p[0:3], roughness=p[3], metallic=p[4], ior=p[5],
+                                            clearcoat=p[6], clearcoatGloss=0.5)))
+    # ground slab, then boxes scattered over it
+    sc.AddMeshInstance(box, mat4(translate=(0, -0.5, 0), scale=(extent * 1.5, 0.5, extent * 1.5)),
+                       mids[int(rng.integers(num_materials))])
+    for i in range(num_boxes - 1):
+        sx, sy, sz = rng.uniform(0.2, 0.9, 3)
+        t = (rng.uniform(-extent, extent), rng.uniform(0.2, 3.0), rng.uniform(-extent, extent))
+        sc.AddMeshInstance(box, mat4(translate=t, scale=(sx, sy, sz), rot=random_rotation(rng)),
+                           mids[int(rng.integers(num_materials))])
+    for k, m in enumerate(spheres):
+        r = rng.uniform(1.0, 2.0)
+        t = (rng.uniform(-0.7, 0.7) * extent, r + 0.2, rng.uniform(-0.7, 0.7) * extent)
+        sc.AddMeshInstance(m, mat4(translate=t, scale=(r, r, r), rot=random_rotation(rng)),
+                           mids[int(rng.integers(num_materials))])
+    h = 12.0
+    sc.AddLight(quad_light((-6, h, -6), (0, h, -6), (-6, h, 0), (12, 11, 10)))
+    sc.AddLight(quad_light((3, h, 3), (9, h, 3), (3, h, 9), (8, 10, 12)))
+    sc.AddCamera((0.0, 9.0, 19.0), (0.0, 1.0, 0.0), 45.0)
+    sc.ProcessScene()
+    return sc, RenderSettings(width, height, spp, max_depth, enable_rr=True, rr_depth=2,
                               enable_uniform_light=True, uniform_light_col=(0.3, 0.3, 0.3))
 
 

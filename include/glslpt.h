@@ -206,10 +206,19 @@ typedef struct pt_job {
 int pt_batch_render(pt_handle h, const pt_job* jobs, int32_t num_jobs);
 
 /* ---- adjoint: gradient of an image loss w.r.t. material parameters ---- */
-/* Parameters per material, in this order: baseColor.r, .g, .b, roughness,
- * metallic, ior, clearcoat (7 floats). dL_dimage is RGB float per pixel
- * (width*height*3, rows bottom-up) for the same samples as pt_render.
- * d_params receives num_materials*7 floats (added to, not overwritten). */
+/* Path-replay backpropagation over samples [first_sample, first_sample +
+ * num_samples) with the current scene, options and camera (no reference
+ * counterpart; the forward path it differentiates is PathTrace,
+ * pathtrace.glsl:285-476). Parameters per material, in this order:
+ * baseColor.r, .g, .b (record floats 0-2), roughness (9), metallic (8),
+ * ior (17), clearcoat (14). dL_dimage is dLoss/d(mean image) where the mean
+ * image is the accumulation of those samples divided by num_samples: RGB
+ * float per pixel (width*height*3, rows bottom-up like pt_read_accum).
+ * d_params receives num_materials*7 floats, added to (not overwritten).
+ * Sampling, MIS weights and Russian roulette are detached: the estimate is
+ * unbiased for the gradient of the expected image. Runs the wavefront form
+ * (primal pass + replay pass) whatever pt_options.mode says; returns
+ * PT_ERR_UNSUPPORTED with roughness mollification enabled. */
 #define PT_ADJOINT_PARAMS 7
 int pt_adjoint(pt_handle h, int32_t first_sample, int32_t num_samples, const float* dL_dimage,
                float* d_params);

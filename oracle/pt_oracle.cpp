@@ -221,6 +221,9 @@ template <class T> struct State {
     int matID = 0;
     Material<T> mat;
     Medium<T> medium;
+    // frozen-sampling mode only: the material evaluated for f (see Ctx::eval_mats)
+    Material<T> mat_e;
+    T eta_e = 0;
     // instrumentation (not in the reference): which primitive was hit
     int hitInstance = -1, hitTriRow = -1, hitLight = -1;
     T baryU = 0, baryV = 0;
@@ -243,6 +246,13 @@ template <class T> struct Ctx {
     uint32_t seed[4];
     Counters* cnt;
     bool stack_overflow = false;
+    // Frozen-sampling mode (test instrument for the adjoint, not in the
+    // reference): directions, lobe choices, pdfs, MIS weights and Russian
+    // roulette come from the scene's materials; the BSDF values f that weight
+    // the path come from this second material table. A central difference of
+    // the image over eval_mats is then the derivative the path-replay adjoint
+    // estimates, sample by sample.
+    const float* eval_mats = nullptr;
     T rand() { return T(rand_f(seed)); }
 };
 
@@ -1022,10 +1032,10 @@ template <class T> bool AnyHit(Ctx<T>& c, const Ray<T>& r, T maxDist) {
 }
 
 // ------------------------------------------------------------ pathtrace.glsl:25-476
-template <class T> void GetMaterial(Ctx<T>& c, State<T>& state, const Ray<T>& r) {
+template <class T> void GetMaterialFrom(Ctx<T>& c, State<T>& state, const Ray<T>& r, const float* table) {
     const oracle_scene* s = c.s;
     const oracle_options* o = c.o;
-    const float* p = s->materials + 32 * static_cast<size_t>(state.matID);
+    const float* p = table + 32 * static_cast<size_t>(state.matID);
     Material<T> mat;
     mat.baseColor = fetch3<T>(p);
     mat.anisotropic = T(p[3]);
@@ -1081,6 +1091,31 @@ template <class T> void GetMaterial(Ctx<T>& c, State<T>& state, const Ray<T>& r)
     state.eta = dot(r.direction, state.normal) < T(0) ? (T(1) / mat.ior) : mat.ior;
 }
 
+template <class T> void GetMaterial(Ctx<T>& c, State<T>& state, const Ray<T>& r) {
+    if (c.eval_mats) {
+        State<T> e = state;  // shading-normal edits of this call are discarded
+        e.mat = state.mat_e;
+        GetMaterialFrom(c, e, r, c.eval_mats);
+        state.mat_e = e.mat;
+        state.eta_e = e.eta;
+    }
+    GetMaterialFrom(c, state, r, c.s->materials);
+}
+
+// f of the BSDF that weights the path: the scene's material, or in
+// frozen-sampling mode the eval table's (pdf always from the scene's).
+template <class T> V3<T> EvalF(const Ctx<T>& c, const State<T>& state, V3<T> V, V3<T> N, V3<T> L, T& pdf) {
+    V3<T> f = DisneyEval(state, V, N, L, pdf);
+    if (c.eval_mats) {
+        State<T> e = state;
+        e.mat = state.mat_e;
+        e.eta = state.eta_e;
+        T unused;
+        f = DisneyEval(e, V, N, L, unused);
+    }
+    return f;
+}
+
 template <class T> V3<T> EvalTransmittance(Ctx<T>& c, Ray<T> r) {
     LightSampleRec<T> lightSample;
     State<T> state;
@@ -1118,7 +1153,7 @@ template <class T> V3<T> DirectLight(Ctx<T>& c, const Ray<T>& r, const State<T>&
         Ray<T> shadowRay{scatterPos, lightDir};
         if (volmis) {
             Li = Li * EvalTransmittance(c, shadowRay);
-            if (isSurface) scatterSample.f = DisneyEval(state, -r.direction, state.ffnormal, lightDir, scatterSample.pdf);
+            if (isSurface) scatterSample.f = EvalF(c, state, -r.direction, state.ffnormal, lightDir, scatterSample.pdf);
             else {
                 T p = PhaseHG(dot(-r.direction, lightDir), state.medium.anisotropy);
                 scatterSample.f = V3<T>(p);
@@ -1131,7 +1166,7 @@ template <class T> V3<T> DirectLight(Ctx<T>& c, const Ray<T>& r, const State<T>&
         } else {
             bool inShadow = AnyHit(c, shadowRay, K<T>::INF - K<T>::EPS);
             if (!inShadow) {
-                scatterSample.f = DisneyEval(state, -r.direction, state.ffnormal, lightDir, scatterSample.pdf);
+                scatterSample.f = EvalF(c, state, -r.direction, state.ffnormal, lightDir, scatterSample.pdf);
                 if (scatterSample.pdf > T(0)) {
                     T misWeight = PowerHeuristic(lightPdf, scatterSample.pdf);
                     if (misWeight > T(0)) Ld = Ld + misWeight * Li * scatterSample.f * T(o->envmap_intensity) / lightPdf;
@@ -1152,7 +1187,7 @@ template <class T> V3<T> DirectLight(Ctx<T>& c, const Ray<T>& r, const State<T>&
             Ray<T> shadowRay{scatterPos, lightSample.direction};
             if (volmis) {
                 Li = Li * EvalTransmittance(c, shadowRay);
-                if (isSurface) scatterSample.f = DisneyEval(state, -r.direction, state.ffnormal, lightSample.direction, scatterSample.pdf);
+                if (isSurface) scatterSample.f = EvalF(c, state, -r.direction, state.ffnormal, lightSample.direction, scatterSample.pdf);
                 else {
                     T p = PhaseHG(dot(-r.direction, lightSample.direction), state.medium.anisotropy);
                     scatterSample.f = V3<T>(p);
@@ -1164,7 +1199,7 @@ template <class T> V3<T> DirectLight(Ctx<T>& c, const Ray<T>& r, const State<T>&
             } else {
                 bool inShadow = AnyHit(c, shadowRay, lightSample.dist - K<T>::EPS);
                 if (!inShadow) {
-                    scatterSample.f = DisneyEval(state, -r.direction, state.ffnormal, lightSample.direction, scatterSample.pdf);
+                    scatterSample.f = EvalF(c, state, -r.direction, state.ffnormal, lightSample.direction, scatterSample.pdf);
                     T misWeight = T(1);
                     if (light.area > T(0)) misWeight = PowerHeuristic(lightSample.pdf, scatterSample.pdf);
                     if (scatterSample.pdf > T(0)) Ld = Ld + misWeight * Li * scatterSample.f / lightSample.pdf;
@@ -1179,6 +1214,7 @@ template <class T> V4<T> PathTrace(Ctx<T>& c, Ray<T> r) {
     const oracle_options* o = c.o;
     V3<T> radiance(T(0));
     V3<T> throughput(T(1));
+    V3<T> thr_s(T(1));  // throughput of the sampling materials (== throughput unless frozen-sampling)
     State<T> state;
     LightSampleRec<T> lightSample;
     ScatterSampleRec<T> scatterSample;
@@ -1223,6 +1259,7 @@ template <class T> V4<T> PathTrace(Ctx<T>& c, Ray<T> r) {
             if (inMedium) {
                 if (state.medium.type == MEDIUM_ABSORB) {
                     throughput = throughput * vexp(-(V3<T>(T(1)) - state.medium.color) * state.hitDist * state.medium.density);
+                    thr_s = thr_s * vexp(-(V3<T>(T(1)) - state.medium.color) * state.hitDist * state.medium.density);
                 } else if (state.medium.type == MEDIUM_EMISSIVE) {
                     radiance = radiance + state.medium.color * state.hitDist * state.medium.density * throughput;
                 } else {
@@ -1230,6 +1267,7 @@ template <class T> V4<T> PathTrace(Ctx<T>& c, Ray<T> r) {
                     mediumSampled = scatterDist < state.hitDist;
                     if (mediumSampled) {
                         throughput = throughput * state.medium.color;
+                        thr_s = thr_s * state.medium.color;
                         r.origin = r.origin + r.direction * scatterDist;
                         state.fhp = r.origin;
                         radiance = radiance + DirectLight(c, r, state, false) * throughput;
@@ -1257,8 +1295,16 @@ template <class T> V4<T> PathTrace(Ctx<T>& c, Ray<T> r) {
                 surfaceScatter = true;
                 radiance = radiance + DirectLight(c, r, state, true) * throughput;
                 scatterSample.f = DisneySample(c, state, -r.direction, state.ffnormal, scatterSample.L, scatterSample.pdf);
-                if (scatterSample.pdf > T(0)) throughput = throughput * (scatterSample.f / scatterSample.pdf);
-                else break;
+                if (scatterSample.pdf > T(0)) {
+                    thr_s = thr_s * (scatterSample.f / scatterSample.pdf);
+                    if (c.eval_mats) {
+                        T unused;
+                        scatterSample.f = EvalF(c, state, -r.direction, state.ffnormal, scatterSample.L, unused);
+                    }
+                    throughput = throughput * (scatterSample.f / scatterSample.pdf);
+                } else {
+                    break;
+                }
             }
             r.direction = scatterSample.L;
             r.origin = state.fhp + r.direction * K<T>::EPS;
@@ -1273,9 +1319,10 @@ template <class T> V4<T> PathTrace(Ctx<T>& c, Ray<T> r) {
         }
 
         if (o->opt_rr && state.depth >= o->rr_depth) {
-            T q = gmin(gmax(throughput.x, gmax(throughput.y, throughput.z)) + T(0.001), T(0.95));
+            T q = gmin(gmax(thr_s.x, gmax(thr_s.y, thr_s.z)) + T(0.001), T(0.95));
             if (c.rand() > q) break;
             throughput = throughput / q;
+            thr_s = thr_s / q;
         }
     }
     return V4<T>{radiance.x, radiance.y, radiance.z, alpha};
@@ -1316,7 +1363,8 @@ template <class T> V4<T> TileSample(Ctx<T>& c, const oracle_camera* cam, int px,
 
 template <class T, class Acc>
 int RenderT(const oracle_scene* s, const oracle_camera* cam, const oracle_options* o, int first, int n,
-            int row_begin, int row_end, int row_step, Acc* accum, int nthreads, oracle_counters* out) {
+            int row_begin, int row_end, int row_step, Acc* accum, int nthreads, oracle_counters* out,
+            const float* eval_mats = nullptr) {
     const int W = o->width;
     const int rows = row_step > 0 ? (row_end - row_begin + row_step - 1) / row_step : 0;
     const int tiles_x = (W + 7) / 8, tiles_y = (rows + 7) / 8;
@@ -1328,6 +1376,7 @@ int RenderT(const oracle_scene* s, const oracle_camera* cam, const oracle_option
     {
         Counters local;
         Ctx<T> c{s, o, {0, 0, 0, 0}, out ? &local : nullptr};
+        c.eval_mats = eval_mats;
 #pragma omp for schedule(dynamic)
         for (int tile = 0; tile < ntiles; ++tile) {
             const int tx = tile % tiles_x, ty = tile / tiles_x;
@@ -1571,6 +1620,14 @@ int oracle_sample_envmap(const oracle_scene* s, const oracle_options* o, uint32_
     rgb[0] = col.x; rgb[1] = col.y; rgb[2] = col.z;
     std::memcpy(seed, c.seed, 16);
     return 0;
+}
+
+int oracle_render_frozen(const oracle_scene* s, const float* eval_materials, const oracle_camera* cam,
+                         const oracle_options* o, int32_t first_sample, int32_t num_samples, float* accum,
+                         int32_t num_threads) {
+    if (!s || !eval_materials || !cam || !o || !accum) return -1;
+    return RenderT<float, float>(s, cam, o, first_sample, num_samples, 0, o->height, 1, accum, num_threads, nullptr,
+                                 eval_materials);
 }
 
 }  // extern "C"

@@ -61,6 +61,14 @@ int oracle_render(int32_t precision, const oracle_scene* s, const oracle_camera*
                   int32_t row_begin, int32_t row_end, int32_t row_step,
                   float* accum_f32, double* accum_f64, int32_t num_threads, oracle_counters* counters);
 
+/* Frozen-sampling render (float path, all rows): samples, pdfs, MIS weights
+ * and Russian roulette from s->materials, BSDF values f from eval_materials
+ * (same layout). Used to check the path-replay adjoint: d(image)/d(eval
+ * parameters) at eval == s->materials is what the adjoint estimates. */
+int oracle_render_frozen(const oracle_scene* s, const float* eval_materials, const oracle_camera* cam,
+                         const oracle_options* o, int32_t first_sample, int32_t num_samples, float* accum,
+                         int32_t num_threads);
+
 /* Per-sample radiance (no accumulation): rgba[4 * (i)] for pixel list px/py,
  * sample index s[i]. */
 int oracle_trace_samples(const oracle_scene* s, const oracle_camera* cam, const oracle_options* o,

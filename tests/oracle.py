@@ -74,6 +74,8 @@ def lib():
         l.oracle_eval_envmap.argtypes = [P(OracleScene), P(OracleOptions), P(f32), P(f32)]
         l.oracle_sample_envmap.argtypes = [P(OracleScene), P(OracleOptions), P(C.c_uint32), P(f32), P(f32)]
         l.oracle_eval_fn.argtypes = [i32, i32, P(f32), P(f32)]
+        l.oracle_render_frozen.argtypes = [P(OracleScene), P(f32), P(OracleCamera), P(OracleOptions), i32, i32,
+                                           P(f32), i32]
         l.oracle_set_inverse_mode.argtypes = [i32]
         l.oracle_set_inverse_mode.restype = None
         _lib = l
@@ -189,6 +191,20 @@ def render(sa, cam, opts, first_sample, num_samples, precision=32, rows=None, th
     if rc != 0:
         raise RuntimeError(f"oracle_render failed {rc}")
     return (acc, cnt) if counters else acc
+
+
+def render_frozen(sa, eval_materials, cam, opts, first_sample, num_samples, threads=0):
+    """Frozen-sampling render: paths sampled with the scene's materials, BSDF
+    values from eval_materials ((num_materials, 32) float32)."""
+    W, H = opts.width, opts.height
+    em = np.ascontiguousarray(eval_materials, np.float32)
+    assert em.shape == sa.a["materials"].shape
+    acc = np.zeros((H, W, 4), np.float32)
+    rc = lib().oracle_render_frozen(C.byref(sa.s), fp(em), C.byref(cam), C.byref(opts), first_sample, num_samples,
+                                    fp(acc), threads)
+    if rc != 0:
+        raise RuntimeError(f"oracle_render_frozen failed {rc}")
+    return acc
 
 
 def closest_hit(sa, opts, o, d, counters=False):

@@ -83,3 +83,42 @@ def test_options_switch_code_paths():
     sa3, cam3, o3 = oracle_setup(si, rsi, opt_background=1)
     bg = oracle.render(sa3, cam3, o3, 0, 2)
     assert bg[..., 3].min() == 0.0 and bg[..., 3].max() == 2.0
+
+
+def test_frozen_sampling_matches_normal_render():
+    """With the evaluation table equal to the scene's materials the
+    frozen-sampling render is the ordinary render (up to the last ulp of the
+    re-evaluated BSDF)."""
+    from glslpt import scenes as S
+    sc, rs = S.material_boxes(num_boxes=24, sphere_freq=5, num_spheres=2, num_materials=6, width=32, height=32,
+                              spp=2, max_depth=4, extent=5.0)
+    sa, cam, o = oracle_setup(sc, rs)
+    base = oracle.render(sa, cam, o, 0, 2)
+    fr = oracle.render_frozen(sa, sa.a["materials"], cam, o, 0, 2)
+    assert np.allclose(fr, base, rtol=1e-5, atol=1e-6)
+
+
+def test_frozen_sampling_derivative_of_diffuse_albedo():
+    """Frozen sampling differentiates only f: for a Burley-diffuse plane (ior
+    1: no specular lobe) under a rect light the radiance of every depth-1 path
+    is linear in baseColor, so a central difference over the evaluation
+    table's baseColor gives radiance / baseColor in that channel and nothing
+    in the others."""
+    color = np.array([0.8, 0.6, 0.4])
+    sc = plane_under_light(0.4, color, np.array([5.0, 4.0, 3.0]))
+    rs = RenderSettings(16, 16, 4, 1, enable_rr=False)
+    sa, cam, o = oracle_setup(sc, rs)
+    m = sa.a["materials"]
+    base = oracle.render_frozen(sa, m, cam, o, 0, 4)[..., :3].astype(np.float64)
+    eps = 1e-3
+    for c in range(3):
+        mp, mm = m.copy(), m.copy()
+        mp[0, c] += eps
+        mm[0, c] -= eps
+        d = (oracle.render_frozen(sa, mp, cam, o, 0, 4)[..., :3].astype(np.float64)
+             - oracle.render_frozen(sa, mm, cam, o, 0, 4)[..., :3]) / (2 * eps)
+        want = base[..., c] / color[c]
+        lit = want > 1e-3 * want.max()
+        assert np.allclose(d[..., c][lit], want[lit], rtol=2e-3), c
+        others = [k for k in range(3) if k != c]
+        assert np.abs(d[..., others]).max() <= 1e-3 * base.max()
