@@ -20,6 +20,13 @@ triangle (three positions), 52 B per instance (inverse affine + BLAS root),
 achieved = bytes per launch / average launch time (HIP events on the
 renderer's stream over the timed region); peak = 8 TB/s HBM.
 
+adjoint: the second half of the metric, backward/forward time ratio, on the
+differentiable-materials workload (256 boxes + 8 noise-displaced spheres,
+199,072 triangles, 64 Disney materials, 512x512, 16 spp, max depth 4): forward
+= one pt_render of the 16 samples, backward = one pt_adjoint of the same
+samples for an L2 loss against a target rendered from a second parameter seed
+(both device time between HIP events, median of --adjoint-reps).
+
 cpu_baseline: the CPU oracle (C++ restatement of the reference shaders,
 OpenMP) on the host cores, rendering a bounded sample of the same frame
 (every k-th row at 1 spp, k chosen for ~15 s of CPU work).
@@ -105,6 +112,49 @@ def cpu_baseline(sc, rs, target_s):
             "sample": f"{rows} rows (every {step}th) x {rs.width} px x {spp} spp of the same frame, {dt:.1f} s"}
 
 
+def adjoint_ratio(reps, wave_spp):
+    """Forward and adjoint device time on the differentiable-materials
+    workload, L2 loss against a target from a second parameter seed."""
+    import numpy as np
+    from glslpt import scenes
+    from glslpt.renderer import Renderer, options_for
+    sc, rs = scenes.material_boxes()
+    a = sc.arrays()
+    target_mats = scenes.apply_params(a["materials"], scenes.diff_material_params(0xD1FF0005, a["materials"].shape[0]))
+    r = Renderer(0)
+    r.set_options(options_for(rs, wave_spp=wave_spp))
+    r.set_camera(sc.camera())
+    r.upload_scene(sc, materials=target_mats)
+    r.clear()
+    r.render(1 << 20, rs.spp)  # target: other samples, other parameters
+    target = r.read_accum()[..., :3] / rs.spp
+    r.upload_scene(sc)
+    fwd, bwd = [], []
+    loss = None
+    for i in range(reps + 1):
+        r.clear()
+        r.render(0, rs.spp)
+        r.synchronize()
+        f_ms = r.stats()["ms_last_render"]
+        img = r.read_accum()[..., :3] / rs.spp
+        diff = img - target
+        loss = float(np.mean(diff.astype(np.float64) ** 2))
+        dl = (2.0 / diff.size) * diff
+        g = r.adjoint(0, rs.spp, dl.astype(np.float32))
+        b_ms = r.stats()["ms_last_render"]
+        if i > 0:  # the first round is warm-up
+            fwd.append(f_ms)
+            bwd.append(b_ms)
+    r.close()
+    f_med, b_med = float(np.median(fwd)), float(np.median(bwd))
+    return {"workload": f"{int(a['vert_indices'].shape[0]) + 12 * 255} triangles (256 boxes + 8 noise-displaced "
+                        f"spheres), {a['materials'].shape[0]} Disney materials x 7 parameters, "
+                        f"{rs.width}x{rs.height}, {rs.spp} spp, max depth {rs.max_depth}",
+            "forward_ms": f_med, "backward_ms": b_med, "backward_over_forward": b_med / f_med,
+            "reps": reps, "loss": loss, "grad_norm": float(np.linalg.norm(g)),
+            "launch": {"wave_spp": wave_spp}}
+
+
 def measured_traffic(cfg):
     """HBM bytes per k_extend dispatch from the committed rocprofv3 counter
     runs (profiles/extend_c2_1080p_traffic.*), when they were taken with the
@@ -134,6 +184,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-count", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--adjoint-reps", type=int, default=5)
+    ap.add_argument("--no-adjoint", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported (the path shards by sample range "
@@ -196,6 +248,8 @@ def main():
                     "stack_spills": cw["stack_spills"]}
     r.close()
 
+    adjoint = None if args.no_adjoint else adjoint_ratio(args.adjoint_reps, 16)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(sc, rs, args.cpu_seconds)
@@ -212,6 +266,7 @@ def main():
         "stage_ms_per_step": {k: v / args.steps for k, v in st_w["ms_stage"].items() if v > 0},
         "plain": plain,
         "wavefront_over_plain": (value / plain["value"]) if plain else None,
+        "adjoint": adjoint,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -719,14 +719,37 @@ __device__ __forceinline__ float slab(float mnx, float mny, float mnz, float mxx
 // current best are skipped; that never changes which primitive wins except for
 // exact ties decided by rounding. ANY returns at the first accepted triangle
 // with t < tmax.
+// Alpha test of an any-hit candidate (anyhit.glsl:118-140): the instance
+// material's opacity times the base-colour texture's alpha at the hit
+// (texture layer texIDs.x, clamped like a GL array texture; alpha 1 without
+// textures). MASK: transparent below the cutoff; BLEND: transparent with
+// probability 1 - opacity, drawing from the path's RNG (null: opaque).
+__device__ __forceinline__ bool alpha_transparent(const DevScene& sc, int row, int mat, float u, float v, float w,
+                                                  Rng* rng) {
+    const int4 vi = sc.vidx[row];
+    const float4 p0 = sc.verts[vi.x], p1 = sc.verts[vi.y], p2 = sc.verts[vi.z];
+    const float q0 = sc.norms[vi.x].w, q1 = sc.norms[vi.y].w, q2 = sc.norms[vi.z].w;
+    const float tu = p0.w * w + p1.w * u + p2.w * v;
+    const float tv = q0 * w + q1 * u + q2 * v;
+    const float4 texIDs = sc.mats[8 * mat + 6];
+    const float4 ap = sc.mats[8 * mat + 7];
+    const float a = sc.num_tex > 0 ? tex_layer(sc, tu, tv, texIDs.x).w : 1.0f;
+    const float opacity = ap.x * a;
+    const int mode = static_cast<int>(ap.y);
+    return (mode == ALPHA_MODE_MASK && opacity < ap.z) || (mode == ALPHA_MODE_BLEND && rng && rnd(*rng) > opacity);
+}
+
+// Two-level traversal. ANY: return at the first accepted hit closer than
+// tmax; with `alpha` set, candidates are alpha-tested first (AnyHit with
+// OPT_ALPHA_TEST and no media).
 template <bool ANY, bool COUNT>
 __device__ bool traverse(const DevScene& sc, vec3 O, vec3 D, float tmax, Stack& stk, Hit& h, TravCount& cnt,
-                         bool& overflow) {
+                         bool& overflow, bool alpha = false, Rng* rng = nullptr) {
     if (COUNT) cnt.rays++;
     const vec3 invD = V(1.0f / D.x, 1.0f / D.y, 1.0f / D.z);
     vec3 o = O, d = D, inv = invD;
     float best = tmax;
-    int sp = 0, blas_base = -1, cur_inst = -1;
+    int sp = 0, blas_base = -1, cur_inst = -1, cur_mat = 0;
     int ref = sc.root;
     for (;;) {
         bool do_pop = false;
@@ -757,6 +780,7 @@ __device__ bool traverse(const DevScene& sc, vec3 O, vec3 D, float tmax, Stack& 
             if (COUNT) cnt.insts++;
             cur_inst = -1 - ref;
             const GInst I = sc.inst[cur_inst];
+            if (ANY && alpha) cur_mat = __float_as_int(I.f2.z);
             inst_to_object(I, O, D, o, d);
             inv = V(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
             blas_base = sp;
@@ -777,7 +801,8 @@ __device__ bool traverse(const DevScene& sc, vec3 O, vec3 D, float tmax, Stack& 
                 float tt = dot(e1, qv) / det;
                 float ww = 1.0f - uu - vv;
                 if (uu >= 0.0f && vv >= 0.0f && tt >= 0.0f && ww >= 0.0f && tt < best) {
-                    if (ANY) return true;
+                    if (ANY && !(alpha && alpha_transparent(sc, first + i, cur_mat, uu, vv, ww, rng))) return true;
+                    if (ANY) continue;
                     best = tt;
                     h.t = tt; h.u = uu; h.v = vv; h.tri = first + i; h.inst = cur_inst; h.light = -1;
                 }

@@ -38,10 +38,36 @@ struct AdjBuf {
     int lds;              // 1: accumulate per block in LDS
 };
 
+// ---- batched render of many jobs (one scene, own camera / size / depth each)
+// One job of a batch.
+struct JobDev {
+    float cam_pos[3], cam_up[3], cam_right[3], cam_fwd[3];
+    float cam_fov, cam_focal, cam_aperture;
+    int width, height, max_depth, npix;  // npix: pixel items (8x8 tiles * 64)
+    long long accum_off;                 // float4 offset of the job's image in the batch accumulation
+};
+// One sample of one job: npix consecutive path slots of a pass starting at slot0.
+struct PlaneDev { int job, sample, slot0, pad; };
+// The planes of one job inside one pass (for the accumulation kernel).
+struct SegDev { int job, plane0, nplanes, thread0; };
+// One pass: planes [plane0, plane0 + nplanes), npaths slots, segments [seg0, seg0 + nsegs).
+struct PassDev { int plane0, nplanes, npaths, seg0, nsegs, nthreads, pad0, pad1; };
+// Pass state read by the kernels: cur[0] = index of the pass being run,
+// cur[1..8) = a copy of that pass's PassDev (written by k_pass_begin).
+struct BatchBuf {
+    const JobDev* jobs;
+    const PlaneDev* planes;
+    const SegDev* segs;
+    const PassDev* passes;
+    int* cur;
+    float4* accum;
+};
+
 }  // namespace ptd
 
 namespace ptk {
-enum { KIND_PLAIN = 0, KIND_EXTEND = 1, KIND_SHADE = 2, KIND_SHADOW = 3, KIND_SHADE_ADJ = 4, KIND_SHADOW_ADJ = 5 };
+enum { KIND_PLAIN = 0, KIND_EXTEND = 1, KIND_SHADE = 2, KIND_SHADOW = 3, KIND_SHADE_ADJ = 4, KIND_SHADOW_ADJ = 5,
+       KIND_SHADE_BATCH = 6 };
 
 // Resident blocks for a persistent launch (occupancy x CUs); extra_lds is the
 // adjoint's per-block gradient table.
@@ -61,6 +87,17 @@ void launch_shadow(hipStream_t s, int blocks, int bs, const ptd::DevScene& sc, c
                    const ptd::PathBuf& pb, const ptd::ShadowBuf& sb, const int* count_in, int* work, int S,
                    int* spill, unsigned long long* counters, int* ovf, bool count, const ptd::AdjBuf* ad = nullptr);
 void launch_accum(hipStream_t s, const ptd::DevOpts& op, const ptd::PathBuf& pb, float4* accum, int npix, int nsamp);
+// batched render: the stage kernels of one pass, parameterised by device-side pass state
+void launch_pass_begin(hipStream_t s, const ptd::BatchBuf& bb, int* counters, int ncounters);
+void launch_pass_end(hipStream_t s, const ptd::BatchBuf& bb);
+void launch_extend_batch(hipStream_t s, int blocks, int bs, const ptd::DevScene& sc, const ptd::DevOpts& op,
+                         const ptd::PathBuf& pb, const ptd::BatchBuf& bb, int* hitq, int* hit_count, int* work, int S,
+                         int* spill, int* ovf);
+void launch_shade_batch(hipStream_t s, int blocks, int bs, const ptd::DevScene& sc, const ptd::DevOpts& op,
+                        const ptd::PathBuf& pb, const ptd::ShadowBuf& sb, const ptd::BatchBuf& bb, bool primary,
+                        const int* hitq, const int* hit_count, int* qout, int* count_out, int* shadow_count, int* work,
+                        bool compact);
+void launch_accum_batch(hipStream_t s, int blocks, const ptd::PathBuf& pb, const ptd::BatchBuf& bb);
 void launch_fill(hipStream_t s, int* p, int n, int v);
 void launch_query_closest(hipStream_t s, const ptd::DevScene& sc, const ptd::DevOpts& op, const float* o,
                           const float* d, int n, float* t, int* inst, int* tri, int* light, float* uv, int S,

@@ -87,12 +87,15 @@ __device__ void camera_ray(const DevOpts& op, int px, int py, Rng& rng, vec3& ro
     ro = V(op.cam_pos[0], op.cam_pos[1], op.cam_pos[2]) + ap;
 }
 
+// AnyHit (anyhit.glsl:25-215): lights, then the BVH; with OPT_ALPHA_TEST and
+// no media, alpha-tested candidates are skipped (BLEND draws from `rng`).
 template <bool COUNT>
 __device__ __forceinline__ bool occluded(const DevScene& sc, const DevOpts& op, vec3 ro, vec3 rd, float maxDist,
-                                         Stack& stk, TravCount& cnt, bool& ovf) {
+                                         Stack& stk, TravCount& cnt, bool& ovf, Rng* rng = nullptr) {
     if ((op.flags & OPT_LIGHTS) && any_light<COUNT>(sc, ro, rd, maxDist, cnt)) return true;
     Hit h;
-    return traverse<true, COUNT>(sc, ro, rd, maxDist, stk, h, cnt, ovf);
+    const bool alpha = (op.flags & OPT_ALPHA_TEST) && !(op.flags & OPT_MEDIUM);
+    return traverse<true, COUNT>(sc, ro, rd, maxDist, stk, h, cnt, ovf, alpha, rng);
 }
 
 template <bool COUNT>
@@ -255,11 +258,11 @@ __device__ vec3 direct_light(const DevScene& sc, const DevOpts& op, vec3 rd_in, 
     vec3 Ld = V1(0.0f);
     if ((op.flags & OPT_ENVMAP) && !(op.flags & OPT_UNIFORM_LIGHT)) {
         NeeRay r = nee_setup(sc, op, 0, rd_in, st, m, eta, rng);
-        if (r.valid && !occluded<COUNT>(sc, op, r.o, r.d, r.maxd, stk, cnt, ovf)) Ld = Ld + r.contrib;
+        if (r.valid && !occluded<COUNT>(sc, op, r.o, r.d, r.maxd, stk, cnt, ovf, &rng)) Ld = Ld + r.contrib;
     }
     if (op.flags & OPT_LIGHTS) {
         NeeRay r = nee_setup(sc, op, 1, rd_in, st, m, eta, rng);
-        if (r.valid && !occluded<COUNT>(sc, op, r.o, r.d, r.maxd, stk, cnt, ovf)) Ld = Ld + r.contrib;
+        if (r.valid && !occluded<COUNT>(sc, op, r.o, r.d, r.maxd, stk, cnt, ovf, &rng)) Ld = Ld + r.contrib;
     }
     return Ld;
 }
@@ -431,11 +434,7 @@ __device__ float4 path_trace(const DevScene& sc, const DevOpts& op, PathRegs& p,
             if ((op.flags & OPT_MEDIUM) && (op.flags & OPT_VOL_MIS)) {
                 Ld = direct_light_volmis<COUNT>(sc, op, p.rd, st, m, eta, p.med, ev == EV_SURFACE, p.rng, stk, cnt, ovf);
             } else {
-                NeeRay nee[2];
-                nee_rays(sc, op, p, p.rng, st, m, eta, nee);
-                for (int k = 0; k < 2; ++k)
-                    if (nee[k].valid && !occluded<COUNT>(sc, op, nee[k].o, nee[k].d, nee[k].maxd, stk, cnt, ovf))
-                        Ld = Ld + nee[k].contrib;
+                Ld = direct_light<COUNT>(sc, op, p.rd, st, m, eta, p.rng, stk, cnt, ovf);
             }
             p.rad = p.rad + Ld * p.thr;
         }
@@ -523,22 +522,57 @@ __device__ __forceinline__ void primary_path(const DevOpts& op, int px, int py, 
     camera_ray(op, px, py, p.rng, p.ro, p.rd);
 }
 
+// ---- batched render: slot -> (job, sample, pixel item) of the current pass
+__device__ __forceinline__ void batch_decode(const BatchBuf& bb, int slot, int& job, int& sample, int& item) {
+    const int p0 = bb.cur[1], np = bb.cur[2];
+    int lo = p0, hi = p0 + np - 1;  // last plane with slot0 <= slot
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (bb.planes[mid].slot0 <= slot) lo = mid;
+        else hi = mid - 1;
+    }
+    const PlaneDev pl = bb.planes[lo];
+    job = pl.job;
+    sample = pl.sample;
+    item = slot - pl.slot0;
+}
+
+// Options of one job: shared switches, the job's camera, size and depth.
+__device__ __forceinline__ DevOpts job_opts(const DevOpts& op, const JobDev& J) {
+    DevOpts o = op;
+    o.width = J.width;
+    o.height = J.height;
+    o.max_depth = J.max_depth;
+    for (int i = 0; i < 3; ++i) {
+        o.cam_pos[i] = J.cam_pos[i];
+        o.cam_up[i] = J.cam_up[i];
+        o.cam_right[i] = J.cam_right[i];
+        o.cam_fwd[i] = J.cam_fwd[i];
+    }
+    o.cam_fov = J.cam_fov;
+    o.cam_focal = J.cam_focal;
+    o.cam_aperture = J.cam_aperture;
+    return o;
+}
+
 // Extend: closest hit of every queued path. Misses finish here (environment
 // or uniform light, pathtrace.glsl:305-339) and write only their radiance;
 // hits are compacted into the shade queue. At bounce 0 (PRIMARY) the input is
-// every slot of the pass and the camera ray is generated in place.
-template <bool COUNT, bool PRIMARY>
+// every slot of the pass and the camera ray is generated in place; BATCH
+// (bounce 0 of a batched pass) takes the slot count and each slot's job,
+// sample and pixel from the device-side pass state.
+template <bool COUNT, bool PRIMARY, bool BATCH>
 __global__ void __launch_bounds__(256) k_extend(DevScene sc, DevOpts op, PathBuf pb, const int* __restrict__ qin,
                                                 const int* count_in, int npaths, int npix, int first_sample,
                                                 int* hitq, int* hit_count, int* work, int S, int* spill,
-                                                unsigned long long* counters, int* ovf_flag) {
+                                                unsigned long long* counters, int* ovf_flag, BatchBuf bb) {
     extern __shared__ int lds[];
     Stack stk{lds, spill, static_cast<int>(threadIdx.x), static_cast<int>(blockDim.x), S,
               blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x,
               static_cast<size_t>(gridDim.x) * blockDim.x};
     TravCount cnt{0, 0, 0, 0, 0, 0};
     bool ovf = false;
-    const int count = PRIMARY ? npaths : *count_in;
+    const int count = PRIMARY ? (BATCH ? bb.cur[3] : npaths) : *count_in;
     for (int base = first_batch(); base < count; base = next_batch(work)) {
         const int i = base + lane_id();
         bool keep = false;
@@ -547,7 +581,14 @@ __global__ void __launch_bounds__(256) k_extend(DevScene sc, DevOpts op, PathBuf
             slot = PRIMARY ? i : qin[i];
             PathRegs p;
             bool valid = true;
-            if (PRIMARY) {
+            if (PRIMARY && BATCH) {
+                int job, smp, item, px, py;
+                batch_decode(bb, slot, job, smp, item);
+                const DevOpts oj = job_opts(op, bb.jobs[job]);
+                item_pixel(item, oj.width, px, py);
+                valid = px < oj.width && py < oj.height;
+                if (valid) primary_path(oj, px, py, smp, p);
+            } else if (PRIMARY) {
                 int s, px, py;
                 valid = slot_pixel(op, slot, npix, s, px, py);
                 if (valid) primary_path(op, px, py, first_sample + s, p);
@@ -663,11 +704,11 @@ __device__ __forceinline__ void adjoint_vertex(const DevScene& sc, const DevOpts
 // compacted into the next extend queue; ending paths write only radiance.
 // ADJ: the replay pass of the adjoint; same path decisions, plus parameter
 // gradients of every surface vertex.
-template <bool PRIMARY, bool ADJ>
-__global__ void __launch_bounds__(256) k_shade(DevScene sc, DevOpts op, PathBuf pb, ShadowBuf sb,
+template <bool PRIMARY, bool ADJ, bool BATCH>
+__global__ void __launch_bounds__(256) k_shade(DevScene sc, DevOpts op0, PathBuf pb, ShadowBuf sb,
                                                const int* __restrict__ hitq, const int* hit_count, int npix,
                                                int first_sample, int* qout, int* count_out, int* shadow_count,
-                                               int* work, int compact, AdjBuf ad) {
+                                               int* work, int compact, AdjBuf ad, BatchBuf bb) {
     extern __shared__ int lds[];
     GradAcc acc{ADJ && ad.lds ? reinterpret_cast<float*>(lds) : nullptr, ad.grad, ad.num_mats * NPARAM};
     if (ADJ) acc.init();
@@ -688,13 +729,22 @@ __global__ void __launch_bounds__(256) k_shade(DevScene sc, DevOpts op, PathBuf 
         if (i < count) {
             slot = hitq[i];
             PathRegs p;
-            if (PRIMARY) {
+            DevOpts op = op0;
+            if (BATCH) {
+                int job, smp, item;
+                batch_decode(bb, slot, job, smp, item);
+                op = job_opts(op0, bb.jobs[job]);
+                if (PRIMARY) {
+                    int px, py;
+                    item_pixel(item, op.width, px, py);
+                    primary_path(op, px, py, smp, p);
+                }
+            } else if (PRIMARY) {
                 int s, px, py;
                 slot_pixel(op, slot, npix, s, px, py);
                 primary_path(op, px, py, first_sample + s, p);
-            } else {
-                load_path(pb, slot, p);
             }
+            if (!PRIMARY) load_path(pb, slot, p);
             const float4 h4 = pb.hit[slot];
             const int2 h2 = pb.hit2[slot];
             Hit h;
@@ -811,6 +861,46 @@ __global__ void __launch_bounds__(256) k_accum(DevOpts op, PathBuf pb, float4* _
         acc.x += r.x; acc.y += r.y; acc.z += r.z; acc.w += r.w;
     }
     *a = acc;
+}
+
+// ---- batched render: pass bookkeeping and accumulation
+// First node of a batched pass: make pass cur[0] current (copy its PassDev to
+// cur[1..8)) and zero the pass's queue and work counters.
+__global__ void __launch_bounds__(256) k_pass_begin(BatchBuf bb, int* counters, int ncounters) {
+    for (int i = threadIdx.x; i < ncounters; i += blockDim.x) counters[i] = 0;
+    if (threadIdx.x < 8) {
+        const int* src = reinterpret_cast<const int*>(bb.passes + bb.cur[0]);
+        bb.cur[1 + threadIdx.x] = threadIdx.x < 7 ? src[threadIdx.x] : 0;
+    }
+}
+
+// Last node of a batched pass: advance to the next pass.
+__global__ void k_pass_end(BatchBuf bb) {
+    if (threadIdx.x == 0) bb.cur[0] = bb.cur[0] + 1;
+}
+
+// Accumulation of a batched pass: one thread per (job, pixel item) of the
+// pass; each adds its pixel's samples of this pass in sample order, so every
+// job's image is bit-identical to rendering that job alone.
+__global__ void __launch_bounds__(256) k_accum_batch(PathBuf pb, BatchBuf bb) {
+    const int seg0 = bb.cur[4], nsegs = bb.cur[5], nthreads = bb.cur[6];
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nthreads; t += gridDim.x * blockDim.x) {
+        int k = seg0;
+        while (k + 1 < seg0 + nsegs && bb.segs[k + 1].thread0 <= t) ++k;
+        const SegDev sg = bb.segs[k];
+        const JobDev J = bb.jobs[sg.job];
+        const int item = t - sg.thread0;
+        int px, py;
+        item_pixel(item, J.width, px, py);
+        if (px >= J.width || py >= J.height) continue;
+        float4* a = bb.accum + J.accum_off + static_cast<size_t>(py) * J.width + px;
+        float4 acc = *a;
+        for (int q = 0; q < sg.nplanes; ++q) {
+            const float4 r = pb.rad[bb.planes[sg.plane0 + q].slot0 + item];
+            acc.x += r.x; acc.y += r.y; acc.z += r.z; acc.w += r.w;
+        }
+        *a = acc;
+    }
 }
 
 __global__ void k_fill_int(int* p, int n, int v) {
@@ -931,10 +1021,11 @@ int persistent_blocks(int kind, int bs, int S, bool count, int num_cu, size_t ex
     int per_cu = 1;
     switch (kind) {
         case KIND_PLAIN: per_cu = count ? max_blocks(k_plain<true>, bs, lds) : max_blocks(k_plain<false>, bs, lds); break;
-        case KIND_EXTEND: per_cu = count ? max_blocks(k_extend<true, false>, bs, lds) : max_blocks(k_extend<false, false>, bs, lds); break;
+        case KIND_EXTEND: per_cu = count ? max_blocks(k_extend<true, false, false>, bs, lds) : max_blocks(k_extend<false, false, false>, bs, lds); break;
         case KIND_SHADOW: per_cu = count ? max_blocks(k_shadow<true, false>, bs, lds) : max_blocks(k_shadow<false, false>, bs, lds); break;
-        case KIND_SHADE: per_cu = max_blocks(k_shade<false, false>, bs, 0); break;
-        case KIND_SHADE_ADJ: per_cu = max_blocks(k_shade<false, true>, bs, extra_lds); break;
+        case KIND_SHADE: per_cu = max_blocks(k_shade<false, false, false>, bs, 0); break;
+        case KIND_SHADE_ADJ: per_cu = max_blocks(k_shade<false, true, false>, bs, extra_lds); break;
+        case KIND_SHADE_BATCH: per_cu = max_blocks(k_shade<false, false, true>, bs, 0); break;
         case KIND_SHADOW_ADJ: per_cu = max_blocks(k_shadow<false, true>, bs, lds + extra_lds); break;
         default: break;
     }
@@ -953,7 +1044,8 @@ void launch_extend(hipStream_t s, int blocks, int bs, const DevScene& sc, const 
                    int* hitq, int* hit_count, int* work, int S, int* spill, unsigned long long* counters, int* ovf,
                    bool count) {
     const size_t lds = static_cast<size_t>(S) * bs * sizeof(int);
-#define PT_EXT(C, P) hipLaunchKernelGGL((k_extend<C, P>), dim3(blocks), dim3(bs), lds, s, sc, op, pb, qin, count_in, npaths, npix, first_sample, hitq, hit_count, work, S, spill, counters, ovf)
+    const BatchBuf nb{};
+#define PT_EXT(C, P) hipLaunchKernelGGL((k_extend<C, P, false>), dim3(blocks), dim3(bs), lds, s, sc, op, pb, qin, count_in, npaths, npix, first_sample, hitq, hit_count, work, S, spill, counters, ovf, nb)
     if (count) { if (primary) PT_EXT(true, true); else PT_EXT(true, false); }
     else { if (primary) PT_EXT(false, true); else PT_EXT(false, false); }
 #undef PT_EXT
@@ -963,15 +1055,43 @@ void launch_shade(hipStream_t s, int blocks, int bs, const DevScene& sc, const D
                   const ShadowBuf& sb, bool primary, const int* hitq, const int* hit_count, int npix, int first_sample,
                   int* qout, int* count_out, int* shadow_count, int* work, bool compact, const AdjBuf* ad) {
     const int c = compact ? 1 : 0;
+    const BatchBuf nb{};
     if (ad) {
         const size_t lds = ad->lds ? static_cast<size_t>(ad->num_mats) * NPARAM * sizeof(float) : 0;
-        if (primary) hipLaunchKernelGGL((k_shade<true, true>), dim3(blocks), dim3(bs), lds, s, sc, op, pb, sb, hitq, hit_count, npix, first_sample, qout, count_out, shadow_count, work, c, *ad);
-        else hipLaunchKernelGGL((k_shade<false, true>), dim3(blocks), dim3(bs), lds, s, sc, op, pb, sb, hitq, hit_count, npix, first_sample, qout, count_out, shadow_count, work, c, *ad);
+        if (primary) hipLaunchKernelGGL((k_shade<true, true, false>), dim3(blocks), dim3(bs), lds, s, sc, op, pb, sb, hitq, hit_count, npix, first_sample, qout, count_out, shadow_count, work, c, *ad, nb);
+        else hipLaunchKernelGGL((k_shade<false, true, false>), dim3(blocks), dim3(bs), lds, s, sc, op, pb, sb, hitq, hit_count, npix, first_sample, qout, count_out, shadow_count, work, c, *ad, nb);
         return;
     }
     const AdjBuf none{};
-    if (primary) hipLaunchKernelGGL((k_shade<true, false>), dim3(blocks), dim3(bs), 0, s, sc, op, pb, sb, hitq, hit_count, npix, first_sample, qout, count_out, shadow_count, work, c, none);
-    else hipLaunchKernelGGL((k_shade<false, false>), dim3(blocks), dim3(bs), 0, s, sc, op, pb, sb, hitq, hit_count, npix, first_sample, qout, count_out, shadow_count, work, c, none);
+    if (primary) hipLaunchKernelGGL((k_shade<true, false, false>), dim3(blocks), dim3(bs), 0, s, sc, op, pb, sb, hitq, hit_count, npix, first_sample, qout, count_out, shadow_count, work, c, none, nb);
+    else hipLaunchKernelGGL((k_shade<false, false, false>), dim3(blocks), dim3(bs), 0, s, sc, op, pb, sb, hitq, hit_count, npix, first_sample, qout, count_out, shadow_count, work, c, none, nb);
+}
+
+void launch_pass_begin(hipStream_t s, const BatchBuf& bb, int* counters, int ncounters) {
+    hipLaunchKernelGGL(k_pass_begin, dim3(1), dim3(256), 0, s, bb, counters, ncounters);
+}
+
+void launch_pass_end(hipStream_t s, const BatchBuf& bb) {
+    hipLaunchKernelGGL(k_pass_end, dim3(1), dim3(64), 0, s, bb);
+}
+
+void launch_extend_batch(hipStream_t s, int blocks, int bs, const DevScene& sc, const DevOpts& op, const PathBuf& pb,
+                         const BatchBuf& bb, int* hitq, int* hit_count, int* work, int S, int* spill, int* ovf) {
+    const size_t lds = static_cast<size_t>(S) * bs * sizeof(int);
+    hipLaunchKernelGGL((k_extend<false, true, true>), dim3(blocks), dim3(bs), lds, s, sc, op, pb, nullptr, nullptr, 0, 0, 0, hitq, hit_count, work, S, spill, nullptr, ovf, bb);
+}
+
+void launch_shade_batch(hipStream_t s, int blocks, int bs, const DevScene& sc, const DevOpts& op, const PathBuf& pb,
+                        const ShadowBuf& sb, const BatchBuf& bb, bool primary, const int* hitq, const int* hit_count,
+                        int* qout, int* count_out, int* shadow_count, int* work, bool compact) {
+    const int c = compact ? 1 : 0;
+    const AdjBuf none{};
+    if (primary) hipLaunchKernelGGL((k_shade<true, false, true>), dim3(blocks), dim3(bs), 0, s, sc, op, pb, sb, hitq, hit_count, 0, 0, qout, count_out, shadow_count, work, c, none, bb);
+    else hipLaunchKernelGGL((k_shade<false, false, true>), dim3(blocks), dim3(bs), 0, s, sc, op, pb, sb, hitq, hit_count, 0, 0, qout, count_out, shadow_count, work, c, none, bb);
+}
+
+void launch_accum_batch(hipStream_t s, int blocks, const PathBuf& pb, const BatchBuf& bb) {
+    hipLaunchKernelGGL(k_accum_batch, dim3(blocks), dim3(256), 0, s, pb, bb);
 }
 
 void launch_shadow(hipStream_t s, int blocks, int bs, const DevScene& sc, const DevOpts& op, const PathBuf& pb,

@@ -92,6 +92,7 @@ struct pt_renderer_s {
     bool has_scene = false;
     int num_tris = 0, num_instances = 0, num_materials = 0;
     bool any_alpha = false, any_medium = false, has_env = false;
+    bool any_blend = false;  // a material with alphaMode BLEND (stochastic alpha)
     // host copy of the flat node array (for TLAS refresh) and node re-layout map
     std::vector<pt_bvh_node> flat;
     int top_level_index = 0;
@@ -111,6 +112,13 @@ struct pt_renderer_s {
     DevBuf<float4> s_o, s_d, s_c;
     DevBuf<float4> a_rad, a_sg;   // adjoint: replay radiance, shadow-ray gradient records
     DevBuf<float> a_dimg, a_grad; // adjoint: dLoss/dimage, gradient sums
+    // batched render: job / plane / segment / pass tables, pass state, images
+    DevBuf<ptd::JobDev> b_jobs;
+    DevBuf<ptd::PlaneDev> b_planes;
+    DevBuf<ptd::SegDev> b_segs;
+    DevBuf<ptd::PassDev> b_passes;
+    DevBuf<int> b_cur;
+    DevBuf<float4> b_accum;
     DevBuf<int> qa, qb, hq, counters;
     DevBuf<int> spill;
     DevBuf<unsigned long long> work_counts;  // [0,6) closest-hit work, [8,14) shadow work: nodes, tris, insts, lights, spills, rays
@@ -144,6 +152,7 @@ struct pt_renderer_s {
         p_ro.free(); p_rd.free(); p_thr.free(); p_rad.free(); p_aux.free(); p_med.free(); p_hit.free();
         p_rng.free(); p_meta.free(); p_hit2.free(); s_o.free(); s_d.free(); s_c.free();
         a_rad.free(); a_sg.free(); a_dimg.free(); a_grad.free();
+        b_jobs.free(); b_planes.free(); b_segs.free(); b_passes.free(); b_cur.free(); b_accum.free();
         work_counts.free(); ldr.free();
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -465,10 +474,11 @@ struct Wavefront {
 Wavefront wavefront_setup(pt_renderer_s* h, const DevOpts& op, int num_samples, size_t adj_lds) {
     if ((op.flags & ptd::OPT_VOL_MIS) && (op.flags & ptd::OPT_MEDIUM))
         throw ArgError(PT_ERR_UNSUPPORTED, "volume MIS is implemented by the plain kernel only (mode PT_MODE_PLAIN)");
-    if ((op.flags & ptd::OPT_ALPHA_TEST) && !(op.flags & ptd::OPT_MEDIUM)) {
-        // Stochastic alpha in AnyHit draws from the path's RNG stream between NEE
-        // and BSDF sampling, which the deferred shadow stage cannot reproduce.
-        throw ArgError(PT_ERR_UNSUPPORTED, "alpha-tested shadow rays need the plain kernel (mode PT_MODE_PLAIN)");
+    if ((op.flags & ptd::OPT_ALPHA_TEST) && !(op.flags & ptd::OPT_MEDIUM) && h->any_blend) {
+        // Stochastic (BLEND) alpha in AnyHit draws from the path's RNG stream
+        // between NEE and BSDF sampling, which the deferred shadow stage cannot
+        // reproduce; MASK alpha is deterministic and runs in the shadow stage.
+        throw ArgError(PT_ERR_UNSUPPORTED, "stochastic (blend) alpha in shadow rays needs the plain kernel (mode PT_MODE_PLAIN)");
     }
     Wavefront w;
     const int W = op.width, H = op.height;
@@ -590,6 +600,211 @@ void adjoint_wavefront(pt_renderer_s* h, int first_sample, int num_samples, floa
         run_bounces(h, op, w, pb, sb, w.npix * ns, first_sample + s0, nullptr);
         run_bounces(h, op, w, pb2, sb, w.npix * ns, first_sample + s0, &ad);
         HIPCHK(hipGetLastError());
+    }
+}
+
+// ---- batched render of uneven jobs
+// Plan: every (job, sample) is a plane of npix_job path slots; planes are
+// taken job by job, sample by sample, and packed into passes of at most
+// `budget` slots (a plane never splits). Each pass runs every bounce of the
+// deepest job; shallower paths end at their own depth.
+struct BatchPlan {
+    std::vector<ptd::JobDev> jobs;
+    std::vector<ptd::PlaneDev> planes;
+    std::vector<ptd::SegDev> segs;
+    std::vector<ptd::PassDev> passes;
+    size_t pixels = 0;    // sum of W*H over jobs
+    int max_paths = 0;    // largest pass
+    int max_depth = 0;
+};
+
+BatchPlan plan_batch(const pt_job* jobs, int n, long long budget) {
+    BatchPlan P;
+    long long max_npix = 0;
+    for (int j = 0; j < n; ++j) {
+        const pt_job& J = jobs[j];
+        if (J.width <= 0 || J.height <= 0 || J.spp <= 0 || J.max_depth < 0 || !J.accum_out)
+            throw ArgError(PT_ERR_INVALID_ARG, "job " + std::to_string(j) + ": invalid size, spp, depth or output");
+        if (static_cast<long long>(J.width) * J.height > (1LL << 26))
+            throw ArgError(PT_ERR_INVALID_ARG, "job " + std::to_string(j) + ": resolution too large");
+        ptd::JobDev d{};
+        for (int i = 0; i < 3; ++i) {
+            d.cam_pos[i] = J.camera.position[i];
+            d.cam_up[i] = J.camera.up[i];
+            d.cam_right[i] = J.camera.right[i];
+            d.cam_fwd[i] = J.camera.forward[i];
+        }
+        d.cam_fov = J.camera.fov;
+        d.cam_focal = J.camera.focalDist;
+        d.cam_aperture = J.camera.aperture;
+        d.width = J.width;
+        d.height = J.height;
+        d.max_depth = J.max_depth;
+        d.npix = ((J.width + 7) / 8) * ((J.height + 7) / 8) * 64;
+        d.accum_off = static_cast<long long>(P.pixels);
+        P.pixels += static_cast<size_t>(J.width) * J.height;
+        P.jobs.push_back(d);
+        max_npix = std::max<long long>(max_npix, d.npix);
+        P.max_depth = std::max(P.max_depth, J.max_depth);
+    }
+    budget = std::max(budget, max_npix);
+    if (budget > INT32_MAX / 2) throw ArgError(PT_ERR_OVERFLOW, "too many paths per pass");
+    ptd::PassDev cur{};
+    long long used = 0;
+    auto close_pass = [&]() {
+        if (cur.nplanes == 0) return;
+        cur.npaths = static_cast<int>(used);
+        cur.seg0 = static_cast<int>(P.segs.size());
+        int thread0 = 0;
+        for (int q = cur.plane0; q < cur.plane0 + cur.nplanes; ++q) {
+            const ptd::PlaneDev& pl = P.planes[q];
+            if (P.segs.size() > static_cast<size_t>(cur.seg0) && P.segs.back().job == pl.job) {
+                P.segs.back().nplanes++;
+            } else {
+                P.segs.push_back({pl.job, q, 1, thread0});
+                thread0 += P.jobs[pl.job].npix;
+            }
+        }
+        cur.nsegs = static_cast<int>(P.segs.size()) - cur.seg0;
+        cur.nthreads = thread0;
+        P.max_paths = std::max(P.max_paths, cur.npaths);
+        P.passes.push_back(cur);
+        cur = ptd::PassDev{};
+        cur.plane0 = static_cast<int>(P.planes.size());
+        used = 0;
+    };
+    cur.plane0 = 0;
+    for (int j = 0; j < n; ++j) {
+        for (int smp = 0; smp < jobs[j].spp; ++smp) {
+            const int np = P.jobs[j].npix;
+            if (used + np > budget) close_pass();
+            P.planes.push_back({j, smp, static_cast<int>(used), 0});
+            cur.nplanes++;
+            used += np;
+        }
+    }
+    close_pass();
+    return P;
+}
+
+void batch_render(pt_renderer_s* h, const pt_job* jobs, int n) {
+    DevOpts op = h->dev_opts();
+    if ((op.flags & ptd::OPT_VOL_MIS) && (op.flags & ptd::OPT_MEDIUM))
+        throw ArgError(PT_ERR_UNSUPPORTED, "volume MIS is implemented by the plain kernel only");
+    if ((op.flags & ptd::OPT_ALPHA_TEST) && !(op.flags & ptd::OPT_MEDIUM) && h->any_blend)
+        throw ArgError(PT_ERR_UNSUPPORTED, "stochastic (blend) alpha in shadow rays needs the plain kernel");
+    // pass budget: wave_spp samples of the largest job
+    long long max_npix = 0;
+    for (int j = 0; j < n; ++j)
+        max_npix = std::max<long long>(max_npix, static_cast<long long>((jobs[j].width + 7) / 8) * ((jobs[j].height + 7) / 8) * 64);
+    const BatchPlan P = plan_batch(jobs, n, std::max(1, h->opts.wave_spp) * max_npix);
+    const size_t max_paths = static_cast<size_t>(P.max_paths);
+    h->p_ro.ensure(max_paths); h->p_rd.ensure(max_paths); h->p_thr.ensure(max_paths); h->p_rad.ensure(max_paths);
+    h->p_rng.ensure(max_paths); h->p_meta.ensure(max_paths); h->p_aux.ensure(max_paths);
+    if (h->any_medium) h->p_med.ensure(max_paths);
+    h->p_hit.ensure(max_paths); h->p_hit2.ensure(max_paths);
+    h->s_o.ensure(2 * max_paths); h->s_d.ensure(2 * max_paths); h->s_c.ensure(2 * max_paths);
+    h->qa.ensure(max_paths); h->qb.ensure(max_paths); h->hq.ensure(max_paths);
+    const int D = P.max_depth + 1 + ((op.flags & ptd::OPT_ALPHA_TEST) ? 64 : 0);
+    const int ncells = counter_cells(D);
+    h->counters.ensure(ncells);
+    h->ensure_misc();
+    const int bs = h->block_size(), S = h->stack_S();
+    const int b_ext = ptk::persistent_blocks(ptk::KIND_EXTEND, bs, S, false, h->num_cu);
+    const int b_sha = ptk::persistent_blocks(ptk::KIND_SHADE_BATCH, bs, S, false, h->num_cu);
+    const int b_sdw = ptk::persistent_blocks(ptk::KIND_SHADOW, bs, S, false, h->num_cu);
+    const int b_acc = 4 * h->num_cu;
+    h->ensure_spill(static_cast<size_t>(std::max(b_ext, b_sdw)) * bs);
+    hipStream_t s = h->stream;
+    h->b_jobs.upload(P.jobs.data(), P.jobs.size(), s);
+    h->b_planes.upload(P.planes.data(), P.planes.size(), s);
+    h->b_segs.upload(P.segs.data(), P.segs.size(), s);
+    h->b_passes.upload(P.passes.data(), P.passes.size(), s);
+    h->b_cur.ensure(16);
+    h->b_accum.ensure(P.pixels);
+    HIPCHK(hipMemsetAsync(h->b_cur.p, 0, 16 * sizeof(int), s));
+    HIPCHK(hipMemsetAsync(h->b_accum.p, 0, P.pixels * sizeof(float4), s));
+    const ptd::BatchBuf bb{h->b_jobs.p, h->b_planes.p, h->b_segs.p, h->b_passes.p, h->b_cur.p, h->b_accum.p};
+    const ptd::PathBuf pb = path_buf(h);
+    const ptd::ShadowBuf sb{h->s_o.p, h->s_d.p, h->s_c.p};
+    int* cnt = h->counters.p;
+    const int cadence = std::max(1, h->opts.compact_every);
+    const bool graph = h->opts.use_graph != 0;
+
+    // the stage launches of one pass (the same for every pass: pass-specific
+    // values are read from the device-side pass state)
+    auto one_pass = [&](bool timed) {
+        hipEvent_t ev;
+        auto stage = [&](int id, auto&& launch) {
+            if (timed) h->stage_begin(id, ev);
+            launch();
+            if (timed) h->stage_end(id, ev);
+        };
+        ptk::launch_pass_begin(s, bb, cnt, ncells);
+        int* qin = h->qa.p;
+        int* qout = h->qb.p;
+        int* c_ext = cnt;
+        for (int b = 0; b < D; ++b) {
+            int* c_hit = cnt + (D + 1) + b;
+            int* c_sh = cnt + 2 * (D + 1) + b;
+            int* wk = cnt + 3 * (D + 1) + 3 * b;
+            const bool primary = b == 0;
+            const bool compact = primary || (b % cadence) == 0;
+            int* c_next = compact ? cnt + b + 1 : c_ext;
+            stage(PT_STAGE_EXTEND, [&] {
+                if (primary) ptk::launch_extend_batch(s, b_ext, bs, h->sc, op, pb, bb, h->hq.p, c_hit, wk, S, h->spill.p, h->ovf.p);
+                else ptk::launch_extend(s, b_ext, bs, h->sc, op, pb, false, qin, c_ext, 0, 0, 0, h->hq.p, c_hit, wk, S,
+                                        h->spill.p, h->work_counts.p, h->ovf.p, false);
+            });
+            stage(PT_STAGE_SHADE, [&] {
+                ptk::launch_shade_batch(s, b_sha, bs, h->sc, op, pb, sb, bb, primary, h->hq.p, c_hit, qout, c_next, c_sh,
+                                        wk + 1, compact);
+            });
+            stage(PT_STAGE_SHADOW, [&] {
+                ptk::launch_shadow(s, b_sdw, bs, h->sc, op, pb, sb, c_sh, wk + 2, S, h->spill.p, h->work_counts.p + 8,
+                                   h->ovf.p, false);
+            });
+            if (compact) {
+                std::swap(qin, qout);
+                c_ext = c_next;
+            }
+        }
+        stage(PT_STAGE_ACCUM, [&] { ptk::launch_accum_batch(s, b_acc, pb, bb); });
+        ptk::launch_pass_end(s, bb);
+    };
+
+    const int npasses = static_cast<int>(P.passes.size());
+    if (graph) {
+        hipGraph_t g = nullptr;
+        hipGraphExec_t ge = nullptr;
+        HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        one_pass(false);
+        hipError_t ec = hipStreamEndCapture(s, &g);
+        if (ec != hipSuccess) throw HipError(std::string("hipStreamEndCapture: ") + hipGetErrorString(ec));
+        ec = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+        if (ec != hipSuccess) {
+            (void)hipGraphDestroy(g);
+            throw HipError(std::string("hipGraphInstantiate: ") + hipGetErrorString(ec));
+        }
+        hipEvent_t ev;
+        h->stage_begin(PT_STAGE_BATCH, ev);
+        for (int p = 0; p < npasses; ++p) ec = ec == hipSuccess ? hipGraphLaunch(ge, s) : ec;
+        h->stage_end(PT_STAGE_BATCH, ev);
+        HIPCHK(hipStreamSynchronize(s));
+        (void)hipGraphExecDestroy(ge);
+        (void)hipGraphDestroy(g);
+        if (ec != hipSuccess) throw HipError(std::string("hipGraphLaunch: ") + hipGetErrorString(ec));
+    } else {
+        hipEvent_t ev;
+        h->stage_begin(PT_STAGE_BATCH, ev);
+        for (int p = 0; p < npasses; ++p) one_pass(true);
+        h->stage_end(PT_STAGE_BATCH, ev);
+    }
+    HIPCHK(hipGetLastError());
+    for (int j = 0; j < n; ++j) {
+        const ptd::JobDev& J = P.jobs[j];
+        HIPCHK(hipMemcpyAsync(jobs[j].accum_out, h->b_accum.p + J.accum_off,
+                              static_cast<size_t>(J.width) * J.height * sizeof(float4), hipMemcpyDeviceToHost, s));
     }
 }
 
@@ -737,9 +952,10 @@ int pt_upload_scene(pt_handle h, const pt_scene_view* v) {
         h->mats.upload(reinterpret_cast<const float4*>(v->materials), 8 * static_cast<size_t>(v->num_materials), s);
         h->inst.upload(insts.data(), insts.size(), s);
         h->lights.upload(lights.data(), lights.size(), s);
-        h->any_alpha = h->any_medium = false;
+        h->any_alpha = h->any_medium = h->any_blend = false;
         for (int i = 0; i < v->num_materials; ++i) {
             if (static_cast<int>(v->materials[i].alphaMode) != 0) h->any_alpha = true;
+            if (static_cast<int>(v->materials[i].alphaMode) == ptd::ALPHA_MODE_BLEND) h->any_blend = true;
             if (static_cast<int>(v->materials[i].mediumType) != 0) h->any_medium = true;
         }
         if (v->num_textures > 0 && v->texture_maps) {
@@ -997,8 +1213,20 @@ int pt_any_hit(pt_handle h, const float* origins, const float* dirs, const float
 
 int pt_batch_render(pt_handle h, const pt_job* jobs, int32_t num_jobs) {
     return Guard(h, [&] {
-        (void)jobs; (void)num_jobs;
-        return Fail(h, PT_ERR_UNSUPPORTED, "pt_batch_render is not available in this build");
+        if (!h->has_scene) return Fail(h, PT_ERR_STATE, "no scene uploaded (pt_upload_scene)");
+        if (!h->has_opts) return Fail(h, PT_ERR_STATE, "no options set (pt_set_options)");
+        if (num_jobs < 0 || (num_jobs > 0 && !jobs)) return Fail(h, PT_ERR_INVALID_ARG, "invalid job list");
+        if (num_jobs == 0) return PT_OK;
+        HIPCHK(hipSetDevice(h->device));
+        h->resolve_events();
+        h->ensure_misc();
+        HIPCHK(hipMemsetAsync(h->ovf.p, 0, sizeof(int), h->stream));
+        HIPCHK(hipEventRecord(h->ev_r0, h->stream));
+        batch_render(h, jobs, num_jobs);
+        HIPCHK(hipEventRecord(h->ev_r1, h->stream));
+        h->render_pending = true;
+        h->resolve_events();
+        return h->stats.stack_overflow ? Fail(h, PT_ERR_OVERFLOW, "BVH traversal exceeded the maximum stack depth") : PT_OK;
     });
 }
 

@@ -1,11 +1,13 @@
 """Launch-configuration autotuner for the wavefront renderer.
 
 Sweeps workgroup size {64, 128, 256}, LDS traversal-stack depth {16, 24, 32},
-samples per wavefront pass and queue-compaction cadence on the actual scene,
-timing short renders on the GPU, and stores the fastest configuration as JSON
+samples per wavefront pass and queue-compaction cadence on the actual
+workload, timing it on the GPU, and stores the fastest configuration as JSON
 in glslpt/tuning/<key>.json, where bench.py and callers look it up.
 
-    python -m glslpt.autotune c2 [spp] [--quick]   # the 1080p instanced-spheres workload
+    python -m glslpt.autotune c2 [spp] [--quick]   # 1080p instanced spheres, pt_render
+    python -m glslpt.autotune c3 [--quick]         # 32 uneven Cornell jobs, pt_batch_render
+                                                   # with graph-captured passes
 """
 import itertools
 import json
@@ -63,9 +65,47 @@ def sweep(r, sc, rs, spp=16, reps=2, grid=None, log=print):
     return results
 
 
+def _time_batch(r, rs, jobs, reps, **kw):
+    r.set_options(options_for(rs, mode=api.PT_MODE_WAVEFRONT, use_graph=1, **kw))
+    r.batch_render(jobs)  # warm-up
+    ms = []
+    for _ in range(reps):
+        r.batch_render(jobs)
+        ms.append(r.stats()["ms_last_render"])
+    return min(ms) / 1e3
+
+
+def sweep_batch(r, rs, jobs, reps=1, grid=None, log=print):
+    grid = grid or dict(block_size=BLOCK_SIZES, stack_depth=STACK_DEPTHS, wave_spp=(2, 4, 8, 16),
+                        compact_every=(1, 2, 3))
+    keys = list(grid)
+    total = sum(j["width"] * j["height"] * j["spp"] for j in jobs)
+    results = []
+    for vals in itertools.product(*[grid[k] for k in keys]):
+        cfg = dict(zip(keys, vals))
+        dt = _time_batch(r, rs, jobs, reps, **cfg)
+        results.append({"config": cfg, "ms": dt * 1e3, "msamples_per_s": total / dt / 1e6})
+        log(json.dumps(results[-1]))
+    results.sort(key=lambda x: -x["msamples_per_s"])
+    return results
+
+
 def main(argv):
-    if len(argv) < 1 or argv[0] not in ("c2",):
-        sys.exit("usage: python -m glslpt.autotune c2 [spp]")
+    if len(argv) < 1 or argv[0] not in ("c2", "c3"):
+        sys.exit("usage: python -m glslpt.autotune c2 [spp] [--quick] | c3 [--quick]")
+    if argv[0] == "c3":
+        from .scenes import batch_jobs
+        sc, rs, jobs = batch_jobs()
+        r = Renderer(0)
+        r.upload_scene(sc)
+        grid = dict(block_size=(64, 256), stack_depth=(16, 24), wave_spp=(4, 8, 16), compact_every=(1, 2)) \
+            if "--quick" in argv else None
+        res = sweep_batch(r, rs, jobs, reps=2, grid=grid)
+        os.makedirs(TUNING_DIR, exist_ok=True)
+        with open(os.path.join(TUNING_DIR, "c3.json"), "w") as f:
+            json.dump({"key": "c3", "best": dict(res[0]["config"], use_graph=1), "results": res}, f, indent=1)
+        print("best", json.dumps(res[0]))
+        return
     spp = int(argv[1]) if len(argv) > 1 and argv[1].isdigit() else 64
     grid = QUICK if "--quick" in argv else None
     from .scenes import instanced_spheres

@@ -113,6 +113,23 @@ class Renderer:
     def reset_stats(self):
         self._check("pt_reset_stats", self._lib.pt_reset_stats(self._h))
 
+    # --- batched jobs ----------------------------------------------------
+    def batch_render(self, jobs):
+        """Render every job (dict: camera, width, height, spp, max_depth) with
+        the uploaded scene and the current options; returns each job's
+        accumulation (sum over its samples), (height, width, 4) float32."""
+        arr = (api.PtJob * len(jobs))()
+        outs = []
+        for i, j in enumerate(jobs):
+            out = np.zeros((j["height"], j["width"], 4), np.float32)
+            outs.append(out)
+            arr[i].camera = j["camera"]
+            arr[i].width, arr[i].height = j["width"], j["height"]
+            arr[i].spp, arr[i].max_depth = j["spp"], j["max_depth"]
+            arr[i].accum_out = api.fptr(out)
+        self._check("pt_batch_render", self._lib.pt_batch_render(self._h, arr, len(jobs)))
+        return outs
+
     # --- adjoint ---------------------------------------------------------
     def adjoint(self, first_sample, num_samples, dl_dimage):
         """Gradient of a loss with respect to the material parameters
@@ -157,6 +174,8 @@ def options_for(settings, mode=api.PT_MODE_WAVEFRONT, **kw):
                         enable_rr=int(settings.enable_rr), rr_depth=settings.rr_depth,
                         enable_uniform_light=int(settings.enable_uniform_light),
                         uniform_light_col=settings.uniform_light_col, mode=mode)
+    for k, v in settings.extra.items():
+        setattr(o, k, v)
     for k, v in kw.items():
         setattr(o, k, v)
     return o

@@ -229,6 +229,11 @@ class Scene:
             transforms=grab(v.transforms, v.num_instances, np.float32, 16),
             lights=grab(v.lights, v.num_lights, np.float32, 15),
         )
+        if v.num_textures > 0 and v.texture_maps:
+            n = v.num_textures * v.tex_width * v.tex_height * 4
+            buf = C.cast(v.texture_maps, C.POINTER(C.c_uint8 * n)).contents
+            out["textures"] = np.frombuffer(buf, np.uint8).reshape(v.num_textures, v.tex_height, v.tex_width, 4).copy()
+            out["tex_size"] = (v.tex_width, v.tex_height)
         if v.env_width > 0:
             out["env_img"] = grab(v.env_img, v.env_width * v.env_height, np.float32, 3)
             out["env_cdf"] = grab(v.env_cdf, v.env_width * v.env_height, np.float32, 1)[:, 0]

@@ -11,6 +11,8 @@
                          3-octave value noise, random rigid transforms and
                          uniform scale, Disney materials drawn from a seed,
                          rect + sphere lights and a constant environment.
+* ``batch_jobs``         32 uneven render jobs of the Cornell scene (own
+                         camera, resolution, depth and spp each).
 * ``material_boxes``     differentiable-materials workload: 256 boxes and 8
                          noise-displaced spheres (~200k triangles), 64
                          materials with seeded optimisable parameters.
@@ -243,6 +245,34 @@ def instanced_spheres(num_instances=64, freq=28, seed=0x5EED0002, width=1920, he
                               enable_uniform_light=True, uniform_light_col=(0.3, 0.3, 0.3))
 
 
+# ------------------------------------------------------------ batched jobs
+BATCH_RESOLUTIONS = ((128, 128), (256, 256), (512, 512), (1024, 1024), (1920, 1080))
+
+
+def batch_jobs(num_jobs=32, seed=0xBA7C0003, radius=10.0, fov=6.0, resolutions=BATCH_RESOLUTIONS,
+               depth_range=(2, 12), spp_range=(4, 32)):
+    """Batched uneven jobs over the Cornell scene: job j (seed + j) looks at
+    the room centre from a point uniform on a sphere of `radius` around it,
+    with a resolution drawn from `resolutions`, max depth uniform in
+    depth_range and spp uniform in spp_range (inclusive). The field of view
+    (degrees) is narrow enough for the ~0.56-unit room to fill the frame from
+    that distance. Returns (scene, settings, jobs); a job is a dict with
+    camera, width, height, spp, max_depth."""
+    sc, rs = cornell()
+    centre = np.array([0.278, 0.2744, 0.2796])
+    jobs = []
+    for j in range(num_jobs):
+        rng = np.random.Generator(np.random.PCG64(seed + j))
+        d = rng.normal(size=3)
+        d /= np.linalg.norm(d)
+        w, h = resolutions[int(rng.integers(len(resolutions)))]
+        depth = int(rng.integers(depth_range[0], depth_range[1] + 1))
+        spp = int(rng.integers(spp_range[0], spp_range[1] + 1))
+        sc.AddCamera(tuple(centre + radius * d), tuple(centre), fov)
+        jobs.append(dict(camera=sc.camera(), width=w, height=h, spp=spp, max_depth=depth))
+    return sc, rs, jobs
+
+
 # --------------------------------------------------- differentiable materials
 def _unit_box():
     """12 outward-facing triangles of the cube [-1, 1]^3 with flat normals."""
@@ -336,6 +366,138 @@ def material_boxes(num_boxes=256, sphere_freq=35, num_spheres=8, num_materials=6
     sc.ProcessScene()
     return sc, RenderSettings(width, height, spp, max_depth, enable_rr=True, rr_depth=2,
                               enable_uniform_light=True, uniform_light_col=(0.3, 0.3, 0.3))
+
+
+# ------------------------------------------------------ feature scenes
+def _uv_quad(corner, u, v, tiles=1.0):
+    """Two triangles of the parallelogram corner + [0,1]u + [0,1]v with UVs
+    in [0, tiles], facing along cross(u, v)."""
+    c = np.asarray(corner, np.float64)
+    u = np.asarray(u, np.float64)
+    v = np.asarray(v, np.float64)
+    p = [c, c + u, c + u + v, c + v]
+    t = [(0, 0), (tiles, 0), (tiles, tiles), (0, tiles)]
+    pos = np.array([[p[0], p[1], p[2]], [p[0], p[2], p[3]]], np.float32)
+    uvs = np.array([[t[0], t[1], t[2]], [t[0], t[2], t[3]]], np.float32)
+    n = np.cross(u, v)
+    n /= np.linalg.norm(n)
+    return pos, np.tile(n.astype(np.float32), (2, 3, 1)), uvs
+
+
+def sky_envmap(width=64, height=32, seed=0xE7A11, sun=(0.3, 0.22), sun_power=400.0):
+    """Procedural HDR sky (RGB float, rows top-down like stb's HDR loader):
+    a vertical gradient, faint value noise and a bright sun disc, so that
+    importance sampling matters."""
+    y = (np.arange(height) + 0.5) / height
+    x = (np.arange(width) + 0.5) / width
+    X, Y = np.meshgrid(x, y)
+    sky = np.stack([0.3 + 0.5 * (1 - Y), 0.4 + 0.5 * (1 - Y), 0.6 + 0.6 * (1 - Y)], -1)
+    rng = np.random.Generator(np.random.PCG64(seed))
+    sky *= 1.0 + 0.2 * rng.random((height, width, 1))
+    d2 = (X - sun[0]) ** 2 + ((Y - sun[1]) * height / width) ** 2
+    sky += (d2 < 0.0012)[..., None] * np.array([sun_power, 0.9 * sun_power, 0.7 * sun_power])
+    return sky.astype(np.float32)
+
+
+def _checker(n, a, b, cells=8, alpha=None):
+    i, j = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    m = ((i * cells // n) + (j * cells // n)) % 2
+    img = np.where(m[..., None] == 0, np.array(a), np.array(b)).astype(np.float64)
+    out = np.zeros((n, n, 4), np.uint8)
+    out[..., :3] = np.clip(img * 255 + 0.5, 0, 255).astype(np.uint8)
+    out[..., 3] = 255 if alpha is None else alpha
+    return out
+
+
+def feature_scene(kind, width=96, height=64, spp=4, max_depth=5, seed=0xFEA70005):
+    """Small scenes exercising one option path each (C5 features):
+
+    ``envmap``       importance-sampled HDR sky (OPT_ENVMAP), rotated, scaled
+    ``textures``     base-colour, metallic-roughness, normal and emission maps
+    ``media``        scatter / absorb / emissive homogeneous media in boxes
+                     (volume_cube.scene style: spectrans 1, blend opacity 0)
+    ``volmis``       the media scene with volume MIS (plain kernel only)
+    ``alpha_mask``   alpha-masked leaf quads (MASK: deterministic in AnyHit)
+    ``alpha_blend``  stochastic alpha (BLEND, plain kernel only)
+    """
+    rng = np.random.Generator(np.random.PCG64(seed))
+    sc = Scene()
+    extra = {}
+    floor_p, floor_n, floor_t = _uv_quad((-4, 0, 4), (8, 0, 0), (0, 0, -8), tiles=4.0)
+    wall_p, wall_n, wall_t = _uv_quad((-4, 0, -4), (8, 0, 0), (0, 5, 0), tiles=2.0)
+    floor = sc.AddMesh(floor_p, floor_n, floor_t, name="floor")
+    wall = sc.AddMesh(wall_p, wall_n, wall_t, name="wall")
+    bpos, bnrm = _unit_box()
+    box = sc.AddMesh(bpos, bnrm, name="box")
+    sph_p, sph_n = displaced_icosphere(6, 0.08, seed)
+    sph = sc.AddMesh(sph_p, sph_n, name="sphere")
+    grey = sc.AddMaterial(baseColor=(0.7, 0.7, 0.7), roughness=0.6)
+    m_floor, m_wall = grey, sc.AddMaterial(baseColor=(0.6, 0.5, 0.4), roughness=0.8)
+    objs = [sc.AddMaterial(baseColor=rng.uniform(0.2, 0.9, 3), roughness=rng.uniform(0.1, 0.8),
+                           metallic=float(k == 1), clearcoat=0.5 * (k == 2), ior=1.5) for k in range(3)]
+    if kind == "textures":
+        sc.SetTextureArraySize(64, 64)
+        t_base = sc.AddTexture(_checker(64, (0.9, 0.8, 0.2), (0.2, 0.3, 0.8)), "base")
+        mr = np.zeros((64, 64, 4), np.uint8)
+        mr[..., 1] = (rng.random((64, 64)) * 200 + 40).astype(np.uint8)   # roughness (g)
+        mr[..., 2] = np.where(rng.random((64, 64)) < 0.3, 255, 0)         # metallic (b)
+        mr[..., 3] = 255
+        t_mr = sc.AddTexture(mr, "metalrough")
+        i, j = np.meshgrid(np.arange(64), np.arange(64), indexing="ij")
+        nx, ny = 0.4 * np.sin(i / 64 * 6 * np.pi), 0.4 * np.cos(j / 64 * 4 * np.pi)
+        nz = np.sqrt(1 - nx * nx - ny * ny)
+        nm = np.stack([nx, ny, nz], -1) * 0.5 + 0.5
+        nmap = np.concatenate([(nm * 255 + 0.5).astype(np.uint8), np.full((64, 64, 1), 255, np.uint8)], -1)
+        t_n = sc.AddTexture(nmap, "normal")
+        t_e = sc.AddTexture(_checker(64, (1, 0.5, 0.1), (0, 0, 0), cells=4), "emission")
+        m_floor = sc.AddMaterial(baseColor=(1, 1, 1), roughness=0.5, baseColorTexId=t_base,
+                                 metallicRoughnessTexID=t_mr, normalmapTexID=t_n)
+        m_wall = sc.AddMaterial(baseColor=(0.8, 0.8, 0.8), roughness=0.4, normalmapTexID=t_n,
+                                emissionmapTexID=t_e)
+        extra["opengl_normal_map"] = 1
+    sc.AddMeshInstance(floor, mat4(), m_floor)
+    sc.AddMeshInstance(wall, mat4(), m_wall)
+    sc.AddMeshInstance(box, mat4(translate=(-1.6, 0.6, -0.5), scale=(0.6, 0.6, 0.6), rot=random_rotation(rng)),
+                       objs[0])
+    sc.AddMeshInstance(sph, mat4(translate=(1.5, 0.9, 0.0), scale=(0.9, 0.9, 0.9)), objs[1])
+    sc.AddMeshInstance(sph, mat4(translate=(0.0, 0.5, 1.5), scale=(0.5, 0.5, 0.5)), objs[2])
+    if kind in ("media", "volmis"):
+        # volume_cube.scene: refractive, fully blended-away surface around a medium
+        for k, (mt, col, dens, pos) in enumerate([(1, (0.9, 0.4, 0.1), 2.0, (-0.2, 1.0, -1.6)),
+                                                   (2, (0.2, 0.5, 0.9), 1.5, (1.8, 0.8, -2.0)),
+                                                   (3, (0.5, 0.2, 0.1), 0.6, (-2.2, 1.2, 1.6))]):
+            vm = sc.AddMaterial(baseColor=(1, 1, 1), specTrans=1.0, roughness=0.0, mediumType=mt,
+                                mediumDensity=dens, mediumColor=col, alphaMode=1, opacity=0.0,
+                                mediumAnisotropy=0.3 * k)
+            sc.AddMeshInstance(box, mat4(translate=pos, scale=(0.7, 0.7, 0.7)), vm)
+        if kind == "volmis":
+            extra["enable_volume_mis"] = 1
+    if kind in ("alpha_mask", "alpha_blend"):
+        sc.SetTextureArraySize(32, 32)
+        leaf = np.zeros((32, 32, 4), np.uint8)
+        i, j = np.meshgrid(np.arange(32), np.arange(32), indexing="ij")
+        leaf[..., 1] = 160
+        leaf[..., 0] = 60
+        leaf[..., 3] = np.where((i - 16) ** 2 + (j - 16) ** 2 * 2.5 < 180, 255, 0)
+        t_leaf = sc.AddTexture(leaf, "leaf")
+        q_p, q_n, q_t = _uv_quad((-0.5, 0, 0), (1, 0, 0), (0, 1, 0))
+        quad = sc.AddMesh(q_p, q_n, q_t, name="leafquad")
+        if kind == "alpha_mask":
+            am = sc.AddMaterial(baseColor=(1, 1, 1), roughness=0.7, baseColorTexId=t_leaf, alphaMode=2,
+                                alphaCutoff=0.5)
+        else:
+            am = sc.AddMaterial(baseColor=(0.4, 0.8, 0.3), roughness=0.7, alphaMode=1, opacity=0.45)
+        for k in range(10):
+            xf = mat4(translate=(rng.uniform(-3, 3), rng.uniform(0.2, 2.5), rng.uniform(-2.5, 2)),
+                      scale=(rng.uniform(0.6, 1.4),) * 3, rot=random_rotation(rng))
+            sc.AddMeshInstance(quad, xf, am)
+    sc.AddLight(quad_light((-1, 4.5, -1), (1, 4.5, -1), (-1, 4.5, 1), (9, 8, 7)))
+    if kind == "envmap":
+        sc.AddEnvMap(sky_envmap())
+        extra.update(enable_envmap=1, envmap_intensity=1.5, envmap_rot=30.0)
+    sc.AddCamera((0.0, 2.6, 7.5), (0.0, 0.8, 0.0), 50.0)
+    sc.ProcessScene()
+    return sc, RenderSettings(width, height, spp, max_depth, enable_rr=True, rr_depth=2, extra=extra)
 
 
 def tiny(width=32, height=32, spp=2, max_depth=3):

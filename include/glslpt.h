@@ -158,7 +158,8 @@ typedef struct pt_stats {
 #define PT_STAGE_SHADOW 3
 #define PT_STAGE_ACCUM 4
 #define PT_STAGE_PLAIN 5
-#define PT_STAGE_ADJOINT 6
+#define PT_STAGE_ADJOINT 6   /* replay shade stage of pt_adjoint */
+#define PT_STAGE_BATCH 7     /* whole pass sequence of pt_batch_render */
 
 typedef struct pt_renderer_s* pt_handle;
 
@@ -198,10 +199,20 @@ int pt_any_hit(pt_handle h, const float* origins, const float* dirs, const float
                int32_t n, int32_t* occluded);
 
 /* ---- batched uneven jobs: one launch sequence over many render jobs ---- */
+/* Renders every job (own camera, resolution, samples [0, spp) and max depth;
+ * scene and the other options shared) in passes that mix jobs: the path slots
+ * of a pass are (job, sample, pixel) planes, packed up to wave_spp samples
+ * worth of slots of the largest job.
+ * With pt_options.use_graph the stage launches of one pass are captured once
+ * in a HIP graph and the graph is launched once per pass. Each job's image is
+ * identical to rendering it alone with pt_render (same seeds, same order of
+ * accumulation). No reference counterpart: the reference renders one scene
+ * view at a time (Renderer::Render, src/core/Renderer.cpp:546-590). */
 typedef struct pt_job {
     pt_camera camera;
     int32_t width, height, spp, max_depth;
-    float* accum_out;        /* host RGBA32F buffer, width*height*4 floats, written (not added) */
+    float* accum_out;        /* host RGBA32F buffer, width*height*4 floats: the sum over the job's
+                                samples, written (not added), rows bottom-up */
 } pt_job;
 int pt_batch_render(pt_handle h, const pt_job* jobs, int32_t num_jobs);
 

@@ -6,13 +6,32 @@ import oracle
 from glslpt import scenes
 
 
+# RenderSettings.extra (pt_options fields) -> oracle option fields
+_EXTRA_TO_ORACLE = {
+    "enable_envmap": ("opt_envmap", int), "envmap_intensity": ("envmap_intensity", float),
+    "envmap_rot": ("envmap_rot", lambda deg: float(deg) / 360.0), "hide_emitters": ("opt_hide_emitters", int),
+    "enable_background": ("opt_background", int), "transparent_background": ("opt_transparent_background", int),
+    "enable_volume_mis": ("opt_vol_mis", int),
+    "enable_roughness_mollification": ("opt_roughness_mollification", int),
+    "roughness_mollification_amt": ("roughness_mollification_amt", float),
+    "opengl_normal_map": ("opt_opengl_normalmap", int),
+}
+
+
 def oracle_setup(sc, rs, **kw):
     a = sc.arrays()
-    sa = oracle.SceneArrays(a)
+    sa = oracle.SceneArrays(a, textures=a.get("textures"), tex_size=a.get("tex_size", (0, 0)))
     cam = oracle.camera_struct(sc.camera())
+    extra = {}
+    for k, v in rs.extra.items():
+        name, conv = _EXTRA_TO_ORACLE[k]
+        extra[name] = conv(v)
+    if rs.extra.get("enable_envmap") and "env_img" not in a:
+        extra["opt_envmap"] = 0  # no map uploaded: the renderer leaves OPT_ENVMAP off too
+    extra.update(kw)
     o = oracle.options(rs.width, rs.height, rs.max_depth, a["lights"].shape[0], rr=rs.enable_rr,
                        rr_depth=rs.rr_depth, uniform_light=rs.enable_uniform_light,
-                       uniform_col=rs.uniform_light_col, materials=a["materials"], **kw)
+                       uniform_col=rs.uniform_light_col, materials=a["materials"], **extra)
     return sa, cam, o
 
 

@@ -167,6 +167,7 @@ def options(width, height, max_depth, num_lights, rr=True, rr_depth=2, uniform_l
     o.opt_rr, o.rr_depth = int(rr), rr_depth
     o.opt_uniform_light = int(uniform_light)
     o.uniform_light_col[:] = list(uniform_col)
+    o.opt_opengl_normalmap = 1  # RenderOptions::openglNormalMap default (src/core/Renderer.h:57)
     if materials is not None:  # derived like InitShaders (src/core/Renderer.cpp:441-459)
         o.opt_alpha_test = int(np.any(materials[:, 29].astype(np.int32) != 0))
         o.opt_medium = int(np.any(materials[:, 18].astype(np.int32) != 0))
