@@ -966,18 +966,17 @@ __global__ void k_tonemap(const float4* __restrict__ accum, int w, int h, float 
                 vec3 r = num / den;
                 color = V(gclamp(r.x, 0.f, 1.f), gclamp(r.y, 0.f, 1.f), gclamp(r.z, 0.f, 1.f));
             } else {
-                // ACES fitted: input matrix, RRT+ODT fit, output matrix (row-vector * mat3).
-                const float im[3][3] = {{0.59719f, 0.35458f, 0.04823f}, {0.07600f, 0.90834f, 0.01566f},
-                                        {0.02840f, 0.13383f, 0.83777f}};
-                const float om[3][3] = {{1.60475f, -0.53108f, -0.07367f}, {-0.10208f, 1.10813f, -0.00605f},
-                                        {-0.00327f, -0.07276f, 1.07602f}};
-                vec3 c = V(dot(color, V(im[0][0], im[1][0], im[2][0])), dot(color, V(im[0][1], im[1][1], im[2][1])),
-                           dot(color, V(im[0][2], im[1][2], im[2][2])));
+                // ACES fitted: input matrix, RRT+ODT fit, output matrix. GLSL's
+                // `color * mat3(c0, c1, c2)` is (dot(color, c0), dot(color, c1), dot(color, c2)).
+                const vec3 i0 = V(0.59719f, 0.35458f, 0.04823f), i1 = V(0.07600f, 0.90834f, 0.01566f),
+                           i2 = V(0.02840f, 0.13383f, 0.83777f);
+                const vec3 o0 = V(1.60475f, -0.53108f, -0.07367f), o1 = V(-0.10208f, 1.10813f, -0.00605f),
+                           o2 = V(-0.00327f, -0.07276f, 1.07602f);
+                vec3 c = V(dot(color, i0), dot(color, i1), dot(color, i2));
                 vec3 aa = c * (c + V1(0.0245786f)) - V1(0.000090537f);
                 vec3 bb = c * (0.983729f * c + V1(0.4329510f)) + V1(0.238081f);
                 c = aa / bb;
-                c = V(dot(c, V(om[0][0], om[1][0], om[2][0])), dot(c, V(om[0][1], om[1][1], om[2][1])),
-                      dot(c, V(om[0][2], om[1][2], om[2][2])));
+                c = V(dot(c, o0), dot(c, o1), dot(c, o2));
                 color = V(gclamp(c.x, 0.f, 1.f), gclamp(c.y, 0.f, 1.f), gclamp(c.z, 0.f, 1.f));
             }
         } else {

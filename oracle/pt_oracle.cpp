@@ -1630,4 +1630,65 @@ int oracle_render_frozen(const oracle_scene* s, const float* eval_materials, con
                                  eval_materials);
 }
 
+// tonemap.glsl:59-133 for one RGBA32F accumulation image (rows bottom-up),
+// then the float -> unsigned byte conversion of glGetTexImage
+// (Renderer.cpp:619-634): clamp to [0, 1], round to nearest.
+int oracle_tonemap(const float* accum, int32_t w, int32_t h, float inv_spp, int32_t enable_tonemap,
+                   int32_t enable_aces, int32_t simple_aces, int32_t bg_mode, const float* bg, uint8_t* out) {
+    if (!accum || !out || w <= 0 || h <= 0) return -1;
+    auto clamp01 = [](float v) { return v < 0.0f ? 0.0f : (v > 1.0f ? 1.0f : v); };
+    auto dot3 = [](const float* a, const float* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; };
+    // mat3 constructor columns; color * M = (dot(color, c0), dot(color, c1), dot(color, c2))
+    static const float IN[3][3] = {{0.59719f, 0.35458f, 0.04823f}, {0.07600f, 0.90834f, 0.01566f},
+                                   {0.02840f, 0.13383f, 0.83777f}};
+    static const float OUT[3][3] = {{1.60475f, -0.53108f, -0.07367f}, {-0.10208f, 1.10813f, -0.00605f},
+                                    {-0.00327f, -0.07276f, 1.07602f}};
+    for (int y = 0; y < h; ++y) {
+        for (int x = 0; x < w; ++x) {
+            const float* a = accum + 4 * (static_cast<size_t>(y) * w + x);
+            float c[3] = {a[0] * inv_spp, a[1] * inv_spp, a[2] * inv_spp};
+            const float alpha = a[3] * inv_spp;
+            if (enable_tonemap) {
+                if (enable_aces) {
+                    if (simple_aces) {
+                        for (int k = 0; k < 3; ++k)
+                            c[k] = clamp01((c[k] * (2.51f * c[k] + 0.03f)) / (c[k] * (2.43f * c[k] + 0.59f) + 0.14f));
+                    } else {
+                        float t[3] = {dot3(c, IN[0]), dot3(c, IN[1]), dot3(c, IN[2])};
+                        for (int k = 0; k < 3; ++k) {
+                            float av = t[k] * (t[k] + 0.0245786f) - 0.000090537f;
+                            float bv = t[k] * (0.983729f * t[k] + 0.4329510f) + 0.238081f;
+                            t[k] = av / bv;
+                        }
+                        c[0] = clamp01(dot3(t, OUT[0]));
+                        c[1] = clamp01(dot3(t, OUT[1]));
+                        c[2] = clamp01(dot3(t, OUT[2]));
+                    }
+                } else {
+                    const float lum = 0.212671f * c[0] + 0.715160f * c[1] + 0.072169f * c[2];
+                    const float s = 1.0f / (1.0f + lum / 1.5f);
+                    for (int k = 0; k < 3; ++k) c[k] = c[k] * s;
+                }
+            }
+            for (int k = 0; k < 3; ++k) c[k] = std::pow(c[k], 1.0f / 2.2f);
+            float out_alpha = 1.0f;
+            float bgc[3] = {bg[0], bg[1], bg[2]};
+            if (bg_mode == 2) {
+                out_alpha = alpha;
+                const float fx = std::floor((static_cast<float>(x) + 0.5f) / 10.0f);
+                const float fy = std::floor((static_cast<float>(y) + 0.5f) / 10.0f);
+                const float m = std::fmod(fx + fy, 2.0f);
+                const float res = std::max(m > 0.0f ? 1.0f : (m < 0.0f ? -1.0f : 0.0f), 0.0f);
+                for (int k = 0; k < 3; ++k) bgc[k] = 0.1f * (1.0f - res) + 0.2f * res;
+            }
+            if (bg_mode != 0)
+                for (int k = 0; k < 3; ++k) c[k] = bgc[k] * (1.0f - alpha) + c[k] * alpha;
+            uint8_t* o = out + 4 * (static_cast<size_t>(y) * w + x);
+            for (int k = 0; k < 3; ++k) o[k] = static_cast<uint8_t>(clamp01(c[k]) * 255.0f + 0.5f);
+            o[3] = static_cast<uint8_t>(clamp01(out_alpha) * 255.0f + 0.5f);
+        }
+    }
+    return 0;
+}
+
 }  // extern "C"

@@ -69,6 +69,12 @@ int oracle_render_frozen(const oracle_scene* s, const float* eval_materials, con
                          const oracle_options* o, int32_t first_sample, int32_t num_samples, float* accum,
                          int32_t num_threads);
 
+/* tonemap.glsl (gamma 2.2, luminance / ACES / simple ACES fit, background
+ * and transparent-checker modes 0/1/2) plus glGetTexImage's float -> byte
+ * conversion: accum RGBA32F (w*h*4) -> out RGBA8 (w*h*4). */
+int oracle_tonemap(const float* accum, int32_t w, int32_t h, float inv_spp, int32_t enable_tonemap,
+                   int32_t enable_aces, int32_t simple_aces, int32_t bg_mode, const float* bg, uint8_t* out);
+
 /* Per-sample radiance (no accumulation): rgba[4 * (i)] for pixel list px/py,
  * sample index s[i]. */
 int oracle_trace_samples(const oracle_scene* s, const oracle_camera* cam, const oracle_options* o,

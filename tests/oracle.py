@@ -76,6 +76,7 @@ def lib():
         l.oracle_eval_fn.argtypes = [i32, i32, P(f32), P(f32)]
         l.oracle_render_frozen.argtypes = [P(OracleScene), P(f32), P(OracleCamera), P(OracleOptions), i32, i32,
                                            P(f32), i32]
+        l.oracle_tonemap.argtypes = [P(f32), i32, i32, f32, i32, i32, i32, i32, P(f32), P(C.c_uint8)]
         l.oracle_set_inverse_mode.argtypes = [i32]
         l.oracle_set_inverse_mode.restype = None
         _lib = l
@@ -206,6 +207,18 @@ def render_frozen(sa, eval_materials, cam, opts, first_sample, num_samples, thre
     if rc != 0:
         raise RuntimeError(f"oracle_render_frozen failed {rc}")
     return acc
+
+
+def tonemap(accum, num_samples, tonemap=True, aces=False, simple_aces=False, bg_mode=0, background=(1, 1, 1)):
+    a = np.ascontiguousarray(accum, np.float32)
+    h, w = a.shape[:2]
+    bg = np.asarray(background, np.float32)
+    out = np.zeros((h, w, 4), np.uint8)
+    rc = lib().oracle_tonemap(fp(a), w, h, np.float32(1.0) / np.float32(num_samples), int(tonemap), int(aces),
+                              int(simple_aces), int(bg_mode), fp(bg), out.ctypes.data_as(P(C.c_uint8)))
+    if rc != 0:
+        raise RuntimeError("oracle_tonemap failed")
+    return out
 
 
 def closest_hit(sa, opts, o, d, counters=False):

@@ -136,3 +136,27 @@ def test_plain_equals_wavefront(renderer, cadence, bs, stack):
     d = np.abs(out[0] - out[1]).max()
     print("plain vs wavefront max abs diff", d)
     assert rel_err(out[0], out[1], floor=1e-3).max() <= 1e-5
+
+
+@pytest.mark.parametrize("tm,aces,simple,bg", [(0, 0, 0, 0), (1, 0, 0, 0), (1, 1, 0, 0), (1, 1, 1, 0), (1, 0, 0, 1),
+                                               (1, 1, 0, 2)])
+def test_ldr_matches_oracle_tonemap(renderer, tm, aces, simple, bg):
+    """pt_read_ldr (tonemap kernel + byte conversion) against the oracle's
+    tonemap of the same accumulation buffer; bg 1 = background colour,
+    2 = transparent checker (alpha from the paths)."""
+    sc, rs = small_instanced()
+    renderer.upload_scene(sc)
+    renderer.set_options(options_for(rs, enable_background=int(bg == 1), transparent_background=int(bg == 2)))
+    renderer.set_camera(sc.camera())
+    renderer.clear()
+    renderer.render(0, 3)
+    renderer.synchronize()
+    acc = renderer.read_accum()
+    col = (0.25, 0.5, 0.75)
+    got = renderer.read_ldr(3, tonemap=bool(tm), aces=bool(aces), simple_aces=bool(simple), background=col)
+    want = oracle.tonemap(acc, 3, tonemap=bool(tm), aces=bool(aces), simple_aces=bool(simple), bg_mode=bg,
+                          background=col)
+    d = np.abs(got.astype(np.int64) - want)
+    print(f"tm={tm} aces={aces} simple={simple} bg={bg}: max diff {d.max()}, off-by-one {(d > 0).mean():.2e}, "
+          f"alpha<1 px {(acc[..., 3] < 3).mean():.3f}")
+    assert d.max() <= 1 and (d > 0).mean() < 1e-3
