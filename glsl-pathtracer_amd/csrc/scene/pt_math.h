@@ -18,6 +18,8 @@
 
 namespace glslpt {
 
+constexpr float kPi = 3.14159265358979323846f;  // PI of src/math/MathUtils.h:27
+
 struct Vec3 {
     float x = 0.f, y = 0.f, z = 0.f;
     Vec3() = default;

@@ -9,6 +9,8 @@
 //   env-map CDF                src/core/EnvironmentMap.cpp:39-61
 #include "scene.h"
 
+#include "image_io.h"
+
 #include <cstdio>
 #include <cstring>
 #include <fstream>
@@ -17,7 +19,6 @@
 namespace glslpt {
 
 namespace {
-constexpr float kPi = 3.14159265358979323846f;
 inline float Degrees(float r) { return r * (180.f / kPi); }
 inline float Radians(float d) { return d * (kPi / 180.f); }
 
@@ -224,6 +225,37 @@ int Scene::AddTexture(const uint8_t* rgba, int width, int height, const std::str
     Texture t{name, width, height, std::vector<uint8_t>(rgba, rgba + 4 * static_cast<size_t>(width) * height)};
     textures.push_back(std::move(t));
     return static_cast<int>(textures.size() - 1);
+}
+
+int Scene::AddTexture(const std::string& filename) {
+    for (size_t i = 0; i < textures.size(); ++i)
+        if (textures[i].name == filename) return static_cast<int>(i);
+    int w = 0, h = 0;
+    std::vector<uint8_t> rgba;
+    std::string err;
+    if (!LoadPNG(filename, w, h, rgba, err)) {
+        error = "unable to load texture " + filename + ": " + err;
+        return -1;
+    }
+    textures.push_back(Texture{filename, w, h, std::move(rgba)});
+    return static_cast<int>(textures.size() - 1);
+}
+
+void Scene::AddEnvMap(const std::string& filename) {
+    int w = 0, h = 0;
+    std::vector<float> rgb;
+    std::string err;
+    if (LoadHDR(filename, w, h, rgb, err)) {
+        AddEnvMap(rgb.data(), w, h);
+        return;
+    }
+    error = "unable to load HDR " + filename + ": " + err;
+    envImg.clear();
+    envCdf.clear();
+    envWidth = envHeight = 0;
+    envTotalSum = 0.f;
+    envMapModified = true;
+    dirty = true;
 }
 
 void Scene::AddCamera(Vec3 eye, Vec3 lookat, float fov_degrees) {

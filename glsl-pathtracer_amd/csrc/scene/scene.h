@@ -98,8 +98,14 @@ public:
     int AddMeshInstance(const MeshInstance& instance);
     int AddLight(const Light& light);
     int AddTexture(const uint8_t* rgba, int width, int height, const std::string& name);
+    // Texture from a PNG file, named by its path (src/core/Scene.cpp:85-107);
+    // -1 if it cannot be read.
+    int AddTexture(const std::string& filename);
     void AddCamera(Vec3 eye, Vec3 lookat, float fov_degrees);
     void AddEnvMap(const float* rgb, int width, int height);
+    // Environment map from a Radiance HDR file (src/core/Scene.cpp:116-132);
+    // a file that cannot be read leaves the scene without an environment map.
+    void AddEnvMap(const std::string& filename);
 
     // Builds every BLAS, the TLAS and the flat arrays (src/core/Scene.cpp:216-297).
     // Returns false and sets `error` on invalid input.

@@ -5,6 +5,7 @@
 
 #include "../../../include/glslpt.h"
 #include "scene.h"
+#include "scene_loader.h"
 
 struct pts_scene_s {
     glslpt::Scene scene;
@@ -55,6 +56,83 @@ int pts_destroy(pts_scene s) {
 }
 
 const char* pts_last_error(pts_scene s) { return s ? s->err.c_str() : "null scene handle"; }
+
+namespace {
+void ToC(const glslpt::RenderOptions& r, pts_render_options* o) {
+    o->render_resolution[0] = r.renderResolution[0]; o->render_resolution[1] = r.renderResolution[1];
+    o->window_resolution[0] = r.windowResolution[0]; o->window_resolution[1] = r.windowResolution[1];
+    o->uniform_light_col[0] = r.uniformLightCol.x; o->uniform_light_col[1] = r.uniformLightCol.y;
+    o->uniform_light_col[2] = r.uniformLightCol.z;
+    o->background_col[0] = r.backgroundCol.x; o->background_col[1] = r.backgroundCol.y;
+    o->background_col[2] = r.backgroundCol.z;
+    o->tile_width = r.tileWidth; o->tile_height = r.tileHeight; o->max_depth = r.maxDepth; o->max_spp = r.maxSpp;
+    o->rr_depth = r.RRDepth; o->tex_array_width = r.texArrayWidth; o->tex_array_height = r.texArrayHeight;
+    o->denoiser_frame_cnt = r.denoiserFrameCnt;
+    o->enable_rr = r.enableRR; o->enable_denoiser = r.enableDenoiser; o->enable_tonemap = r.enableTonemap;
+    o->enable_aces = r.enableAces; o->simple_aces_fit = r.simpleAcesFit; o->opengl_normal_map = r.openglNormalMap;
+    o->enable_envmap = r.enableEnvMap; o->enable_uniform_light = r.enableUniformLight;
+    o->hide_emitters = r.hideEmitters; o->enable_background = r.enableBackground;
+    o->transparent_background = r.transparentBackground; o->independent_render_size = r.independentRenderSize;
+    o->enable_roughness_mollification = r.enableRoughnessMollification;
+    o->enable_volume_mis = r.enableVolumeMIS;
+    o->envmap_intensity = r.envMapIntensity; o->envmap_rot = r.envMapRot;
+    o->roughness_mollification_amt = r.roughnessMollificationAmt;
+}
+
+glslpt::RenderOptions FromC(const pts_render_options* o) {
+    glslpt::RenderOptions r;
+    r.renderResolution[0] = o->render_resolution[0]; r.renderResolution[1] = o->render_resolution[1];
+    r.windowResolution[0] = o->window_resolution[0]; r.windowResolution[1] = o->window_resolution[1];
+    r.uniformLightCol = glslpt::Vec3(o->uniform_light_col[0], o->uniform_light_col[1], o->uniform_light_col[2]);
+    r.backgroundCol = glslpt::Vec3(o->background_col[0], o->background_col[1], o->background_col[2]);
+    r.tileWidth = o->tile_width; r.tileHeight = o->tile_height; r.maxDepth = o->max_depth; r.maxSpp = o->max_spp;
+    r.RRDepth = o->rr_depth; r.texArrayWidth = o->tex_array_width; r.texArrayHeight = o->tex_array_height;
+    r.denoiserFrameCnt = o->denoiser_frame_cnt;
+    r.enableRR = o->enable_rr; r.enableDenoiser = o->enable_denoiser; r.enableTonemap = o->enable_tonemap;
+    r.enableAces = o->enable_aces; r.simpleAcesFit = o->simple_aces_fit; r.openglNormalMap = o->opengl_normal_map;
+    r.enableEnvMap = o->enable_envmap; r.enableUniformLight = o->enable_uniform_light;
+    r.hideEmitters = o->hide_emitters; r.enableBackground = o->enable_background;
+    r.transparentBackground = o->transparent_background; r.independentRenderSize = o->independent_render_size;
+    r.enableRoughnessMollification = o->enable_roughness_mollification;
+    r.enableVolumeMIS = o->enable_volume_mis;
+    r.envMapIntensity = o->envmap_intensity; r.envMapRot = o->envmap_rot;
+    r.roughnessMollificationAmt = o->roughness_mollification_amt;
+    return r;
+}
+}  // namespace
+
+int pts_get_texture(pts_scene s, int32_t id, int32_t* width, int32_t* height, uint8_t* rgba) {
+    return Guard(s, [&] {
+        if (!width || !height) return Fail(s, PT_ERR_INVALID_ARG, "null argument");
+        if (id < 0 || id >= static_cast<int32_t>(s->scene.textures.size())) return Fail(s, PT_ERR_INVALID_ARG, "no such texture");
+        const auto& t = s->scene.textures[id];
+        *width = t.width;
+        *height = t.height;
+        if (rgba) std::memcpy(rgba, t.rgba.data(), t.rgba.size());
+        return PT_OK;
+    });
+}
+
+void pts_default_render_options(pts_render_options* o) {
+    if (o) ToC(glslpt::RenderOptions(), o);
+}
+
+int pts_load_scene_file(pts_scene s, const char* filename, pts_render_options* options) {
+    return Guard(s, [&] {
+        if (!filename || !options) return Fail(s, PT_ERR_INVALID_ARG, "null argument");
+        glslpt::RenderOptions r = FromC(options);
+        std::string log;
+        const std::string fn(filename);
+        const bool ok = glslpt::LoadSceneFromFile(fn, s->scene, r, log);
+        ToC(r, options);
+        if (!ok) {
+            const bool gltf = log.find("glTF") != std::string::npos;
+            return Fail(s, gltf ? PT_ERR_UNSUPPORTED : PT_ERR_IO, log);
+        }
+        s->err = log;
+        return PT_OK;
+    });
+}
 
 int pts_add_mesh_obj(pts_scene s, const char* filename, int32_t* id) {
     return Guard(s, [&] {

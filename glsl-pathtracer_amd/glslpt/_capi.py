@@ -92,6 +92,19 @@ class PtJob(C.Structure):
                 ("max_depth", i32), ("accum_out", P(f32))]
 
 
+class PtsRenderOptions(C.Structure):
+    _fields_ = [("render_resolution", i32 * 2), ("window_resolution", i32 * 2),
+                ("uniform_light_col", f32 * 3), ("background_col", f32 * 3),
+                ("tile_width", i32), ("tile_height", i32), ("max_depth", i32), ("max_spp", i32), ("rr_depth", i32),
+                ("tex_array_width", i32), ("tex_array_height", i32), ("denoiser_frame_cnt", i32),
+                ("enable_rr", i32), ("enable_denoiser", i32), ("enable_tonemap", i32), ("enable_aces", i32),
+                ("simple_aces_fit", i32), ("opengl_normal_map", i32), ("enable_envmap", i32),
+                ("enable_uniform_light", i32), ("hide_emitters", i32), ("enable_background", i32),
+                ("transparent_background", i32), ("independent_render_size", i32),
+                ("enable_roughness_mollification", i32), ("enable_volume_mis", i32),
+                ("envmap_intensity", f32), ("envmap_rot", f32), ("roughness_mollification_amt", f32)]
+
+
 _SCENE_SIGS = {
     "pts_create": [P(C.c_void_p)],
     "pts_destroy": [C.c_void_p],
@@ -111,6 +124,8 @@ _SCENE_SIGS = {
     "pts_get_camera": [C.c_void_p, P(PtCamera)],
     "pts_camera_offset_orientation": [C.c_void_p, f32, f32],
     "pts_get_blas_root": [C.c_void_p, i32, P(i32)],
+    "pts_load_scene_file": [C.c_void_p, C.c_char_p, P(PtsRenderOptions)],
+    "pts_get_texture": [C.c_void_p, i32, P(i32), P(i32), P(C.c_uint8)],
 }
 
 _HIP_SIGS = {
@@ -135,7 +150,7 @@ _HIP_SIGS = {
 }
 
 # Every symbol include/glslpt.h declares, per library.
-SCENE_SYMBOLS = tuple(_SCENE_SIGS) + ("pts_last_error",)
+SCENE_SYMBOLS = tuple(_SCENE_SIGS) + ("pts_last_error", "pts_default_render_options")
 HIP_SYMBOLS = tuple(_HIP_SIGS) + ("pt_last_error", "pt_default_options")
 
 _scene_lib = None
@@ -159,6 +174,8 @@ def scene_lib():
             getattr(lib, fn).restype = C.c_int
         lib.pts_last_error.argtypes = [C.c_void_p]
         lib.pts_last_error.restype = C.c_char_p
+        lib.pts_default_render_options.argtypes = [P(PtsRenderOptions)]
+        lib.pts_default_render_options.restype = None
         _scene_lib = lib
     return _scene_lib
 

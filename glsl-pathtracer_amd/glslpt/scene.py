@@ -184,6 +184,30 @@ class Scene:
         h, w = a.shape[:2]
         self._check("pts_add_env_map", self._lib.pts_add_env_map(self._h, api.fptr(a), w, h))
 
+    def LoadSceneFile(self, filename, options=None):
+        """LoadSceneFromFile (src/loaders/Loader.cpp:36-517) into this scene;
+        returns the RenderOptions the file sets (pts_render_options, starting
+        from `options` or the reference defaults). Messages about skipped
+        elements are in self.load_log."""
+        o = api.PtsRenderOptions()
+        if options is None:
+            self._lib.pts_default_render_options(C.byref(o))
+        else:
+            C.pointer(o)[0] = options
+        self._check("pts_load_scene_file", self._lib.pts_load_scene_file(self._h, str(filename).encode(), C.byref(o)))
+        msg = self._lib.pts_last_error(self._h)
+        self.load_log = msg.decode() if msg else ""
+        return o
+
+    def texture(self, tex_id):
+        """Decoded RGBA8 pixels of texture tex_id as added (height, width, 4)."""
+        w, h = C.c_int32(), C.c_int32()
+        self._check("pts_get_texture", self._lib.pts_get_texture(self._h, tex_id, C.byref(w), C.byref(h), None))
+        out = np.zeros((h.value, w.value, 4), np.uint8)
+        self._check("pts_get_texture", self._lib.pts_get_texture(self._h, tex_id, C.byref(w), C.byref(h),
+                                                                 out.ctypes.data_as(C.POINTER(C.c_uint8))))
+        return out
+
     def ProcessScene(self):
         self._check("pts_process", self._lib.pts_process(self._h))
 

@@ -110,6 +110,31 @@ int pts_get_view(pts_scene s, pt_scene_view* out);  /* pointers valid until the 
 int pts_get_camera(pts_scene s, pt_camera* out);
 int pts_camera_offset_orientation(pts_scene s, float dx, float dy);       /* Camera.cpp:152-157 */
 int pts_get_blas_root(pts_scene s, int32_t mesh_id, int32_t* root);
+/* Texture `id` as added (before the resize into the texture array): size, and
+ * its RGBA8 pixels when rgba is not NULL (width*height*4 bytes). */
+int pts_get_texture(pts_scene s, int32_t id, int32_t* width, int32_t* height, uint8_t* rgba);
+
+/* RenderOptions (src/core/Renderer.h:37-100) as a .scene file sets them. */
+typedef struct pts_render_options {
+    int32_t render_resolution[2], window_resolution[2];
+    float uniform_light_col[3], background_col[3];
+    int32_t tile_width, tile_height, max_depth, max_spp, rr_depth;
+    int32_t tex_array_width, tex_array_height, denoiser_frame_cnt;
+    int32_t enable_rr, enable_denoiser, enable_tonemap, enable_aces, simple_aces_fit, opengl_normal_map;
+    int32_t enable_envmap, enable_uniform_light, hide_emitters, enable_background, transparent_background;
+    int32_t independent_render_size, enable_roughness_mollification, enable_volume_mis;
+    float envmap_intensity, envmap_rot, roughness_mollification_amt;
+} pts_render_options;
+/* RenderOptions() defaults (src/core/Renderer.h:39-72). */
+void pts_default_render_options(pts_render_options* o);
+/* LoadSceneFromFile (src/loaders/Loader.cpp:36-517): parses a .scene file into
+ * the scene (materials, lights, camera, OBJ meshes and instances, PNG
+ * textures, HDR environment map) and *options (in/out: start from
+ * pts_default_render_options). The texture-array size of the scene is taken
+ * from the options. Missing meshes, textures or maps are skipped as in the
+ * reference (reported by pts_last_error after PT_OK); glTF blocks return
+ * PT_ERR_UNSUPPORTED. Call pts_process afterwards. */
+int pts_load_scene_file(pts_scene s, const char* filename, pts_render_options* options);
 
 /* ------------------------------------------------------------------ renderer */
 

@@ -9,10 +9,12 @@
 //   Scene::AddMaterial/AddMeshInstance/AddCamera/ProcessScene  src/core/Scene.cpp:56-297
 //   Mesh::LoadFromFile (tinyobj)                               src/core/Mesh.cpp:44-119
 //   Camera basis                                               src/core/Camera.cpp:120-188
+//   LoadSceneFromFile (.scene)                                 src/loaders/Loader.cpp:36-517
 #include <cstring>
 #include <vector>
 
 #include "Camera.h"
+#include "Loader.h"
 #include "Mesh.h"
 #include "Scene.h"
 
@@ -99,6 +101,70 @@ int ref_get_arrays(float* nodes, int* vert_indices, float* verts_uvx, float* nor
         for (int i = 0; i < 4; ++i) { camera15[3 * i] = v[i]->x; camera15[3 * i + 1] = v[i]->y; camera15[3 * i + 2] = v[i]->z; }
         camera15[12] = c->fov; camera15[13] = c->focalDist; camera15[14] = c->aperture;
     }
+    return 0;
+}
+
+// Loads a .scene file with the reference loader into a fresh scene, copies
+// its RenderOptions into the scene as the application does
+// (src/Main.cpp:121-154, without the default-HDR fallback) and processes it.
+// options35: the 35 RenderOptions values in pts_render_options order, as floats.
+// Returns 0, or -1 when the loader fails.
+int ref_load_scene_file(const char* filename, float* options35) {
+    delete g_last.scene;
+    g_last.scene = new Scene();
+    RenderOptions ro;
+    if (!LoadSceneFromFile(filename, g_last.scene, ro)) return -1;
+    g_last.scene->renderOptions = ro;
+    g_last.scene->ProcessScene();
+    const float o[] = {
+        float(ro.renderResolution.x), float(ro.renderResolution.y), float(ro.windowResolution.x),
+        float(ro.windowResolution.y), ro.uniformLightCol.x, ro.uniformLightCol.y, ro.uniformLightCol.z,
+        ro.backgroundCol.x, ro.backgroundCol.y, ro.backgroundCol.z, float(ro.tileWidth), float(ro.tileHeight),
+        float(ro.maxDepth), float(ro.maxSpp), float(ro.RRDepth), float(ro.texArrayWidth), float(ro.texArrayHeight),
+        float(ro.denoiserFrameCnt), float(ro.enableRR), float(ro.enableDenoiser), float(ro.enableTonemap),
+        float(ro.enableAces), 0.f /* simpleAcesFit: not initialised by RenderOptions() */, float(ro.openglNormalMap),
+        float(ro.enableEnvMap), float(ro.enableUniformLight), float(ro.hideEmitters), float(ro.enableBackground),
+        float(ro.transparentBackground), float(ro.independentRenderSize), float(ro.enableRoughnessMollification),
+        float(ro.enableVolumeMIS), ro.envMapIntensity, ro.envMapRot, ro.roughnessMollificationAmt};
+    std::memcpy(options35, o, sizeof(o));
+    return 0;
+}
+
+// Counts of the last scene's materials, lights, instances, textures (and the
+// texture-array layer size) and whether an environment map was loaded.
+int ref_get_counts(int* counts7) {
+    Scene* s = g_last.scene;
+    if (!s) return -1;
+    counts7[0] = static_cast<int>(s->materials.size());
+    counts7[1] = static_cast<int>(s->lights.size());
+    counts7[2] = static_cast<int>(s->transforms.size());
+    counts7[3] = static_cast<int>(s->textures.size());
+    counts7[4] = s->renderOptions.texArrayWidth;
+    counts7[5] = s->renderOptions.texArrayHeight;
+    counts7[6] = s->envMap ? 1 : 0;
+    return 0;
+}
+
+// Copies materials (32 floats each), lights (15) and transforms (16) of the
+// last scene.
+int ref_get_records(float* materials, float* lights, float* transforms) {
+    Scene* s = g_last.scene;
+    if (!s) return -1;
+    std::memcpy(materials, s->materials.data(), s->materials.size() * sizeof(Material));
+    std::memcpy(lights, s->lights.data(), s->lights.size() * sizeof(Light));
+    std::memcpy(transforms, s->transforms.data(), s->transforms.size() * sizeof(Mat4));
+    return 0;
+}
+
+// Decoded pixels of texture i of the last scene (before the resize into the
+// texture array): call with rgba NULL for the size.
+int ref_get_texture(int i, int* w, int* h, unsigned char* rgba) {
+    Scene* s = g_last.scene;
+    if (!s || i < 0 || i >= static_cast<int>(s->textures.size())) return -1;
+    const Texture* t = s->textures[i];
+    *w = t->width;
+    *h = t->height;
+    if (rgba) std::memcpy(rgba, t->texData.data(), t->texData.size());
     return 0;
 }
 

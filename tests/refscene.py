@@ -73,3 +73,41 @@ def build(meshes, instances, num_materials, camera=None):
     camo = np.zeros(15, np.float32)
     lib().ref_get_arrays(fp(nodes), ip(vi), fp(vu), fp(nu), fp(camo))
     return dict(nodes=nodes, top_level_index=tli, vert_indices=vi, vertices_uvx=vu, normals_uvy=nu, camera=camo)
+
+
+def load_scene_file(filename):
+    """Reference .scene loader + ProcessScene: render options (35 values in
+    pts_render_options order), flattened arrays, records and decoded textures."""
+    l = lib()
+    l.ref_load_scene_file.argtypes = [C.c_char_p, P(C.c_float)]
+    l.ref_get_counts.argtypes = [P(C.c_int)]
+    l.ref_get_records.argtypes = [P(C.c_float), P(C.c_float), P(C.c_float)]
+    l.ref_get_texture.argtypes = [C.c_int, P(C.c_int), P(C.c_int), P(C.c_uint8)]
+    opts = np.zeros(35, np.float32)
+    if l.ref_load_scene_file(str(filename).encode(), fp(opts)) != 0:
+        raise IOError(filename)
+    s = [C.c_int() for _ in range(4)]
+    l.ref_get_sizes(*[C.byref(x) for x in s])
+    nn, tli, nt, nv = [x.value for x in s]
+    nodes = np.zeros((nn, 9), np.float32)
+    vi = np.zeros((nt, 3), np.int32)
+    vu = np.zeros((nv, 4), np.float32)
+    nu = np.zeros((nv, 4), np.float32)
+    camo = np.zeros(15, np.float32)
+    l.ref_get_arrays(fp(nodes), ip(vi), fp(vu), fp(nu), fp(camo))
+    cnt = np.zeros(7, np.int32)
+    l.ref_get_counts(ip(cnt))
+    mats = np.zeros((cnt[0], 32), np.float32)
+    lights = np.zeros((cnt[1], 15), np.float32)
+    xf = np.zeros((cnt[2], 16), np.float32)
+    l.ref_get_records(fp(mats), fp(lights), fp(xf))
+    textures = []
+    for i in range(cnt[3]):
+        w, h = C.c_int(), C.c_int()
+        l.ref_get_texture(i, C.byref(w), C.byref(h), None)
+        t = np.zeros((h.value, w.value, 4), np.uint8)
+        l.ref_get_texture(i, C.byref(w), C.byref(h), t.ctypes.data_as(P(C.c_uint8)))
+        textures.append(t)
+    return dict(options=opts, nodes=nodes, top_level_index=tli, vert_indices=vi, vertices_uvx=vu, normals_uvy=nu,
+                camera=camo, materials=mats, lights=lights, transforms=xf, textures=textures,
+                tex_array=(int(cnt[4]), int(cnt[5])), has_env=bool(cnt[6]))

@@ -1,0 +1,165 @@
+"""The .scene loader (pts_load_scene_file) against the reference's own
+LoadSceneFromFile + ProcessScene (src/loaders/Loader.cpp:36-517, compiled
+from its sources into oracle/_ref/libref_scene.so): render options, every
+flattened array, material / light / instance records and the decoded
+textures must be bit-identical.
+
+* Fixture scenes written by tests/golden/make_scene_fixtures.py (every block
+  and key, PNG textures of every colour type, an RLE Radiance HDR map, the
+  blank-line `matrix` quirk, duplicate and unknown material names, a missing
+  mesh) against the reference's outputs stored next to them.
+* The reference's own scene files (assets/*.scene) when the reference tree is
+  present, against the compiled reference loader.
+
+Known deviation: textures whose size differs from the texture-array size are
+resampled with nearest filtering here (the reference uses stb_image_resize),
+so the texture array is compared only when no resampling happens; the
+decoded texture pixels are compared always.
+"""
+import glob
+import os
+
+import numpy as np
+import pytest
+
+import refscene
+from glslpt._capi import PtsRenderOptions
+from glslpt.scene import Scene
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+FIX = os.path.join(HERE, "golden", "scenes")
+ASSETS = "/root/reference/assets"
+
+OPTION_FIELDS = [f for f, _ in PtsRenderOptions._fields_]
+
+
+def options_vector(o):
+    out = []
+    for f in OPTION_FIELDS:
+        v = getattr(o, f)
+        out.extend(list(v) if not isinstance(v, (int, float)) else [v])
+    return np.array(out, np.float32)
+
+
+def bits(a):
+    return np.ascontiguousarray(a).view(np.uint32) if np.asarray(a).dtype == np.float32 else np.asarray(a)
+
+
+def load_ours(path):
+    sc = Scene()
+    o = sc.LoadSceneFile(path)
+    sc.ProcessScene()
+    a = sc.arrays()
+    v = sc.view()
+    cam = sc.camera()
+    cam15 = np.array(list(cam.position) + list(cam.up) + list(cam.right) + list(cam.forward)
+                     + [cam.fov, cam.focalDist, cam.aperture], np.float32)
+    textures = [sc.texture(i) for i in range(v.num_textures)]
+    return sc, o, a, cam15, textures
+
+
+def compare(path, exp):
+    sc, o, a, cam15, textures = load_ours(path)
+    assert np.array_equal(options_vector(o), exp["options"]), (options_vector(o), exp["options"])
+    for k in ("nodes", "vert_indices", "vertices_uvx", "normals_uvy", "materials", "lights", "transforms"):
+        assert a[k].shape == exp[k].shape, k
+        got, want = bits(a[k]).copy(), bits(exp[k]).copy()
+        if k == "lights":
+            # Loader.cpp leaves Light::radius uninitialised unless the block sets it;
+            # it is read only for sphere lights (type 1)
+            unset = a[k][:, 14] != 1
+            got[unset, 12] = want[unset, 12] = 0
+        assert np.array_equal(got, want), k
+    assert a["top_level_index"] == int(exp["top_level_index"])
+    assert np.array_equal(bits(cam15), bits(exp["camera"]))
+    assert len(textures) == int(exp["num_textures"])
+    for i, t in enumerate(textures):
+        assert np.array_equal(t, exp[f"texture{i}"]), f"texture {i}"
+    assert ("env_img" in a) == bool(exp["has_env"])
+    return sc, a
+
+
+def expected_from_reference(path):
+    r = refscene.load_scene_file(path)
+    exp = {k: v for k, v in r.items() if isinstance(v, np.ndarray)}
+    for i, t in enumerate(r["textures"]):
+        exp[f"texture{i}"] = t
+    exp["num_textures"] = len(r["textures"])
+    exp["top_level_index"] = r["top_level_index"]
+    exp["has_env"] = r["has_env"]
+    return exp
+
+
+@pytest.mark.parametrize("name", ["feature", "camera_matrix"])
+def test_fixture_scenes_match_reference_outputs(name):
+    exp = dict(np.load(os.path.join(FIX, f"expected_{name}.npz")))
+    sc, a = compare(os.path.join(FIX, name + ".scene"), exp)
+    if name == "feature":
+        # no resampling (16x16 textures, texarray 16x16): the layers are the decoded textures
+        v = sc.view()
+        assert (v.tex_width, v.tex_height) == (16, 16)
+        assert np.array_equal(a["textures"], np.stack([exp[f"texture{i}"] for i in range(int(exp["num_textures"]))]))
+        assert "env_img" in a and a["env_size"] == (24, 12)
+        assert "missing.obj" in sc.load_log and "nosuchmaterial" in sc.load_log
+
+
+def test_hdr_decoder_closed_form():
+    """RGBE decoding m * 2^(e - 136) of the fixture map, re-derived from the
+    file's bytes by an independent run-length decoder."""
+    sc = Scene()
+    sc.LoadSceneFile(os.path.join(FIX, "feature.scene"))
+    sc.ProcessScene()
+    img = sc.arrays()["env_img"].reshape(12, 24, 3)
+    data = open(os.path.join(FIX, "sky.hdr"), "rb").read()
+    pos = data.index(b"+X 24\n") + len(b"+X 24\n")
+    rgbe = np.zeros((12, 24, 4), np.uint8)
+    for y in range(12):
+        assert data[pos:pos + 2] == b"\x02\x02"
+        pos += 4
+        for c in range(4):
+            x = 0
+            while x < 24:
+                n = data[pos]
+                pos += 1
+                if n > 128:
+                    rgbe[y, x:x + n - 128, c] = data[pos]
+                    pos += 1
+                    x += n - 128
+                else:
+                    rgbe[y, x:x + n, c] = list(data[pos:pos + n])
+                    pos += n
+                    x += n
+    want = rgbe[..., :3].astype(np.float64) * np.exp2(rgbe[..., 3:4].astype(np.float64) - 136)
+    want[rgbe[..., 3] == 0] = 0
+    assert np.array_equal(img.astype(np.float64), want)
+
+
+def test_png_decoder_matches_pillow():
+    from PIL import Image
+    sc = Scene()
+    sc.LoadSceneFile(os.path.join(FIX, "feature.scene"))
+    names = ["rgba8.png", "rgb8.png", "grey8.png", "palette.png", "grey16.png"]
+    for i, n in enumerate(names):
+        im = Image.open(os.path.join(FIX, n))
+        if im.mode == "I;16":
+            ref = (np.asarray(im).astype(np.uint16) >> 8).astype(np.uint8)
+            ref = np.dstack([ref, ref, ref, np.full_like(ref, 255)])
+        else:
+            ref = np.asarray(im.convert("RGBA"))
+        assert np.array_equal(sc.texture(i), ref), n
+
+
+def test_missing_file_and_gltf(tmp_path):
+    sc = Scene()
+    with pytest.raises(Exception):
+        sc.LoadSceneFile(str(tmp_path / "nope.scene"))
+    p = tmp_path / "g.scene"
+    p.write_text("gltf\n{\n\tfile model.gltf\n}\n")
+    with pytest.raises(Exception, match="glTF"):
+        Scene().LoadSceneFile(str(p))
+
+
+@pytest.mark.skipif(not (refscene.available() and os.path.isdir(ASSETS)), reason="reference tree not present")
+@pytest.mark.parametrize("path", sorted(glob.glob(os.path.join(ASSETS, "*.scene"))))
+def test_reference_asset_scenes(path):
+    compare(path, expected_from_reference(path))
