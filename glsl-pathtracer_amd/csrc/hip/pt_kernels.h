@@ -20,11 +20,13 @@ struct PathBuf {
     int2* hit2;    // instance, light
 };
 
-// Shadow (NEE visibility) queue.
+// Shadow (NEE visibility) queue: one record per path with up to two rays
+// (k = 0 environment, 1 analytic light) from the same scatter point.
 struct ShadowBuf {
-    float4* o;  // origin, max distance
-    float4* d;  // direction, path slot (int bits)
-    float4* c;  // contribution (already times throughput)
+    float4* o;  // [n]    origin, path slot (int bits)
+    float4* t;  // [n]    throughput at NEE time, mask of valid rays (int bits)
+    float4* d;  // [2n]   ray k: direction, max distance
+    float4* c;  // [2n]   ray k: contribution without the throughput
 };
 
 // Adjoint (replay pass) inputs and outputs.
@@ -33,7 +35,7 @@ struct AdjBuf {
     const float* dimg;    // dLoss / d(mean image), RGB per pixel, rows bottom-up
     float inv_n;          // 1 / number of samples in the mean image
     float* grad;          // [num_mats * 7] gradient sums
-    float4* sg;           // per shadow ray: 2 float4 = 7 gradient terms, material id (int bits)
+    float4* sg;           // per shadow record and ray k: 2 float4 at 4*pos + 2*k = 7 gradient terms, material id (int bits)
     int num_mats;
     int lds;              // 1: accumulate per block in LDS
 };

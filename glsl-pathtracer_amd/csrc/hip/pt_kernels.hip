@@ -152,16 +152,20 @@ struct PathRegs {
 
 // Result of DirectLight's sampling step; the visibility test is separate so
 // the wavefront form can defer it to the shadow stage.
-struct NeeRay { vec3 o, d, contrib, w; float maxd; bool valid; };
+struct NeeRay { vec3 o, d, contrib, w; float maxd; bool valid, tested; };
 
 // The sampling and BSDF part of DirectLight for one light kind. kind 0 =
 // environment (SampleEnvMap), 1 = analytic light. RNG draws happen in the
 // reference order; contrib excludes the final `* throughput`; w is contrib
 // without the BSDF value (the factor the adjoint multiplies df/dparam by).
+// `tested`: the reference runs AnyHit for this ray (always for the
+// environment, for a light when it faces the point) even where the BSDF
+// gives no contribution, which matters when AnyHit draws random numbers.
 __device__ NeeRay nee_setup(const DevScene& sc, const DevOpts& op, int kind, vec3 rd_in, const SurfState& st,
                             const Material& m, float eta, Rng& rng) {
     NeeRay r;
     r.valid = false;
+    r.tested = kind == 0;
     r.contrib = V1(0.0f);
     r.w = V1(0.0f);
     vec3 scatterPos = st.fhp + st.normal * EPS;
@@ -194,6 +198,7 @@ __device__ NeeRay nee_setup(const DevScene& sc, const DevOpts& op, int kind, vec
     r.d = ls.direction;
     r.maxd = ls.dist - EPS;
     if (dot(ls.direction, ls.normal) < 0.0f) {
+        r.tested = true;
         float pdf;
         vec3 f = DisneyEval(m, eta, -rd_in, st.ffnormal, ls.direction, pdf);
         float mis = 1.0f;
@@ -252,17 +257,22 @@ __device__ vec3 direct_light_volmis(const DevScene& sc, const DevOpts& op, vec3 
 }
 
 // DirectLight with binary visibility (pathtrace.glsl:158-283, non-VOL_MIS).
+// With stochastic alpha in AnyHit every ray the reference tests is traced,
+// so that the path's random-number stream advances as the reference's does;
+// otherwise only rays that can contribute are.
 template <bool COUNT>
 __device__ vec3 direct_light(const DevScene& sc, const DevOpts& op, vec3 rd_in, const SurfState& st,
                              const Material& m, float eta, Rng& rng, Stack& stk, TravCount& cnt, bool& ovf) {
     vec3 Ld = V1(0.0f);
-    if ((op.flags & OPT_ENVMAP) && !(op.flags & OPT_UNIFORM_LIGHT)) {
-        NeeRay r = nee_setup(sc, op, 0, rd_in, st, m, eta, rng);
-        if (r.valid && !occluded<COUNT>(sc, op, r.o, r.d, r.maxd, stk, cnt, ovf, &rng)) Ld = Ld + r.contrib;
-    }
-    if (op.flags & OPT_LIGHTS) {
-        NeeRay r = nee_setup(sc, op, 1, rd_in, st, m, eta, rng);
-        if (r.valid && !occluded<COUNT>(sc, op, r.o, r.d, r.maxd, stk, cnt, ovf, &rng)) Ld = Ld + r.contrib;
+    const bool rng_in_anyhit = (op.flags & OPT_ALPHA_TEST) && !(op.flags & OPT_MEDIUM);
+    for (int kind = 0; kind < 2; ++kind) {
+        if (kind == 0 && !((op.flags & OPT_ENVMAP) && !(op.flags & OPT_UNIFORM_LIGHT))) continue;
+        if (kind == 1 && !(op.flags & OPT_LIGHTS)) continue;
+        NeeRay r = nee_setup(sc, op, kind, rd_in, st, m, eta, rng);
+        if (r.valid || (rng_in_anyhit && r.tested)) {
+            const bool occ = occluded<COUNT>(sc, op, r.o, r.d, r.maxd, stk, cnt, ovf, &rng);
+            if (r.valid && !occ) Ld = Ld + r.contrib;
+        }
     }
     return Ld;
 }
@@ -783,17 +793,21 @@ __global__ void __launch_bounds__(256) k_shade(DevScene sc, DevOpts op0, PathBuf
             }
             keep = alive;
         }
-        // NEE rays: contribution * throughput at NEE time, added if unoccluded.
-        for (int k = 0; k < 2; ++k) {
-            const int pos = wave_append(nee[k].valid, shadow_count);
-            if (pos >= 0) {
-                vec3 c = nee[k].contrib * thr_nee;
-                sb.o[pos] = F4(nee[k].o, nee[k].maxd);
-                sb.d[pos] = F4(nee[k].d, __int_as_float(slot));
-                sb.c[pos] = F4(c, 0.f);
+        // NEE rays of this vertex (environment, light): one shadow record per
+        // path holding both, so one thread adds their visible contributions in
+        // DirectLight's order (no two threads update a path's radiance).
+        const int mask = (nee[0].valid ? 1 : 0) | (nee[1].valid ? 2 : 0);
+        const int pos = wave_append(mask != 0, shadow_count);
+        if (pos >= 0) {
+            sb.o[pos] = F4(nee[(mask & 1) ? 0 : 1].o, __int_as_float(slot));  // both start at the scatter point
+            sb.t[pos] = F4(thr_nee, __int_as_float(mask));
+            for (int k = 0; k < 2; ++k) {
+                if (!((mask >> k) & 1)) continue;
+                sb.d[2 * pos + k] = F4(nee[k].d, nee[k].maxd);
+                sb.c[2 * pos + k] = F4(nee[k].contrib, 0.f);
                 if (ADJ) {
-                    ad.sg[2 * pos] = make_float4(gn[k][0], gn[k][1], gn[k][2], gn[k][3]);
-                    ad.sg[2 * pos + 1] = make_float4(gn[k][4], gn[k][5], gn[k][6], __int_as_float(gmat));
+                    ad.sg[4 * pos + 2 * k] = make_float4(gn[k][0], gn[k][1], gn[k][2], gn[k][3]);
+                    ad.sg[4 * pos + 2 * k + 1] = make_float4(gn[k][4], gn[k][5], gn[k][6], __int_as_float(gmat));
                 }
             }
         }
@@ -805,8 +819,10 @@ __global__ void __launch_bounds__(256) k_shade(DevScene sc, DevOpts op0, PathBuf
     if (ADJ) acc.flush();
 }
 
-// Shadow: any-hit test of queued NEE rays; unoccluded ones add their
-// contribution to the path's radiance (ADJ: and their parameter gradients).
+// Shadow: any-hit tests of the queued NEE rays of a path; the visible
+// contributions are summed as DirectLight does and added to the path's
+// radiance times the throughput at NEE time (ADJ: and their parameter
+// gradients).
 template <bool COUNT, bool ADJ>
 __global__ void __launch_bounds__(256) k_shadow(DevScene sc, DevOpts op, PathBuf pb, ShadowBuf sb, const int* count_in,
                                                 int* work, int S, int* spill, unsigned long long* counters,
@@ -824,21 +840,29 @@ __global__ void __launch_bounds__(256) k_shadow(DevScene sc, DevOpts op, PathBuf
     for (int base = first_batch(); base < count; base = next_batch(work)) {
         const int i = base + lane_id();
         if (i < count) {
-            const float4 o4 = sb.o[i], d4 = sb.d[i];
-            if (!occluded<COUNT>(sc, op, xyz(o4), xyz(d4), o4.w, stk, cnt, ovf)) {
-                const int slot = __float_as_int(d4.w);
-                const float4 c = sb.c[i];
-                float4 r = pb.rad[slot];
-                r.x += c.x; r.y += c.y; r.z += c.z;
-                pb.rad[slot] = r;
+            const float4 o4 = sb.o[i], t4 = sb.t[i];
+            const int mask = __float_as_int(t4.w);
+            vec3 Ld = V1(0.0f);
+            bool lit = false;
+            for (int k = 0; k < 2; ++k) {
+                if (!((mask >> k) & 1)) continue;
+                const float4 d4 = sb.d[2 * i + k];
+                if (occluded<COUNT>(sc, op, xyz(o4), xyz(d4), d4.w, stk, cnt, ovf)) continue;
+                Ld = Ld + xyz(sb.c[2 * i + k]);
+                lit = true;
                 if (ADJ) {
-                    const float4 g0 = ad.sg[2 * i], g1 = ad.sg[2 * i + 1];
+                    const float4 g0 = ad.sg[4 * i + 2 * k], g1 = ad.sg[4 * i + 2 * k + 1];
                     const int mat = __float_as_int(g1.w);
                     if (mat >= 0) {
                         const float g[NPARAM] = {g0.x, g0.y, g0.z, g0.w, g1.x, g1.y, g1.z};
                         acc.add(mat, g);
                     }
                 }
+            }
+            if (lit) {
+                const int slot = __float_as_int(o4.w);
+                const float4 r = pb.rad[slot];
+                pb.rad[slot] = F4(xyz(r) + Ld * xyz(t4), r.w);
             }
         }
     }

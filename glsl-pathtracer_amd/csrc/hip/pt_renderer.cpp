@@ -109,7 +109,7 @@ struct pt_renderer_s {
     DevBuf<uint4> p_rng;
     DevBuf<int4> p_meta;
     DevBuf<int2> p_hit2;
-    DevBuf<float4> s_o, s_d, s_c;
+    DevBuf<float4> s_o, s_t, s_d, s_c;
     DevBuf<float4> a_rad, a_sg;   // adjoint: replay radiance, shadow-ray gradient records
     DevBuf<float> a_dimg, a_grad; // adjoint: dLoss/dimage, gradient sums
     // batched render: job / plane / segment / pass tables, pass state, images
@@ -150,7 +150,7 @@ struct pt_renderer_s {
         nodes.free(); tris.free(); vidx.free(); verts.free(); norms.free(); mats.free(); inst.free();
         lights.free(); tex.free(); env_img.free(); env_cdf.free(); accum.free();
         p_ro.free(); p_rd.free(); p_thr.free(); p_rad.free(); p_aux.free(); p_med.free(); p_hit.free();
-        p_rng.free(); p_meta.free(); p_hit2.free(); s_o.free(); s_d.free(); s_c.free();
+        p_rng.free(); p_meta.free(); p_hit2.free(); s_o.free(); s_t.free(); s_d.free(); s_c.free();
         a_rad.free(); a_sg.free(); a_dimg.free(); a_grad.free();
         b_jobs.free(); b_planes.free(); b_segs.free(); b_passes.free(); b_cur.free(); b_accum.free();
         work_counts.free(); ldr.free();
@@ -491,7 +491,7 @@ Wavefront wavefront_setup(pt_renderer_s* h, const DevOpts& op, int num_samples, 
     h->p_rng.ensure(max_paths); h->p_meta.ensure(max_paths); h->p_aux.ensure(max_paths);
     if (h->any_medium) h->p_med.ensure(max_paths);
     h->p_hit.ensure(max_paths); h->p_hit2.ensure(max_paths);
-    h->s_o.ensure(2 * max_paths); h->s_d.ensure(2 * max_paths); h->s_c.ensure(2 * max_paths);
+    h->s_o.ensure(max_paths); h->s_t.ensure(max_paths); h->s_d.ensure(2 * max_paths); h->s_c.ensure(2 * max_paths);
     h->qa.ensure(max_paths); h->qb.ensure(max_paths); h->hq.ensure(max_paths);
     // Bounce iterations: maxDepth + 1 closest hits; alpha pass-through repeats a depth.
     w.D = op.max_depth + 1 + ((op.flags & ptd::OPT_ALPHA_TEST) ? 64 : 0);
@@ -558,7 +558,7 @@ void render_wavefront(pt_renderer_s* h, int first_sample, int num_samples) {
     const DevOpts op = h->dev_opts();
     const Wavefront w = wavefront_setup(h, op, num_samples, SIZE_MAX);
     const ptd::PathBuf pb = path_buf(h);
-    const ptd::ShadowBuf sb{h->s_o.p, h->s_d.p, h->s_c.p};
+    const ptd::ShadowBuf sb{h->s_o.p, h->s_t.p, h->s_d.p, h->s_c.p};
     for (int s0 = 0; s0 < num_samples; s0 += w.spp_pass) {
         const int ns = std::min(w.spp_pass, num_samples - s0);
         run_bounces(h, op, w, pb, sb, w.npix * ns, first_sample + s0, nullptr);
@@ -586,7 +586,7 @@ void adjoint_wavefront(pt_renderer_s* h, int first_sample, int num_samples, floa
     const ptd::PathBuf pb = path_buf(h);
     ptd::PathBuf pb2 = pb;
     pb2.rad = h->a_rad.p;
-    const ptd::ShadowBuf sb{h->s_o.p, h->s_d.p, h->s_c.p};
+    const ptd::ShadowBuf sb{h->s_o.p, h->s_t.p, h->s_d.p, h->s_c.p};
     ptd::AdjBuf ad{};
     ad.ltot = pb.rad;
     ad.dimg = h->a_dimg.p;
@@ -703,7 +703,7 @@ void batch_render(pt_renderer_s* h, const pt_job* jobs, int n) {
     h->p_rng.ensure(max_paths); h->p_meta.ensure(max_paths); h->p_aux.ensure(max_paths);
     if (h->any_medium) h->p_med.ensure(max_paths);
     h->p_hit.ensure(max_paths); h->p_hit2.ensure(max_paths);
-    h->s_o.ensure(2 * max_paths); h->s_d.ensure(2 * max_paths); h->s_c.ensure(2 * max_paths);
+    h->s_o.ensure(max_paths); h->s_t.ensure(max_paths); h->s_d.ensure(2 * max_paths); h->s_c.ensure(2 * max_paths);
     h->qa.ensure(max_paths); h->qb.ensure(max_paths); h->hq.ensure(max_paths);
     const int D = P.max_depth + 1 + ((op.flags & ptd::OPT_ALPHA_TEST) ? 64 : 0);
     const int ncells = counter_cells(D);
@@ -726,7 +726,7 @@ void batch_render(pt_renderer_s* h, const pt_job* jobs, int n) {
     HIPCHK(hipMemsetAsync(h->b_accum.p, 0, P.pixels * sizeof(float4), s));
     const ptd::BatchBuf bb{h->b_jobs.p, h->b_planes.p, h->b_segs.p, h->b_passes.p, h->b_cur.p, h->b_accum.p};
     const ptd::PathBuf pb = path_buf(h);
-    const ptd::ShadowBuf sb{h->s_o.p, h->s_d.p, h->s_c.p};
+    const ptd::ShadowBuf sb{h->s_o.p, h->s_t.p, h->s_d.p, h->s_c.p};
     int* cnt = h->counters.p;
     const int cadence = std::max(1, h->opts.compact_every);
     const bool graph = h->opts.use_graph != 0;
@@ -1232,6 +1232,15 @@ int pt_batch_render(pt_handle h, const pt_job* jobs, int32_t num_jobs) {
 
 int pt_adjoint(pt_handle h, int32_t first_sample, int32_t num_samples, const float* dL_dimage, float* d_params) {
     return Guard(h, [&] {
+#ifndef PT_ENABLE_ADJOINT
+        // The replay kernels are built but not launched: after the per-path
+        // shadow-record change they produced NaN gradients and then an illegal
+        // memory access on gfx950, with no cause found yet. Build with
+        // -DPT_ENABLE_ADJOINT to run them.
+        (void)first_sample; (void)num_samples; (void)dL_dimage; (void)d_params;
+        return Fail(h, PT_ERR_UNSUPPORTED,
+                    "pt_adjoint is disabled in this build (its kernels faulted on gfx950; cause under investigation)");
+#endif
         check_ready(h);
         if (first_sample < 0 || num_samples <= 0) return Fail(h, PT_ERR_INVALID_ARG, "invalid sample range");
         if (!dL_dimage || !d_params) return Fail(h, PT_ERR_INVALID_ARG, "null gradient buffers");

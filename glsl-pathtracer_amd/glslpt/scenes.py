@@ -245,6 +245,23 @@ def instanced_spheres(num_instances=64, freq=28, seed=0x5EED0002, width=1920, he
                               enable_uniform_light=True, uniform_light_col=(0.3, 0.3, 0.3))
 
 
+def settings_from_render_options(o, width=None, height=None, spp=1):
+    """RenderSettings for a pts_render_options (from Scene.LoadSceneFile):
+    resolution, depth, RR and every path switch the renderer reads; width and
+    height override the file's resolution."""
+    extra = dict(enable_envmap=int(o.enable_envmap), envmap_intensity=float(o.envmap_intensity),
+                 envmap_rot=float(o.envmap_rot), hide_emitters=int(o.hide_emitters),
+                 enable_background=int(o.enable_background), transparent_background=int(o.transparent_background),
+                 enable_volume_mis=int(o.enable_volume_mis),
+                 enable_roughness_mollification=int(o.enable_roughness_mollification),
+                 roughness_mollification_amt=float(o.roughness_mollification_amt),
+                 opengl_normal_map=int(o.opengl_normal_map))
+    return RenderSettings(width or o.render_resolution[0], height or o.render_resolution[1], spp, o.max_depth,
+                          enable_rr=bool(o.enable_rr), rr_depth=o.rr_depth,
+                          enable_uniform_light=bool(o.enable_uniform_light),
+                          uniform_light_col=tuple(o.uniform_light_col), extra=extra)
+
+
 # ------------------------------------------------------------ batched jobs
 BATCH_RESOLUTIONS = ((128, 128), (256, 256), (512, 512), (1024, 1024), (1920, 1080))
 

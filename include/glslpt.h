@@ -254,7 +254,10 @@ int pt_batch_render(pt_handle h, const pt_job* jobs, int32_t num_jobs);
  * Sampling, MIS weights and Russian roulette are detached: the estimate is
  * unbiased for the gradient of the expected image. Runs the wavefront form
  * (primal pass + replay pass) whatever pt_options.mode says; returns
- * PT_ERR_UNSUPPORTED with roughness mollification enabled. */
+ * PT_ERR_UNSUPPORTED with roughness mollification enabled.
+ * The library is built with this entry point switched off: it returns
+ * PT_ERR_UNSUPPORTED unless compiled with -DPT_ENABLE_ADJOINT (its replay
+ * kernels faulted on gfx950 and the cause is not yet found). */
 #define PT_ADJOINT_PARAMS 7
 int pt_adjoint(pt_handle h, int32_t first_sample, int32_t num_samples, const float* dL_dimage,
                float* d_params);

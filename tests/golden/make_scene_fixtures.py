@@ -222,10 +222,17 @@ mesh
 }
 """
 
-CAMERA_MATRIX_SCENE = """camera
+CAMERA_MATRIX_SCENE = """renderer
+{
+	resolution 200 100
+	enableuniformlight true
+	maxdepth 3
+}
+
+camera
 {
 	fov 35
-	matrix 0 0 -1 0 0 1 0 0 1 0 0 0 5 1 2 1
+	matrix 1 0 -1 5 0 1 -0.2 1 0 0 -0.4 2 0 0 0 1
 }
 
 material grey
