@@ -689,9 +689,13 @@ struct Stack {
         if (sp < S) lds[sp * bs + lane] = v;
         else spill[static_cast<size_t>(sp - S) * gstride + gslot] = v;
     }
+    // The backing-store read is a relaxed atomic load so that the compiler
+    // keeps the two reads apart: merged, they become one generic (flat) load,
+    // which puts the LDS pop on the vector-memory path and its wait counter.
     __device__ __forceinline__ int pop(int sp) {
         if (sp < S) return lds[sp * bs + lane];
-        return spill[static_cast<size_t>(sp - S) * gstride + gslot];
+        return __hip_atomic_load(spill + static_cast<size_t>(sp - S) * gstride + gslot, __ATOMIC_RELAXED,
+                                 __HIP_MEMORY_SCOPE_WAVEFRONT);
     }
 };
 
