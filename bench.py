@@ -31,6 +31,13 @@ with --adjoint: the library is built with pt_adjoint switched off.
 batch: the batched form on 32 uneven Cornell-box jobs (pt_batch_render,
 graph-captured passes), reported beside the headline.
 
+heavy: the heavy mixed scene (1000 instances of 8 seeded meshes, ~24.5M
+instanced triangles of which ~32% alpha-tested leaf quads, two scattering
+volume cubes, an importance-sampled 2048x1024 HDR sky) at 3840x2160, 16 spp,
+max depth 12: wavefront and plain throughput and k_extend's algorithmic
+bandwidth (bytes from an instrumented 1-spp frame, scaled to the timed
+frames) against HBM peak; the scene far exceeds L2 and MALL.
+
 cpu_baseline: the CPU oracle (C++ restatement of the reference shaders,
 OpenMP) on the host cores, rendering a bounded sample of the same frame
 (every k-th row at 1 spp, k chosen for ~15 s of CPU work).
@@ -184,6 +191,58 @@ def batch_throughput(reps):
             "samples": total, "ms": best, "msamples_per_s": total / best / 1e3, "launch": cfg}
 
 
+def heavy_throughput(steps, warmup, count=True):
+    """The heavy mixed scene (1000 instances of 8 meshes, ~24.5M instanced
+    triangles, ~32% alpha-tested leaf quads, two scattering volume cubes, an
+    importance-sampled 2048x1024 sky) at 3840x2160, 16 spp, depth 12:
+    wavefront frames (the C2 launch configuration), one plain frame, and
+    k_extend's algorithmic bytes from an instrumented 1-spp frame."""
+    from glslpt import scenes
+    from glslpt._capi import PT_MODE_PLAIN, PT_MODE_WAVEFRONT
+    from glslpt.autotune import tuned
+    from glslpt.renderer import Renderer
+    t0 = time.perf_counter()
+    sc, rs = scenes.heavy_mixed()
+    build_s = time.perf_counter() - t0
+    a = sc.arrays()
+    inst_tris = sum(c * 20 * f * f for c, f in zip(scenes.HEAVY_COUNTS, scenes.HEAVY_SPHERE_FREQS))
+    leaf_tris = sum(c * 2 * q for c, q in zip(scenes.HEAVY_COUNTS[len(scenes.HEAVY_SPHERE_FREQS):],
+                                              scenes.HEAVY_LEAF_QUADS))
+    cfg = dict(block_size=256, stack_depth=24, wave_spp=4, compact_every=1)
+    cfg.update(tuned("c2") or {})
+    r = Renderer(0)
+    r.upload_scene(sc)
+    samples = rs.width * rs.height * rs.spp
+    ms_w, st_w = run_mode(r, sc, rs, PT_MODE_WAVEFRONT, steps, warmup, rs.spp, cfg)
+    ms_p, st_p = run_mode(r, sc, rs, PT_MODE_PLAIN, 1, 0, rs.spp, cfg)
+    out = {"workload": f"{a['transforms'].shape[0]} instances of 8 meshes + ground + 2 volume cubes, "
+                       f"{inst_tris + leaf_tris} instanced triangles ({leaf_tris} alpha-tested), "
+                       f"{a['vert_indices'].shape[0]} unique, 2048x1024 HDR sky, "
+                       f"{rs.width}x{rs.height}, {rs.spp} spp, max depth {rs.max_depth}",
+           "scene_build_s": build_s, "launch": cfg,
+           "wavefront": {"msamples_per_s": samples / ms_w / 1e3, "ms_per_frame": ms_w,
+                         "stage_ms": {k: v / steps for k, v in st_w["ms_stage"].items() if v > 0}},
+           "plain": {"msamples_per_s": samples / ms_p / 1e3, "ms_per_frame": ms_p}}
+    out["wavefront_over_plain"] = out["wavefront"]["msamples_per_s"] / out["plain"]["msamples_per_s"]
+    if count:
+        cw = count_work(r, sc, rs, 1, cfg)
+        scale = rs.spp * steps
+        bytes_total = scale * (B_NODE * cw["nodes_visited"] + B_TRI * cw["tris_tested"] +
+                               B_INST * cw["instances_entered"] + B_LIGHT * cw["lights_tested"] +
+                               B_RAY * cw["rays_closest"])
+        ext_ms = st_w["ms_stage"]["extend"]
+        achieved = bytes_total / (ext_ms / 1e3) / 1e9
+        out["roofline_extend"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                  "frac": achieved / HBM_PEAK_GBS, "bytes_per_frame": bytes_total / steps,
+                                  "extend_ms_per_frame": ext_ms / steps,
+                                  "per_ray": {"nodes": cw["nodes_visited"] / max(1, cw["rays_closest"]),
+                                              "tris": cw["tris_tested"] / max(1, cw["rays_closest"]),
+                                              "instances": cw["instances_entered"] / max(1, cw["rays_closest"])},
+                                  "rays_per_sample": cw["rays_closest"] / (rs.width * rs.height)}
+    r.close()
+    return out
+
+
 def measured_traffic(cfg):
     """HBM bytes per k_extend dispatch from the committed rocprofv3 counter
     runs (profiles/extend_c2_1080p_traffic.*), when they were taken with the
@@ -217,6 +276,8 @@ def main():
     ap.add_argument("--adjoint", action="store_true",
                     help="time pt_adjoint (needs a library built with -DPT_ENABLE_ADJOINT)")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-heavy", action="store_true")
+    ap.add_argument("--heavy-steps", type=int, default=2)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported (the path shards by sample range "
@@ -283,6 +344,7 @@ def main():
         "status": "not measured", "reason": "pt_adjoint is switched off in this build: its replay kernels "
                                             "faulted on gfx950 and the cause is not yet found"}
     batch = None if args.no_batch else batch_throughput(3)
+    heavy = None if args.no_heavy else heavy_throughput(args.heavy_steps, 1, count=not args.no_count)
 
     cpu = None
     if not args.no_cpu:
@@ -302,6 +364,7 @@ def main():
         "wavefront_over_plain": (value / plain["value"]) if plain else None,
         "adjoint": adjoint,
         "batch": batch,
+        "heavy": heavy,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -16,6 +16,14 @@
 * ``material_boxes``     differentiable-materials workload: 256 boxes and 8
                          noise-displaced spheres (~200k triangles), 64
                          materials with seeded optimisable parameters.
+* ``feature_scene``      small scenes exercising one option path each
+                         (environment map, textures, media, alpha).
+* ``heavy_mixed``        the heavy mixed scene: 1000 instances of 8 seeded
+                         meshes (five noise-displaced spheres of 20k-2M
+                         triangles, three clusters of alpha-tested leaf quads),
+                         ~24M instanced triangles of which ~30% alpha-tested,
+                         two scattering volume cubes, a 2048x1024 HDR sky,
+                         3840x2160, 16 spp, depth 12.
 * ``tiny``               a few triangles, for smoke runs.
 
 Each builder returns (Scene, RenderSettings) where RenderSettings carries the
@@ -137,7 +145,55 @@ def _icosahedron():
 
 def geodesic_sphere(freq):
     """Frequency-`freq` geodesic subdivision of the icosahedron: 20*freq^2
-    triangles on the unit sphere, shared vertices. Returns (verts, faces)."""
+    triangles on the unit sphere, shared vertices. Returns (verts, faces).
+
+    Vertices are numbered in order of first appearance over the faces' grids
+    (vectorised; identical to `_geodesic_sphere_loop`)."""
+    v0, f0 = _icosahedron()
+    ii, jj = [], []
+    for i in range(freq + 1):
+        for j in range(freq + 1 - i):
+            ii.append(i)
+            jj.append(j)
+    ii = np.array(ii)
+    jj = np.array(jj)
+    kk = freq - ii - jj
+    ng = ii.shape[0]
+    A, B, C = v0[f0[:, 0]], v0[f0[:, 1]], v0[f0[:, 2]]
+    pts = (A[:, None, :] * kk[None, :, None] + B[:, None, :] * ii[None, :, None]
+           + C[:, None, :] * jj[None, :, None]) / freq          # (20, ng, 3), face-major grid order
+    pts = pts.reshape(-1, 3)
+    keys = np.round(pts, 12)
+    _, first, inv = np.unique(keys, axis=0, return_index=True, return_inverse=True)
+    order = np.argsort(first, kind="stable")                  # unique keys by first appearance
+    rank = np.empty_like(order)
+    rank[order] = np.arange(order.shape[0])
+    gid = rank[inv.reshape(-1)]                                # grid point -> vertex id
+    first_pts = pts[first[order]]
+    # per-vector norm, computed as np.linalg.norm of one vector rounds (a BLAS dot)
+    norms = np.fromiter((np.linalg.norm(p) for p in first_pts), np.float64, first_pts.shape[0])
+    verts = first_pts / norms[:, None]
+    # grid (i, j) -> position in the per-face grid list
+    pos = -np.ones((freq + 2, freq + 2), np.int64)
+    pos[ii, jj] = np.arange(ng)
+    ti, tj = [], []
+    up = []
+    for i in range(freq):
+        for j in range(freq - i):
+            ti.append(i); tj.append(j); up.append(True)
+            if i + j < freq - 1:
+                ti.append(i); tj.append(j); up.append(False)
+    ti = np.array(ti); tj = np.array(tj); up = np.array(up)
+    c0 = np.where(up, pos[ti, tj], pos[ti + 1, tj])
+    c1 = np.where(up, pos[ti + 1, tj], pos[ti + 1, tj + 1])
+    c2 = pos[ti, tj + 1]
+    base = (np.arange(20) * ng)[:, None]
+    faces = np.stack([gid[base + c0[None, :]], gid[base + c1[None, :]], gid[base + c2[None, :]]], -1).reshape(-1, 3)
+    return verts, faces.astype(np.int64)
+
+
+def _geodesic_sphere_loop(freq):
+    """Reference loop form of `geodesic_sphere` (kept for its test)."""
     v0, f0 = _icosahedron()
     verts, index = [], {}
 
@@ -515,6 +571,116 @@ def feature_scene(kind, width=96, height=64, spp=4, max_depth=5, seed=0xFEA70005
     sc.AddCamera((0.0, 2.6, 7.5), (0.0, 0.8, 0.0), 50.0)
     sc.ProcessScene()
     return sc, RenderSettings(width, height, spp, max_depth, enable_rr=True, rr_depth=2, extra=extra)
+
+
+def leaf_quads(num_quads, seed, size=(0.12, 0.3)):
+    """Triangle soup of `num_quads` randomly oriented square quads (UVs 0..1)
+    with centres uniform in the unit ball: a bush of leaf cards."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    c = rng.normal(size=(num_quads, 3))
+    c *= (rng.random(num_quads) ** (1 / 3) / np.linalg.norm(c, axis=1))[:, None]
+    a = rng.normal(size=(num_quads, 3))
+    a /= np.linalg.norm(a, axis=1, keepdims=True)
+    b = np.cross(a, rng.normal(size=(num_quads, 3)))
+    b /= np.linalg.norm(b, axis=1, keepdims=True)
+    h = rng.uniform(size[0], size[1], num_quads)[:, None] * 0.5
+    u, v = a * h, b * h
+    p0, p1, p2, p3 = c - u - v, c + u - v, c + u + v, c - u + v
+    pos = np.stack([np.stack([p0, p1, p2], 1), np.stack([p0, p2, p3], 1)], 1).reshape(-1, 3, 3)
+    n = np.cross(u, v)
+    n /= np.linalg.norm(n, axis=1, keepdims=True)
+    nrm = np.repeat(n[:, None, :], 6, axis=1).reshape(-1, 3, 3)
+    t = np.array([[[0, 0], [1, 0], [1, 1]], [[0, 0], [1, 1], [0, 1]]], np.float32)
+    uvs = np.tile(t, (num_quads, 1, 1))
+    return pos.astype(np.float32), nrm.astype(np.float32), uvs
+
+
+def leaf_texture(n=64, seed=0x1EAF):
+    """RGBA8 leaf card: green with a binary alpha mask (an ellipse with a
+    seeded ragged edge)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    i, j = np.meshgrid((np.arange(n) + 0.5) / n - 0.5, (np.arange(n) + 0.5) / n - 0.5, indexing="ij")
+    ang = np.arctan2(i, j)
+    edge = 0.42 + 0.05 * np.sin(7 * ang + rng.uniform(0, 6.3)) + 0.02 * rng.standard_normal((n, n))
+    inside = (i / 1.0) ** 2 + (j / 0.55) ** 2 < edge ** 2
+    img = np.zeros((n, n, 4), np.uint8)
+    img[..., 0] = 70
+    img[..., 1] = (150 + 60 * rng.random((n, n))).astype(np.uint8)
+    img[..., 2] = 40
+    img[..., 3] = np.where(inside, 255, 0)
+    return img
+
+
+HEAVY_SPHERE_FREQS = (32, 45, 64, 128, 316)    # 20,480 ... 1,997,120 triangles
+HEAVY_LEAF_QUADS = (10000, 20000, 40000)         # 20k, 40k, 80k triangles
+# instance counts per mesh (spheres then leaf clusters), 1000 in total
+HEAVY_COUNTS = (611, 25, 6, 2, 1, 330, 20, 5)
+
+
+def heavy_mixed(width=3840, height=2160, spp=16, max_depth=12, seed=0xC5EA0005, sphere_freqs=HEAVY_SPHERE_FREQS,
+                leaf_quads_per_mesh=HEAVY_LEAF_QUADS, counts=HEAVY_COUNTS, extent=80.0, sky=(2048, 1024),
+                num_materials=16, num_volumes=2):
+    """Heavy mixed scene: instances of 8 seeded meshes (noise-displaced
+    spheres and alpha-tested leaf-quad clusters) spread over a ground plane,
+    two scattering volume cubes (density 0.1-2, albedo 0.8) and a procedural
+    HDR sky that is importance-sampled; no analytic lights. The leaf material
+    is stochastic (BLEND) alpha with opacity 1 times a binary texture alpha;
+    with media present shadow rays ignore alpha (anyhit.glsl:118-141).
+    `sky=None` lights the scene with a constant environment instead."""
+    assert len(counts) == len(sphere_freqs) + len(leaf_quads_per_mesh)
+    rng = np.random.Generator(np.random.PCG64(seed))
+    sc = Scene()
+    sc.SetTextureArraySize(64, 64)
+    t_leaf = sc.AddTexture(leaf_texture(64, seed ^ 0x1EAF), "leaf")
+    meshes, kinds = [], []
+    for k, f in enumerate(sphere_freqs):
+        p, n = displaced_icosphere(f, 0.1, seed + 17 * k)
+        meshes.append(sc.AddMesh(p, n, name=f"sphere{f}"))
+        kinds.append("sphere")
+    for k, nq in enumerate(leaf_quads_per_mesh):
+        p, n, t = leaf_quads(nq, seed + 101 * k)
+        meshes.append(sc.AddMesh(p, n, t, name=f"leaves{nq}"))
+        kinds.append("leaves")
+    g_p, g_n, g_t = _uv_quad((-extent, 0, extent), (2 * extent, 0, 0), (0, 0, -2 * extent), tiles=8.0)
+    ground = sc.AddMesh(g_p, g_n, g_t, name="ground")
+    bpos, bnrm = _unit_box()
+    box = sc.AddMesh(bpos, bnrm, name="box")
+    mids = [sc.AddMaterial(m) for m in disney_materials(rng, num_materials)]
+    m_leaf = sc.AddMaterial(baseColor=(1, 1, 1), roughness=0.7, baseColorTexId=t_leaf, alphaMode=1, opacity=1.0)
+    m_ground = sc.AddMaterial(baseColor=(0.45, 0.42, 0.38), roughness=0.9)
+    sc.AddMeshInstance(ground, mat4(), m_ground)
+    order = []
+    for mesh, kind, cnt in zip(meshes, kinds, counts):
+        order += [(mesh, kind)] * cnt
+    for idx in rng.permutation(len(order)):
+        mesh, kind = order[idx]
+        x, z = rng.uniform(-extent * 0.9, extent * 0.9, 2)
+        if kind == "sphere":
+            s = rng.uniform(0.5, 2.0)
+            xf = mat4(translate=(x, 1.1 * s, z), scale=(s, s, s), rot=random_rotation(rng))
+            sc.AddMeshInstance(mesh, xf, mids[int(rng.integers(num_materials))])
+        else:
+            s = rng.uniform(1.0, 3.0)
+            xf = mat4(translate=(x, s, z), scale=(s, s, s), rot=random_rotation(rng))
+            sc.AddMeshInstance(mesh, xf, m_leaf)
+    for k in range(num_volumes):
+        s = rng.uniform(4.0, 8.0)
+        vm = sc.AddMaterial(baseColor=(1, 1, 1), specTrans=1.0, roughness=0.0, mediumType=2,
+                            mediumDensity=float(rng.uniform(0.1, 2.0)), mediumColor=(0.8, 0.8, 0.8), alphaMode=1,
+                            opacity=0.0, mediumAnisotropy=0.2)
+        x, z = rng.uniform(-extent * 0.4, extent * 0.4, 2)
+        # lifted off the ground: a bottom face coplanar with the ground would
+        # make every ray below the cube an exact tie between two surfaces
+        sc.AddMeshInstance(box, mat4(translate=(x, s + 0.05, z), scale=(s, s, s)), vm)
+    if sky is not None:
+        sc.AddEnvMap(sky_envmap(sky[0], sky[1], seed=seed ^ 0x5C1, sun=(0.3, 0.2), sun_power=2000.0))
+    sc.AddCamera((0.0, extent * 0.16, extent * 1.2), (0.0, 2.0, 0.0), 40.0)
+    sc.ProcessScene()
+    if sky is None:
+        return sc, RenderSettings(width, height, spp, max_depth, enable_rr=True, rr_depth=2,
+                                  enable_uniform_light=True, uniform_light_col=(1.0, 1.0, 1.0))
+    return sc, RenderSettings(width, height, spp, max_depth, enable_rr=True, rr_depth=2,
+                              extra=dict(enable_envmap=1, envmap_intensity=1.0))
 
 
 def tiny(width=32, height=32, spp=2, max_depth=3):
