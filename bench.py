@@ -36,7 +36,8 @@ instanced triangles of which ~32% alpha-tested leaf quads, two scattering
 volume cubes, an importance-sampled 2048x1024 HDR sky) at 3840x2160, 16 spp,
 max depth 12: wavefront and plain throughput and k_extend's algorithmic
 bandwidth (bytes from an instrumented 1-spp frame, scaled to the timed
-frames) against HBM peak; the scene far exceeds L2 and MALL.
+frames) against HBM peak. Its ~260 MB of nodes and triangles exceed L2 and
+MALL, yet L2 still serves ~96% of the closest-hit reads (rocprofv3 counters).
 
 cpu_baseline: the CPU oracle (C++ restatement of the reference shaders,
 OpenMP) on the host cores, rendering a bounded sample of the same frame

@@ -1,11 +1,18 @@
-"""Calibration helper for HBM traffic counters.
+"""Workloads for profiler runs.
 
     python -m glslpt.profiling copy NBYTES
+    python -m glslpt.profiling heavy [SPP]
 
-copies NBYTES device-to-device with hipMemcpy (one blit-kernel dispatch that
-reads and writes NBYTES). Profiled with `rocprofv3 --pmc FETCH_SIZE` (then
-WRITE_SIZE), its per-dispatch counter values give the factor that turns the
-counters of the path's kernels into bytes of HBM traffic.
+copy: copies NBYTES device-to-device with hipMemcpy (one blit-kernel dispatch
+that reads and writes NBYTES). Profiled with `rocprofv3 --pmc FETCH_SIZE`
+(then WRITE_SIZE), its per-dispatch counter values give the factor that turns
+the counters of the path's kernels into bytes of HBM traffic.
+
+heavy: one wavefront frame of SPP samples (default 4) of the heavy mixed
+scene at 3840x2160, depth 12, then an instrumented frame of the same samples
+(k_extend<true, ...> kernels, so a kernel filter on "k_extend<false" selects
+the plain frame). Prints k_extend's time and algorithmic bytes (64 B per node,
+36 B per triangle, 52 B per instance, 60 B per light record, 56 B per ray).
 """
 import ctypes as C
 import sys
@@ -29,8 +36,43 @@ def copy(nbytes):
     print(f"copied {nbytes} bytes device to device")
 
 
+def heavy(spp):
+    import json
+    from . import _capi as api
+    from .autotune import tuned
+    from .renderer import Renderer, options_for
+    from .scenes import heavy_mixed
+    sc, rs = heavy_mixed(spp=spp)
+    cfg = dict(block_size=256, stack_depth=24, wave_spp=4, compact_every=1)
+    cfg.update(tuned("c2") or {})
+    r = Renderer(0)
+    r.upload_scene(sc)
+    out = {}
+    for count in (0, 1):
+        r.set_options(options_for(rs, mode=api.PT_MODE_WAVEFRONT, count_work=count, **cfg))
+        r.set_camera(sc.camera())
+        r.clear()
+        r.reset_stats()
+        r.render(0, spp)
+        r.synchronize()
+        st = r.stats()
+        if count == 0:
+            out["extend_ms"] = st["ms_stage"]["extend"]
+            out["extend_launches"] = st["launches_stage"]["extend"]
+            out["frame_ms"] = st["ms_last_render"]
+        else:
+            out["rays"] = st["rays_closest"]
+            out["algorithmic_bytes"] = (64 * st["nodes_visited"] + 36 * st["tris_tested"] + 52 * st["instances_entered"]
+                                        + 60 * st["lights_tested"] + 56 * st["rays_closest"])
+    r.close()
+    out.update(spp=spp, width=rs.width, height=rs.height, launch=cfg)
+    print(json.dumps(out))
+
+
 if __name__ == "__main__":
     if len(sys.argv) == 3 and sys.argv[1] == "copy":
         copy(int(sys.argv[2]))
+    elif len(sys.argv) in (2, 3) and sys.argv[1] == "heavy":
+        heavy(int(sys.argv[2]) if len(sys.argv) == 3 else 4)
     else:
-        sys.exit("usage: python -m glslpt.profiling copy NBYTES")
+        sys.exit("usage: python -m glslpt.profiling copy NBYTES | heavy [SPP]")
