@@ -928,6 +928,21 @@ __global__ void __launch_bounds__(256) k_accum_batch(PathBuf pb, BatchBuf bb) {
     }
 }
 
+// Environment-map CDF (EnvironmentMap::BuildCDF, src/core/EnvironmentMap.cpp:39-61):
+// one running sum of the luminance over the row-major image, in the same order
+// and float arithmetic as the host build, so one lane computes it; *total gets
+// the last entry (totalSum).
+__global__ void k_env_cdf(const float* __restrict__ img, int n, float* __restrict__ cdf, float* __restrict__ total) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    float run = 0.0f;
+    for (int i = 0; i < n; ++i) {
+        const float w = 0.212671f * img[3 * i] + 0.715160f * img[3 * i + 1] + 0.072169f * img[3 * i + 2];
+        run = (i == 0) ? w : run + w;
+        cdf[i] = run;
+    }
+    *total = n > 0 ? run : 0.0f;
+}
+
 __global__ void k_fill_int(int* p, int n, int v) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
@@ -1135,6 +1150,10 @@ void launch_shadow(hipStream_t s, int blocks, int bs, const DevScene& sc, const 
 
 void launch_accum(hipStream_t s, const DevOpts& op, const PathBuf& pb, float4* accum, int npix, int nsamp) {
     hipLaunchKernelGGL(k_accum, dim3((npix + 255) / 256), dim3(256), 0, s, op, pb, accum, npix, nsamp);
+}
+
+void launch_env_cdf(hipStream_t s, const float* img, int n, float* cdf, float* total) {
+    hipLaunchKernelGGL(k_env_cdf, dim3(1), dim3(64), 0, s, img, n, cdf, total);
 }
 
 void launch_fill(hipStream_t s, int* p, int n, int v) {

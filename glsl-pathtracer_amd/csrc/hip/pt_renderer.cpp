@@ -87,7 +87,7 @@ struct pt_renderer_s {
     DevBuf<GInst> inst;
     DevBuf<GLight> lights;
     DevBuf<uint8_t> tex;
-    DevBuf<float> env_img, env_cdf;
+    DevBuf<float> env_img, env_cdf, env_total;
     DevScene sc{};
     bool has_scene = false;
     int num_tris = 0, num_instances = 0, num_materials = 0;
@@ -148,7 +148,7 @@ struct pt_renderer_s {
         DevBuf<int>* ib[] = {&qa, &qb, &hq, &counters, &spill, &ovf};
         for (auto* b : ib) b->free();
         nodes.free(); tris.free(); vidx.free(); verts.free(); norms.free(); mats.free(); inst.free();
-        lights.free(); tex.free(); env_img.free(); env_cdf.free(); accum.free();
+        lights.free(); tex.free(); env_img.free(); env_cdf.free(); env_total.free(); accum.free();
         p_ro.free(); p_rd.free(); p_thr.free(); p_rad.free(); p_aux.free(); p_med.free(); p_hit.free();
         p_rng.free(); p_meta.free(); p_hit2.free(); s_o.free(); s_t.free(); s_d.free(); s_c.free();
         a_rad.free(); a_sg.free(); a_dimg.free(); a_grad.free();
@@ -909,6 +909,22 @@ int pt_destroy(pt_handle h) {
 // A null handle reports the last pt_create failure of this thread.
 const char* pt_last_error(pt_handle h) { return h ? h->err.c_str() : g_create_err.c_str(); }
 
+// Uploads an RGB32F environment image and builds its CDF on the device;
+// returns the total luminance (the CDF's last entry).
+static float build_env(pt_renderer_s* h, const float* rgb, int width, int height) {
+    const size_t n = static_cast<size_t>(width) * height;
+    if (n > (1u << 28)) throw ArgError(PT_ERR_INVALID_ARG, "environment map too large");
+    h->env_img.upload(rgb, 3 * n, h->stream);
+    h->env_cdf.ensure(n);
+    h->env_total.ensure(1);
+    ptk::launch_env_cdf(h->stream, h->env_img.p, static_cast<int>(n), h->env_cdf.p, h->env_total.p);
+    HIPCHK(hipGetLastError());
+    float total = 0.f;
+    HIPCHK(hipMemcpyAsync(&total, h->env_total.p, sizeof(float), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return total;
+}
+
 int pt_upload_scene(pt_handle h, const pt_scene_view* v) {
     return Guard(h, [&] {
         if (!v || !v->nodes || v->num_nodes <= 0 || !v->vert_indices || v->num_tris <= 0 || !v->vertices_uvx ||
@@ -961,11 +977,16 @@ int pt_upload_scene(pt_handle h, const pt_scene_view* v) {
         if (v->num_textures > 0 && v->texture_maps) {
             h->tex.upload(v->texture_maps, 4 * static_cast<size_t>(v->tex_width) * v->tex_height * v->num_textures, s);
         }
-        h->has_env = v->env_width > 0 && v->env_height > 0 && v->env_img && v->env_cdf;
+        h->has_env = v->env_width > 0 && v->env_height > 0 && v->env_img;
+        float env_total = v->env_total_sum;
         if (h->has_env) {
             const size_t n = static_cast<size_t>(v->env_width) * v->env_height;
-            h->env_img.upload(v->env_img, 3 * n, s);
-            h->env_cdf.upload(v->env_cdf, n, s);
+            if (v->env_cdf) {
+                h->env_img.upload(v->env_img, 3 * n, s);
+                h->env_cdf.upload(v->env_cdf, n, s);
+            } else {
+                env_total = build_env(h, v->env_img, v->env_width, v->env_height);
+            }
         }
         DevScene& sc = h->sc;
         sc.nodes = h->nodes.p; sc.tris = h->tris.p; sc.vidx = h->vidx.p; sc.verts = h->verts.p; sc.norms = h->norms.p;
@@ -979,7 +1000,7 @@ int pt_upload_scene(pt_handle h, const pt_scene_view* v) {
         sc.num_tex = sc.tex ? v->num_textures : 0;
         sc.env_w = h->has_env ? v->env_width : 0;
         sc.env_h = h->has_env ? v->env_height : 0;
-        sc.env_total = v->env_total_sum;
+        sc.env_total = h->has_env ? env_total : 0.f;
         h->flat.assign(v->nodes, v->nodes + v->num_nodes);
         h->top_level_index = v->top_level_index;
         h->num_tris = v->num_tris;
@@ -1015,6 +1036,36 @@ int pt_update_instances(pt_handle h, const float* transforms, int32_t num_instan
         h->sc.mats = h->mats.p;
         h->sc.root = R.root;
         HIPCHK(hipStreamSynchronize(h->stream));
+        return PT_OK;
+    });
+}
+
+int pt_set_env_map(pt_handle h, const float* rgb, int32_t width, int32_t height) {
+    return Guard(h, [&] {
+        if (!h->has_scene) return Fail(h, PT_ERR_STATE, "no scene uploaded (pt_upload_scene)");
+        if (!rgb || width <= 0 || height <= 0) return Fail(h, PT_ERR_INVALID_ARG, "invalid environment map");
+        HIPCHK(hipSetDevice(h->device));
+        HIPCHK(hipStreamSynchronize(h->stream));
+        const float total = build_env(h, rgb, width, height);
+        h->has_env = true;
+        h->sc.env_img = h->env_img.p;
+        h->sc.env_cdf = h->env_cdf.p;
+        h->sc.env_w = width;
+        h->sc.env_h = height;
+        h->sc.env_total = total;
+        return PT_OK;
+    });
+}
+
+int pt_read_env_cdf(pt_handle h, float* cdf, size_t count, float* total) {
+    return Guard(h, [&] {
+        if (!h->has_env) return Fail(h, PT_ERR_STATE, "no environment map");
+        const size_t n = static_cast<size_t>(h->sc.env_w) * h->sc.env_h;
+        if (!cdf || count != n) return Fail(h, PT_ERR_INVALID_ARG, "cdf buffer must hold width*height floats");
+        HIPCHK(hipSetDevice(h->device));
+        HIPCHK(hipMemcpyAsync(cdf, h->env_cdf.p, n * sizeof(float), hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
+        if (total) *total = h->sc.env_total;
         return PT_OK;
     });
 }

@@ -147,6 +147,8 @@ _HIP_SIGS = {
     "pt_any_hit": [C.c_void_p, P(f32), P(f32), P(f32), i32, P(i32)],
     "pt_batch_render": [C.c_void_p, P(PtJob), i32],
     "pt_adjoint": [C.c_void_p, i32, i32, P(f32), P(f32)],
+    "pt_set_env_map": [C.c_void_p, P(f32), i32, i32],
+    "pt_read_env_cdf": [C.c_void_p, P(f32), C.c_size_t, P(f32)],
 }
 
 # Every symbol include/glslpt.h declares, per library.

@@ -48,10 +48,14 @@ class Renderer:
             raise api.PtError(fn, rc, msg.decode() if msg else "")
 
     # --- GLSLPT::Renderer surface ---------------------------------------
-    def upload_scene(self, scene, materials=None):
+    def upload_scene(self, scene, materials=None, device_env_cdf=False):
         """Upload the scene's arrays; `materials` ((num_materials, 32) float32)
-        replaces the scene's material table in the upload."""
+        replaces the scene's material table in the upload; `device_env_cdf`
+        builds the environment map's CDF on the device instead of uploading
+        the host-built one."""
         v = scene.view()
+        if device_env_cdf:
+            v.env_cdf = None
         if materials is not None:
             m = np.ascontiguousarray(materials, np.float32)
             if m.shape != (v.num_materials, 32):
@@ -59,6 +63,21 @@ class Renderer:
             v.materials = C.cast(m.ctypes.data, C.POINTER(api.PtMaterial))
         self._check("pt_upload_scene", self._lib.pt_upload_scene(self._h, C.byref(v)))
         self.num_materials = v.num_materials
+
+    def set_env_map(self, rgb):
+        """Replace the environment map: (height, width, 3) float32, rows top-down;
+        its CDF is built on the device."""
+        img = np.ascontiguousarray(rgb, np.float32)
+        if img.ndim != 3 or img.shape[2] != 3:
+            raise ValueError("rgb must be (height, width, 3)")
+        self._check("pt_set_env_map", self._lib.pt_set_env_map(self._h, api.fptr(img), img.shape[1], img.shape[0]))
+
+    def read_env_cdf(self, count):
+        """(cdf[count], total) of the environment map on the device."""
+        cdf = np.zeros(count, np.float32)
+        total = C.c_float(0.0)
+        self._check("pt_read_env_cdf", self._lib.pt_read_env_cdf(self._h, api.fptr(cdf), count, C.byref(total)))
+        return cdf, total.value
 
     def update_instances(self, scene, first_node=None):
         """Instance-edit refresh after Scene.RebuildInstances (Renderer.cpp:649-665)."""

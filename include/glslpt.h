@@ -77,7 +77,8 @@ typedef struct pt_scene_view {
     const float* transforms; int32_t num_instances;            /* Mat4 data[4][4], column per data[i] */
     const pt_light* lights; int32_t num_lights;
     const uint8_t* texture_maps; int32_t tex_width, tex_height, num_textures; /* RGBA8 layers */
-    const float* env_img; const float* env_cdf;                /* RGB32F image, R32F running-sum CDF */
+    const float* env_img; const float* env_cdf;                /* RGB32F image, R32F running-sum CDF
+                                                                  (NULL: built on the device) */
     int32_t env_width, env_height; float env_total_sum;
 } pt_scene_view;
 
@@ -200,6 +201,15 @@ int pt_update_instances(pt_handle h, const float* transforms, int32_t num_instan
                         const pt_material* materials, int32_t num_materials,
                         const pt_bvh_node* nodes, int32_t first_node, int32_t num_nodes);
 int pt_set_options(pt_handle h, const pt_options* o);          /* InitShaders defines + uniforms */
+/* Replaces the environment map of the uploaded scene: uploads the RGB32F image
+ * (width*height*3 floats, rows top-down as EnvironmentMap::LoadMap reads them)
+ * and builds its luminance CDF and total on the device, bit-identical to
+ * EnvironmentMap::BuildCDF (src/core/EnvironmentMap.cpp:39-61) and to
+ * pts_add_env_map; envMapModified refresh (src/core/Renderer.cpp:666-678). */
+int pt_set_env_map(pt_handle h, const float* rgb, int32_t width, int32_t height);
+/* The device CDF of the current environment map (count = width*height floats)
+ * and its total luminance. */
+int pt_read_env_cdf(pt_handle h, float* cdf, size_t count, float* total);
 int pt_set_camera(pt_handle h, const pt_camera* c);            /* camera uniforms (:764-771) */
 int pt_clear_accum(pt_handle h);                               /* accumFBO clear (:733-744) */
 /* Adds samples [first_sample, first_sample + num_samples) of every pixel to the

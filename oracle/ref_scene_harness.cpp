@@ -168,6 +168,22 @@ int ref_get_texture(int i, int* w, int* h, unsigned char* rgba) {
     return 0;
 }
 
+// Environment map of the last scene as EnvironmentMap holds it: size, RGB
+// image, running-sum CDF (BuildCDF, src/core/EnvironmentMap.cpp:39-61) and
+// totalSum. Call with img NULL for the size.
+int ref_get_env(int* w, int* h, float* img, float* cdf, float* total) {
+    Scene* s = g_last.scene;
+    if (!s || !s->envMap) return -1;
+    const EnvironmentMap* e = s->envMap;
+    *w = e->width;
+    *h = e->height;
+    const size_t n = static_cast<size_t>(e->width) * e->height;
+    if (img) std::memcpy(img, e->img, 3 * n * sizeof(float));
+    if (cdf) std::memcpy(cdf, e->cdf, n * sizeof(float));
+    if (total) *total = e->totalSum;
+    return 0;
+}
+
 void ref_free(void) {
     delete g_last.scene;
     g_last.scene = nullptr;

@@ -108,6 +108,15 @@ def load_scene_file(filename):
         t = np.zeros((h.value, w.value, 4), np.uint8)
         l.ref_get_texture(i, C.byref(w), C.byref(h), t.ctypes.data_as(P(C.c_uint8)))
         textures.append(t)
+    env = None
+    if cnt[6]:
+        l.ref_get_env.argtypes = [P(C.c_int), P(C.c_int), P(C.c_float), P(C.c_float), P(C.c_float)]
+        w, h, tot = C.c_int(), C.c_int(), C.c_float()
+        l.ref_get_env(C.byref(w), C.byref(h), None, None, None)
+        img = np.zeros((h.value, w.value, 3), np.float32)
+        cdf = np.zeros(w.value * h.value, np.float32)
+        l.ref_get_env(C.byref(w), C.byref(h), fp(img), fp(cdf), C.byref(tot))
+        env = dict(img=img, cdf=cdf, total=tot.value)
     return dict(options=opts, nodes=nodes, top_level_index=tli, vert_indices=vi, vertices_uvx=vu, normals_uvy=nu,
                 camera=camo, materials=mats, lights=lights, transforms=xf, textures=textures,
-                tex_array=(int(cnt[4]), int(cnt[5])), has_env=bool(cnt[6]))
+                tex_array=(int(cnt[4]), int(cnt[5])), has_env=bool(cnt[6]), env=env)

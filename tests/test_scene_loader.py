@@ -163,3 +163,26 @@ def test_missing_file_and_gltf(tmp_path):
 @pytest.mark.parametrize("path", sorted(glob.glob(os.path.join(ASSETS, "*.scene"))))
 def test_reference_asset_scenes(path):
     compare(path, expected_from_reference(path))
+
+
+def test_env_cdf_matches_reference_build():
+    """The environment map of the fixture scene (24x12 RLE HDR sky): image,
+    running-sum CDF and total luminance equal the reference's EnvironmentMap
+    (stbi_loadf + BuildCDF, compiled from its sources) bit for bit, and equal
+    a float32 sequential sum of the luminance computed independently here."""
+    path = os.path.join(FIX, "feature.scene")
+    ref = refscene.load_scene_file(path)
+    assert ref["env"] is not None
+    sc = Scene()
+    sc.LoadSceneFile(path)
+    sc.ProcessScene()
+    a = sc.arrays()
+    img = a["env_img"].reshape(ref["env"]["img"].shape)
+    assert np.array_equal(img, ref["env"]["img"])
+    assert np.array_equal(a["env_cdf"], ref["env"]["cdf"])
+    assert np.float32(a["env_total_sum"]) == np.float32(ref["env"]["total"])
+    f = img.reshape(-1, 3)
+    w = np.float32(0.212671) * f[:, 0] + np.float32(0.715160) * f[:, 1]
+    w = w + np.float32(0.072169) * f[:, 2]
+    run = np.cumsum(w, dtype=np.float32)
+    assert np.array_equal(run, a["env_cdf"])
