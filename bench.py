@@ -38,6 +38,10 @@ max depth 12: wavefront and plain throughput and k_extend's algorithmic
 bandwidth (bytes from an instrumented 1-spp frame, scaled to the timed
 frames) against HBM peak. Its ~260 MB of nodes and triangles exceed L2 and
 MALL, yet L2 still serves ~96% of the closest-hit reads (rocprofv3 counters).
+heavy.tlas_rebuild: an edit that moves all 1000 instances, as the device
+rebuild (pt_rebuild_instances) against the host path it replaces (the scene
+builder's RebuildInstances + pt_update_instances with the TLAS range), wall
+time per call.
 
 cpu_baseline: the CPU oracle (C++ restatement of the reference shaders,
 OpenMP) on the host cores, rendering a bounded sample of the same frame
@@ -240,8 +244,52 @@ def heavy_throughput(steps, warmup, count=True):
                                               "tris": cw["tris_tested"] / max(1, cw["rays_closest"]),
                                               "instances": cw["instances_entered"] / max(1, cw["rays_closest"])},
                                   "rays_per_sample": cw["rays_closest"] / (rs.width * rs.height)}
+    out["tlas_rebuild"] = tlas_rebuild_times(r, sc, a)
     r.close()
     return out
+
+
+def tlas_rebuild_times(r, sc, a, reps=5):
+    """Instance edit of the heavy scene (every instance moved): the device
+    rebuild (pt_rebuild_instances: transforms up, TLAS and instance records
+    built on the GPU) against the host path it replaces (the scene builder's
+    RebuildInstances, then pt_update_instances with the TLAS node range, which
+    re-lays the node array out on the host and uploads it). Wall time per call."""
+    import ctypes as C
+
+    import numpy as np
+
+    from glslpt import _capi as api
+    xf0 = np.ascontiguousarray(a["transforms"], np.float32).reshape(-1, 16)
+    n = xf0.shape[0]
+    rng = np.random.default_rng(0)
+    xf1 = xf0.copy()
+    xf1[:, 12:15] += rng.uniform(-0.5, 0.5, (n, 3)).astype(np.float32)
+    dev = []
+    for k in range(reps + 1):
+        t0 = time.perf_counter()
+        r.rebuild_instances(xf1 if k % 2 == 0 else xf0)
+        dev.append(time.perf_counter() - t0)
+    for i in range(n):
+        sc.SetInstanceTransform(i, xf1[i])
+    host_build, host_upload = [], []
+    for k in range(2):
+        t0 = time.perf_counter()
+        sc.RebuildInstances()
+        host_build.append(time.perf_counter() - t0)
+        v = sc.view()
+        tli = v.top_level_index
+        nodes = C.cast(C.byref(v.nodes.contents, tli * C.sizeof(api.PtBvhNode)), C.POINTER(api.PtBvhNode))
+        t0 = time.perf_counter()
+        rc = r._lib.pt_update_instances(r._h, v.transforms, v.num_instances, v.materials, v.num_materials,
+                                        nodes, tli, v.num_nodes - tli)
+        host_upload.append(time.perf_counter() - t0)
+        if rc != api.PT_OK:
+            raise RuntimeError("pt_update_instances failed")
+    d_ms = 1e3 * float(np.median(dev[1:]))
+    hb, hu = 1e3 * min(host_build), 1e3 * min(host_upload)
+    return {"instances": n, "device_ms": d_ms, "host_rebuild_ms": hb, "host_update_ms": hu,
+            "host_total_ms": hb + hu, "host_over_device": (hb + hu) / d_ms}
 
 
 def measured_traffic(cfg):

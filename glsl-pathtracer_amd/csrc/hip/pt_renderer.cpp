@@ -19,6 +19,7 @@
 
 #include "../../../include/glslpt.h"
 #include "pt_kernels.h"
+#include "pt_tlas.h"
 
 using ptd::DevOpts;
 using ptd::DevScene;
@@ -96,6 +97,16 @@ struct pt_renderer_s {
     // host copy of the flat node array (for TLAS refresh) and node re-layout map
     std::vector<pt_bvh_node> flat;
     int top_level_index = 0;
+    // per-instance BLAS root reference and material id (TLAS leaves)
+    std::vector<int> inst_blas, inst_mat;
+    // device TLAS rebuild: inputs, scratch, and the TLAS placement
+    bool tlas_rebuildable = false;
+    bool tlas_on_device = false;  // the device TLAS was rebuilt after the last host node upload
+    int tlas_base = 0;
+    DevBuf<float> t_xf, t_local, t_box, t_cen;
+    DevBuf<int> t_blas, t_mat, t_ord, t_pre, t_pos, t_pseg;
+    DevBuf<ptd::TlasSeg> t_seg;
+    DevBuf<unsigned long long> t_keys;
 
     // render state
     pt_options opts{};
@@ -154,6 +165,8 @@ struct pt_renderer_s {
         a_rad.free(); a_sg.free(); a_dimg.free(); a_grad.free();
         b_jobs.free(); b_planes.free(); b_segs.free(); b_passes.free(); b_cur.free(); b_accum.free();
         work_counts.free(); ldr.free();
+        t_xf.free(); t_local.free(); t_box.free(); t_cen.free(); t_blas.free(); t_mat.free(); t_ord.free();
+        t_pre.free(); t_pos.free(); t_pseg.free(); t_seg.free(); t_keys.free();
         if (stream) (void)hipStreamDestroy(stream);
     }
 
@@ -290,50 +303,9 @@ int Guard(pt_handle h, F&& f) {
     }
 }
 
-// Inverse of the affine part of a column-major Mat4, in double.
-void affine_inverse(const float* m, double inv[4][3]) {
-    double a[3][3];
-    for (int c = 0; c < 3; ++c)
-        for (int r = 0; r < 3; ++r) a[c][r] = m[4 * c + r];  // a[col][row]
-    // determinant and adjugate of the 3x3 (column-major)
-    double det = a[0][0] * (a[1][1] * a[2][2] - a[2][1] * a[1][2]) - a[1][0] * (a[0][1] * a[2][2] - a[2][1] * a[0][2]) +
-                 a[2][0] * (a[0][1] * a[1][2] - a[1][1] * a[0][2]);
-    if (det == 0.0 || !std::isfinite(det)) throw ArgError(PT_ERR_INVALID_ARG, "singular instance transform");
-    double id = 1.0 / det;
-    double b[3][3];  // b[col][row] = inverse
-    b[0][0] = (a[1][1] * a[2][2] - a[2][1] * a[1][2]) * id;
-    b[1][0] = -(a[1][0] * a[2][2] - a[2][0] * a[1][2]) * id;
-    b[2][0] = (a[1][0] * a[2][1] - a[2][0] * a[1][1]) * id;
-    b[0][1] = -(a[0][1] * a[2][2] - a[2][1] * a[0][2]) * id;
-    b[1][1] = (a[0][0] * a[2][2] - a[2][0] * a[0][2]) * id;
-    b[2][1] = -(a[0][0] * a[2][1] - a[2][0] * a[0][1]) * id;
-    b[0][2] = (a[0][1] * a[1][2] - a[1][1] * a[0][2]) * id;
-    b[1][2] = -(a[0][0] * a[1][2] - a[1][0] * a[0][2]) * id;
-    b[2][2] = (a[0][0] * a[1][1] - a[1][0] * a[0][1]) * id;
-    const double t[3] = {m[12], m[13], m[14]};
-    for (int c = 0; c < 3; ++c)
-        for (int r = 0; r < 3; ++r) inv[c][r] = b[c][r];
-    for (int r = 0; r < 3; ++r) inv[3][r] = -(b[0][r] * t[0] + b[1][r] * t[1] + b[2][r] * t[2]);
-}
-
 GInst make_inst(const float* m, int blas_ref, int material) {
-    double inv[4][3];
-    affine_inverse(m, inv);
-    float ic[4][3], fc[3][3];
-    for (int c = 0; c < 4; ++c)
-        for (int r = 0; r < 3; ++r) ic[c][r] = static_cast<float>(inv[c][r]);
-    for (int c = 0; c < 3; ++c)
-        for (int r = 0; r < 3; ++r) fc[c][r] = m[4 * c + r];
     GInst g;
-    g.i0 = make_float4(ic[0][0], ic[0][1], ic[0][2], ic[1][0]);
-    g.i1 = make_float4(ic[1][1], ic[1][2], ic[2][0], ic[2][1]);
-    g.i2 = make_float4(ic[2][2], ic[3][0], ic[3][1], ic[3][2]);
-    g.f0 = make_float4(fc[0][0], fc[0][1], fc[0][2], fc[1][0]);
-    g.f1 = make_float4(fc[1][1], fc[1][2], fc[2][0], fc[2][1]);
-    float br, mt;
-    std::memcpy(&br, &blas_ref, 4);
-    std::memcpy(&mt, &material, 4);
-    g.f2 = make_float4(fc[2][2], br, mt, 0.f);
+    if (!ptd::inst_record(m, blas_ref, material, &g)) throw ArgError(PT_ERR_INVALID_ARG, "singular instance transform");
     return g;
 }
 
@@ -344,12 +316,19 @@ struct Relayout {
     std::vector<GNode> nodes;
     int root = 0;
     std::vector<int> inst_blas_ref, inst_material;
+    std::vector<float> inst_local;  // box of each instance's BLAS root: min xyz, max xyz
+    // The TLAS as the device rebuild writes it: one instance per leaf, every
+    // instance once, internal nodes numbered in preorder from tlas_base to the
+    // end of the node array.
+    bool tlas_rebuildable = false;
+    int tlas_base = 0;
 };
 
 Relayout relayout(const pt_bvh_node* flat, int num_nodes, int top, int num_tris, int num_instances) {
     Relayout R;
     R.inst_blas_ref.assign(num_instances, 0);
     R.inst_material.assign(num_instances, 0);
+    R.inst_local.assign(6 * static_cast<size_t>(num_instances), 0.f);
     auto kind = [&](int i) {
         float z = flat[i].LRLeaf[2];
         return z > 0.f ? 1 : (z < 0.f ? -1 : 0);
@@ -414,11 +393,43 @@ Relayout relayout(const pt_bvh_node* flat, int num_nodes, int top, int num_tris,
             g.d = make_int4(ref_of(l), ref_of(r), 0, 0);
         } else if (k < 0) {
             int inst = -as_int(flat[i].LRLeaf[2]) - 1;
-            R.inst_blas_ref[inst] = ref_of(as_int(flat[i].LRLeaf[0]));
+            const int root = as_int(flat[i].LRLeaf[0]);
+            R.inst_blas_ref[inst] = ref_of(root);
             R.inst_material[inst] = as_int(flat[i].LRLeaf[1]);
+            for (int j = 0; j < 3; ++j) {
+                R.inst_local[6 * inst + j] = flat[root].bboxmin[j];
+                R.inst_local[6 * inst + 3 + j] = flat[root].bboxmax[j];
+            }
         }
     }
     R.root = ref_of(top);
+    {
+        bool ok = true;
+        int internal = 0, leaves = 0;
+        const int base = kind(top) == 0 ? map[top] : n;
+        std::vector<char> inst_seen(num_instances, 0);
+        st.assign(1, top);
+        while (!st.empty() && ok) {
+            const int i = st.back();
+            st.pop_back();
+            const int k = kind(i);
+            if (k == 0) {
+                ok = map[i] == base + internal;
+                ++internal;
+                st.push_back(as_int(flat[i].LRLeaf[1]));
+                st.push_back(as_int(flat[i].LRLeaf[0]));
+            } else if (k < 0) {
+                const int inst = -as_int(flat[i].LRLeaf[2]) - 1;
+                ok = !inst_seen[inst];
+                inst_seen[inst] = 1;
+                ++leaves;
+            } else {
+                ok = false;
+            }
+        }
+        R.tlas_rebuildable = ok && leaves == num_instances && internal == num_instances - 1 && base + internal == n;
+        R.tlas_base = base;
+    }
     return R;
 }
 
@@ -1003,6 +1014,14 @@ int pt_upload_scene(pt_handle h, const pt_scene_view* v) {
         sc.env_total = h->has_env ? env_total : 0.f;
         h->flat.assign(v->nodes, v->nodes + v->num_nodes);
         h->top_level_index = v->top_level_index;
+        h->inst_blas = R.inst_blas_ref;
+        h->inst_mat = R.inst_material;
+        h->tlas_rebuildable = R.tlas_rebuildable;
+        h->tlas_on_device = false;
+        h->tlas_base = R.tlas_base;
+        h->t_local.upload(R.inst_local.data(), R.inst_local.size(), s);
+        h->t_blas.upload(R.inst_blas_ref.data(), R.inst_blas_ref.size(), s);
+        h->t_mat.upload(R.inst_material.data(), R.inst_material.size(), s);
         h->num_tris = v->num_tris;
         h->num_instances = v->num_instances;
         h->num_materials = v->num_materials;
@@ -1018,24 +1037,127 @@ int pt_update_instances(pt_handle h, const float* transforms, int32_t num_instan
         if (!h->has_scene) return Fail(h, PT_ERR_STATE, "no scene uploaded");
         if (num_instances != h->num_instances || !transforms) return Fail(h, PT_ERR_INVALID_ARG, "instance count mismatch");
         if (materials && num_materials != h->num_materials) return Fail(h, PT_ERR_INVALID_ARG, "material count mismatch");
-        if (nodes && (first_node < 0 || first_node + num_nodes > static_cast<int>(h->flat.size())))
+        const int nflat = static_cast<int>(h->flat.size());
+        if (nodes && (first_node < 0 || num_nodes < 0 || first_node + num_nodes > nflat))
             return Fail(h, PT_ERR_INVALID_ARG, "node range out of bounds");
+        if (nodes && h->tlas_on_device && !(first_node <= h->top_level_index && first_node + num_nodes == nflat))
+            return Fail(h, PT_ERR_STATE, "the TLAS was rebuilt on the device: pass the whole TLAS node range");
         HIPCHK(hipSetDevice(h->device));
         HIPCHK(hipStreamSynchronize(h->stream));
-        if (nodes) std::copy(nodes, nodes + num_nodes, h->flat.begin() + first_node);
-        Relayout R = relayout(h->flat.data(), static_cast<int>(h->flat.size()), h->top_level_index, h->num_tris,
-                              h->num_instances);
         std::vector<GInst> insts(num_instances);
-        for (int i = 0; i < num_instances; ++i)
-            insts[i] = make_inst(transforms + 16 * static_cast<size_t>(i), R.inst_blas_ref[i], R.inst_material[i]);
-        h->nodes.upload(R.nodes.data(), R.nodes.size(), h->stream);
+        if (nodes) {
+            // Everything is checked before any state changes.
+            std::vector<pt_bvh_node> flat = h->flat;
+            std::copy(nodes, nodes + num_nodes, flat.begin() + first_node);
+            Relayout R = relayout(flat.data(), nflat, h->top_level_index, h->num_tris, h->num_instances);
+            for (int i = 0; i < num_instances; ++i)
+                if (R.inst_material[i] < 0 || R.inst_material[i] >= h->num_materials)
+                    return Fail(h, PT_ERR_INVALID_ARG, "TLAS leaf material out of range");
+            for (int i = 0; i < num_instances; ++i)
+                insts[i] = make_inst(transforms + 16 * static_cast<size_t>(i), R.inst_blas_ref[i], R.inst_material[i]);
+            h->nodes.upload(R.nodes.data(), R.nodes.size(), h->stream);
+            h->t_local.upload(R.inst_local.data(), R.inst_local.size(), h->stream);
+            h->t_blas.upload(R.inst_blas_ref.data(), R.inst_blas_ref.size(), h->stream);
+            h->t_mat.upload(R.inst_material.data(), R.inst_material.size(), h->stream);
+            h->flat.swap(flat);
+            h->inst_blas = R.inst_blas_ref;
+            h->inst_mat = R.inst_material;
+            h->tlas_rebuildable = R.tlas_rebuildable;
+            h->tlas_base = R.tlas_base;
+            h->tlas_on_device = false;
+            h->sc.nodes = h->nodes.p;
+            h->sc.root = R.root;
+        } else {
+            // TLAS unchanged: only the instance records.
+            for (int i = 0; i < num_instances; ++i)
+                insts[i] = make_inst(transforms + 16 * static_cast<size_t>(i), h->inst_blas[i], h->inst_mat[i]);
+        }
         h->inst.upload(insts.data(), insts.size(), h->stream);
         if (materials) h->mats.upload(reinterpret_cast<const float4*>(materials), 8 * static_cast<size_t>(num_materials), h->stream);
-        h->sc.nodes = h->nodes.p;
         h->sc.inst = h->inst.p;
         h->sc.mats = h->mats.p;
-        h->sc.root = R.root;
         HIPCHK(hipStreamSynchronize(h->stream));
+        return PT_OK;
+    });
+}
+
+int pt_rebuild_instances(pt_handle h, const float* transforms, int32_t num_instances, const int32_t* material_ids) {
+    return Guard(h, [&] {
+        if (!h->has_scene) return Fail(h, PT_ERR_STATE, "no scene uploaded (pt_upload_scene)");
+        if (!transforms || num_instances != h->num_instances) return Fail(h, PT_ERR_INVALID_ARG, "instance count mismatch");
+        if (!h->tlas_rebuildable)
+            return Fail(h, PT_ERR_UNSUPPORTED, "the uploaded TLAS is not a one-instance-per-leaf preorder tree at the end of the node array");
+        const int n = num_instances;
+        for (int i = 0; i < n; ++i) {
+            GInst g;
+            if (!ptd::inst_record(transforms + 16 * static_cast<size_t>(i), 0, 0, &g))
+                return Fail(h, PT_ERR_INVALID_ARG, "singular instance transform");
+        }
+        if (material_ids)
+            for (int i = 0; i < n; ++i)
+                if (material_ids[i] < 0 || material_ids[i] >= h->num_materials)
+                    return Fail(h, PT_ERR_INVALID_ARG, "instance material out of range");
+        HIPCHK(hipSetDevice(h->device));
+        hipStream_t s = h->stream;
+        const size_t nn = static_cast<size_t>(n), half = nn / 2 + 1;
+        h->t_xf.upload(transforms, 16 * nn, s);
+        if (material_ids) {
+            h->t_mat.upload(material_ids, nn, s);
+            h->inst_mat.assign(material_ids, material_ids + n);
+        }
+        h->t_box.ensure(6 * nn);
+        h->t_cen.ensure(3 * nn);
+        h->t_ord.ensure(2 * nn);
+        h->t_pre.ensure(nn + 1);
+        h->t_pos.ensure(2 * nn);
+        h->t_pseg.ensure(nn);
+        h->t_seg.ensure(2 * half);
+        h->t_keys.ensure(24 * half);
+        ptd::TlasBuild a{};
+        a.n = n;
+        a.base = h->tlas_base;
+        a.xf = h->t_xf.p;
+        a.local = h->t_local.p;
+        a.blas_ref = h->t_blas.p;
+        a.mat = h->t_mat.p;
+        a.inst = h->inst.p;
+        a.nodes = h->nodes.p;
+        a.box = h->t_box.p;
+        a.cen = h->t_cen.p;
+        a.ord0 = h->t_ord.p;
+        a.ord1 = h->t_ord.p + nn;
+        a.pre = h->t_pre.p;
+        a.apos = h->t_pos.p;
+        a.bpos = h->t_pos.p + nn;
+        a.pseg = h->t_pseg.p;
+        a.seg0 = h->t_seg.p;
+        a.seg1 = h->t_seg.p + half;
+        a.keys = h->t_keys.p;
+        ptk::launch_tlas_rebuild(s, a);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(s));
+        h->tlas_on_device = true;
+        return PT_OK;
+    });
+}
+
+int pt_read_tlas(pt_handle h, float* nodes, int32_t num_nodes, float* instances, int32_t num_instances, int32_t* root) {
+    return Guard(h, [&] {
+        if (!h->has_scene) return Fail(h, PT_ERR_STATE, "no scene uploaded (pt_upload_scene)");
+        const int n = h->num_instances;
+        if (nodes && !h->tlas_rebuildable)
+            return Fail(h, PT_ERR_UNSUPPORTED, "the uploaded TLAS is not a one-instance-per-leaf preorder tree at the end of the node array");
+        if (nodes && num_nodes != n - 1) return Fail(h, PT_ERR_INVALID_ARG, "nodes must hold instances - 1 records");
+        if (instances && num_instances != n) return Fail(h, PT_ERR_INVALID_ARG, "instances must hold one record per instance");
+        HIPCHK(hipSetDevice(h->device));
+        if (nodes && n > 1)
+            HIPCHK(hipMemcpyAsync(nodes, h->nodes.p + h->tlas_base, static_cast<size_t>(n - 1) * sizeof(GNode),
+                                  hipMemcpyDeviceToHost, h->stream));
+        if (instances)
+            HIPCHK(hipMemcpyAsync(instances, h->inst.p, static_cast<size_t>(n) * sizeof(GInst), hipMemcpyDeviceToHost,
+                                  h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
+        if (root) *root = h->sc.root;
         return PT_OK;
     });
 }

@@ -1,0 +1,91 @@
+// Instance records and the device rebuild of the instance TLAS: the record
+// math shared by the host upload and the device rebuild, the rebuild's
+// buffers, and its launcher.
+#pragma once
+
+#include "pt_device.h"
+
+namespace ptd {
+
+// Instance record from a column-major Mat4 (16 floats): inverse of the
+// affine part in double, rounded to float, plus the forward linear part, the
+// BLAS root reference and the material id. The same code runs on the host
+// (scene upload) and on the device (instance rebuild), so both give the same
+// bits. Returns false for a singular or non-finite linear part.
+__host__ __device__ __forceinline__ bool inst_record(const float* m, int blas_ref, int material, GInst* g) {
+    double a[3][3];  // a[col][row]
+    for (int c = 0; c < 3; ++c)
+        for (int r = 0; r < 3; ++r) a[c][r] = m[4 * c + r];
+    const double det = a[0][0] * (a[1][1] * a[2][2] - a[2][1] * a[1][2]) -
+                       a[1][0] * (a[0][1] * a[2][2] - a[2][1] * a[0][2]) +
+                       a[2][0] * (a[0][1] * a[1][2] - a[1][1] * a[0][2]);
+    if (det == 0.0 || !__builtin_isfinite(det)) return false;
+    const double id = 1.0 / det;
+    double b[3][3];  // b[col][row] = inverse
+    b[0][0] = (a[1][1] * a[2][2] - a[2][1] * a[1][2]) * id;
+    b[1][0] = -(a[1][0] * a[2][2] - a[2][0] * a[1][2]) * id;
+    b[2][0] = (a[1][0] * a[2][1] - a[2][0] * a[1][1]) * id;
+    b[0][1] = -(a[0][1] * a[2][2] - a[2][1] * a[0][2]) * id;
+    b[1][1] = (a[0][0] * a[2][2] - a[2][0] * a[0][2]) * id;
+    b[2][1] = -(a[0][0] * a[2][1] - a[2][0] * a[0][1]) * id;
+    b[0][2] = (a[0][1] * a[1][2] - a[1][1] * a[0][2]) * id;
+    b[1][2] = -(a[0][0] * a[1][2] - a[1][0] * a[0][2]) * id;
+    b[2][2] = (a[0][0] * a[1][1] - a[1][0] * a[0][1]) * id;
+    const double t[3] = {m[12], m[13], m[14]};
+    float ic[4][3];
+    for (int c = 0; c < 3; ++c)
+        for (int r = 0; r < 3; ++r) ic[c][r] = static_cast<float>(b[c][r]);
+    for (int r = 0; r < 3; ++r) ic[3][r] = static_cast<float>(-(b[0][r] * t[0] + b[1][r] * t[1] + b[2][r] * t[2]));
+    g->i0 = make_float4(ic[0][0], ic[0][1], ic[0][2], ic[1][0]);
+    g->i1 = make_float4(ic[1][1], ic[1][2], ic[2][0], ic[2][1]);
+    g->i2 = make_float4(ic[2][2], ic[3][0], ic[3][1], ic[3][2]);
+    g->f0 = make_float4(m[0], m[1], m[2], m[4]);
+    g->f1 = make_float4(m[5], m[6], m[8], m[9]);
+    g->f2 = make_float4(m[10], __builtin_bit_cast(float, blas_ref), __builtin_bit_cast(float, material), 0.f);
+    return true;
+}
+
+// One node of a TLAS level under construction: elements [start, start+count)
+// of the order array, its internal node index, its split plane and the
+// outcome of the split.
+struct TlasSeg {
+    int start, count, node;
+    int mode;        // 0 middle split, no partition; 1 all right; 2 all left; 3 partitioned
+    int axis, near2far;
+    int split;       // partition split (modes 1..3)
+    int csplit;      // first element of the right child
+    int m;           // swapped pairs (mode 3)
+    int child[2];    // next-level segment of each child, -1 for a leaf
+    float border;
+    float cb[6];     // centroid bounds: min xyz, max xyz
+};
+
+// Buffers of one rebuild over n instances (device pointers).
+struct TlasBuild {
+    int n;                  // instances
+    int base;               // node index of the TLAS root (n >= 2)
+    const float* xf;        // [n * 16] instance transforms, column-major
+    const float* local;     // [n * 6] object-space box of each instance's BLAS: min xyz, max xyz
+    const int* blas_ref;    // [n] BLAS root reference
+    const int* mat;         // [n] material id
+    GInst* inst;            // [n] out: instance records
+    GNode* nodes;           // out: nodes [base, base + n - 1)
+    float* box;             // [n * 6] world boxes
+    float* cen;             // [n * 3] box centres
+    int* ord0;              // [n] element order (two buffers, swapped per level)
+    int* ord1;
+    int* pre;               // [n + 1] exclusive prefix count of left-going flags
+    int* apos;              // [n] positions of the swapped pairs, per node
+    int* bpos;
+    int* pseg;              // [n] level segment of each position, -1 when in a leaf
+    TlasSeg* seg0;          // [n / 2 + 1] segments of the current and the next level
+    TlasSeg* seg1;
+    unsigned long long* keys;  // [(n / 2 + 1) * 24] child box reduction keys per segment
+};
+
+}  // namespace ptd
+
+namespace ptk {
+// Instance records, world boxes and the TLAS nodes of a rebuild (a.n >= 1).
+void launch_tlas_rebuild(hipStream_t s, const ptd::TlasBuild& a);
+}  // namespace ptk

@@ -149,6 +149,8 @@ _HIP_SIGS = {
     "pt_adjoint": [C.c_void_p, i32, i32, P(f32), P(f32)],
     "pt_set_env_map": [C.c_void_p, P(f32), i32, i32],
     "pt_read_env_cdf": [C.c_void_p, P(f32), C.c_size_t, P(f32)],
+    "pt_rebuild_instances": [C.c_void_p, P(f32), i32, P(i32)],
+    "pt_read_tlas": [C.c_void_p, P(f32), i32, P(f32), i32, P(i32)],
 }
 
 # Every symbol include/glslpt.h declares, per library.

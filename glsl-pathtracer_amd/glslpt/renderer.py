@@ -90,6 +90,27 @@ class Renderer:
             tli, v.num_nodes - tli))
         return a
 
+    def rebuild_instances(self, transforms, material_ids=None):
+        """Instance edit with the TLAS rebuilt on the device (Scene::RebuildInstances,
+        Scene.cpp:200-214): transforms (num_instances, 4, 4) or (num_instances, 16)
+        in the reference's Mat4 memory order; optional per-instance material ids."""
+        t = np.ascontiguousarray(transforms, np.float32).reshape(-1, 16)
+        m = None if material_ids is None else np.ascontiguousarray(material_ids, np.int32).reshape(-1)
+        if m is not None and m.shape[0] != t.shape[0]:
+            raise ValueError("one material id per instance")
+        self._check("pt_rebuild_instances", self._lib.pt_rebuild_instances(
+            self._h, api.fptr(t), t.shape[0], api.iptr(m)))
+
+    def read_tlas(self, num_instances):
+        """(nodes (n-1, 16) float32, instance records (n, 24) float32, root) of the
+        device TLAS; node columns 12 and 13 hold the child references as int bits."""
+        nodes = np.zeros((max(num_instances - 1, 0), 16), np.float32)
+        insts = np.zeros((num_instances, 24), np.float32)
+        root = C.c_int32(0)
+        self._check("pt_read_tlas", self._lib.pt_read_tlas(
+            self._h, api.fptr(nodes), nodes.shape[0], api.fptr(insts), num_instances, C.byref(root)))
+        return nodes, insts, root.value
+
     def set_options(self, opts=None, **kw):
         o = opts if opts is not None else default_options(**kw)
         self._check("pt_set_options", self._lib.pt_set_options(self._h, C.byref(o)))

@@ -200,6 +200,24 @@ int pt_upload_scene(pt_handle h, const pt_scene_view* v);      /* InitGPUDataBuf
 int pt_update_instances(pt_handle h, const float* transforms, int32_t num_instances,
                         const pt_material* materials, int32_t num_materials,
                         const pt_bvh_node* nodes, int32_t first_node, int32_t num_nodes);
+/* Instance edit with the TLAS rebuilt on the device: Scene::RebuildInstances
+ * (src/core/Scene.cpp:200-214; world boxes :150-184; Bvh(10, 64, false) median
+ * build, thirdparty/RadeonRays/bvh.cpp:68-243; UpdateTLAS,
+ * bvh_translator.cpp:108-114) followed by the instancesModified refresh
+ * (src/core/Renderer.cpp:649-665). Only the transforms (16 floats per instance,
+ * column-major Mat4) and, if not NULL, the instance material ids cross to the
+ * device; the TLAS nodes and instance records are written there, identical to
+ * pts_rebuild_instances followed by pt_update_instances with the TLAS range.
+ * Requires the uploaded TLAS to hold one instance per leaf, as the reference
+ * builds it (else PT_ERR_UNSUPPORTED). Synchronous. */
+int pt_rebuild_instances(pt_handle h, const float* transforms, int32_t num_instances,
+                         const int32_t* material_ids);
+/* Copies the device TLAS: its num_instances - 1 internal nodes (16 floats each:
+ * left box min xyz, max xyz, right box min xyz, max xyz, left and right
+ * reference as int bits, 2 unused), the instance records (24 floats each) and
+ * the root reference. Any output may be NULL. */
+int pt_read_tlas(pt_handle h, float* nodes, int32_t num_nodes, float* instances, int32_t num_instances,
+                 int32_t* root);
 int pt_set_options(pt_handle h, const pt_options* o);          /* InitShaders defines + uniforms */
 /* Replaces the environment map of the uploaded scene: uploads the RGB32F image
  * (width*height*3 floats, rows top-down as EnvironmentMap::LoadMap reads them)

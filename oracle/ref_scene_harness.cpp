@@ -7,6 +7,7 @@
 //
 // Reference entry points exercised:
 //   Scene::AddMaterial/AddMeshInstance/AddCamera/ProcessScene  src/core/Scene.cpp:56-297
+//   Scene::RebuildInstances                                    src/core/Scene.cpp:200-214
 //   Mesh::LoadFromFile (tinyobj)                               src/core/Mesh.cpp:44-119
 //   Camera basis                                               src/core/Camera.cpp:120-188
 //   LoadSceneFromFile (.scene)                                 src/loaders/Loader.cpp:36-517
@@ -101,6 +102,18 @@ int ref_get_arrays(float* nodes, int* vert_indices, float* verts_uvx, float* nor
         for (int i = 0; i < 4; ++i) { camera15[3 * i] = v[i]->x; camera15[3 * i + 1] = v[i]->y; camera15[3 * i + 2] = v[i]->z; }
         camera15[12] = c->fov; camera15[13] = c->focalDist; camera15[14] = c->aperture;
     }
+    return 0;
+}
+
+// Replaces every instance transform of the last scene (16 floats each, Mat4
+// data order) and runs the reference's Scene::RebuildInstances
+// (src/core/Scene.cpp:200-214); ref_get_arrays then returns the rebuilt nodes.
+int ref_rebuild_instances(const float* transforms) {
+    Scene* s = g_last.scene;
+    if (!s) return -1;
+    for (size_t i = 0; i < s->meshInstances.size(); ++i)
+        std::memcpy(s->meshInstances[i].transform.data, transforms + 16 * i, sizeof(s->meshInstances[i].transform.data));
+    s->RebuildInstances();
     return 0;
 }
 

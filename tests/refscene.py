@@ -28,6 +28,7 @@ def lib():
                                       P(C.c_int), P(C.c_int), P(C.c_float), P(C.c_float)]
         l.ref_get_sizes.argtypes = [P(C.c_int)] * 4
         l.ref_get_arrays.argtypes = [P(C.c_float), P(C.c_int), P(C.c_float), P(C.c_float), P(C.c_float)]
+        l.ref_rebuild_instances.argtypes = [P(C.c_float)]
         _lib = l
     return _lib
 
@@ -73,6 +74,23 @@ def build(meshes, instances, num_materials, camera=None):
     camo = np.zeros(15, np.float32)
     lib().ref_get_arrays(fp(nodes), ip(vi), fp(vu), fp(nu), fp(camo))
     return dict(nodes=nodes, top_level_index=tli, vert_indices=vi, vertices_uvx=vu, normals_uvy=nu, camera=camo)
+
+
+def rebuild_instances(transforms):
+    """Reference Scene::RebuildInstances of the last build with new transforms
+    (n, 4, 4) in Mat4 data order; returns the rebuilt flattened nodes."""
+    xf = np.ascontiguousarray(transforms, np.float32).reshape(-1, 16)
+    if lib().ref_rebuild_instances(fp(xf)) != 0:
+        raise RuntimeError("no reference scene built")
+    s = [C.c_int() for _ in range(4)]
+    lib().ref_get_sizes(*[C.byref(x) for x in s])
+    nn, tli, nt, nv = [x.value for x in s]
+    nodes = np.zeros((nn, 9), np.float32)
+    vi = np.zeros((nt, 3), np.int32)
+    vu = np.zeros((nv, 4), np.float32)
+    nu = np.zeros((nv, 4), np.float32)
+    lib().ref_get_arrays(fp(nodes), ip(vi), fp(vu), fp(nu), None)
+    return nodes
 
 
 def load_scene_file(filename):
