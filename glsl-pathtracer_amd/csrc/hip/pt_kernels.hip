@@ -929,18 +929,98 @@ __global__ void __launch_bounds__(256) k_accum_batch(PathBuf pb, BatchBuf bb) {
 }
 
 // Environment-map CDF (EnvironmentMap::BuildCDF, src/core/EnvironmentMap.cpp:39-61):
-// one running sum of the luminance over the row-major image, in the same order
-// and float arithmetic as the host build, so one lane computes it; *total gets
-// the last entry (totalSum).
-__global__ void k_env_cdf(const float* __restrict__ img, int n, float* __restrict__ cdf, float* __restrict__ total) {
-    if (blockIdx.x != 0 || threadIdx.x != 0) return;
-    float run = 0.0f;
-    for (int i = 0; i < n; ++i) {
-        const float w = 0.212671f * img[3 * i] + 0.715160f * img[3 * i + 1] + 0.072169f * img[3 * i + 2];
-        run = (i == 0) ? w : run + w;
-        cdf[i] = run;
+// the luminance of every texel, then one running sum over the row-major image
+// in the same order and float arithmetic as the host build.
+//
+// The luminance is elementwise and runs over the whole grid, written into the
+// CDF buffer. The running sum is a chain of dependent fp32 adds, which a
+// different association would round differently, so one lane performs it: one
+// workgroup, lane 0 of wave 0 adding chunks of ENV_CHUNK texels held in LDS
+// (16 per step, the next 16 already loaded), while waves 1..3 store the chunk
+// before and load the chunk after. Adding texel 0 to -0.0f returns it exactly.
+constexpr int ENV_CHUNK = 4096;
+constexpr int ENV_SCAN_THREADS = 256;
+
+__global__ void k_env_lum(const float* __restrict__ img, int n, float* __restrict__ w) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) w[i] = 0.212671f * img[3 * static_cast<size_t>(i)] + 0.715160f * img[3 * static_cast<size_t>(i) + 1] +
+                      0.072169f * img[3 * static_cast<size_t>(i) + 2];
+}
+
+__global__ void __launch_bounds__(ENV_SCAN_THREADS) k_env_scan(float* cdf, int n, float* __restrict__ total) {
+    extern __shared__ int lds[];
+    float* buf = reinterpret_cast<float*>(lds);  // two chunks
+    const int tid = threadIdx.x;
+    const int nch = (n + ENV_CHUNK - 1) / ENV_CHUNK;
+    for (int j = tid; j < min(ENV_CHUNK, n); j += ENV_SCAN_THREADS) buf[j] = cdf[j];
+    __syncthreads();
+    float run = -0.0f;
+    for (int c = 0; c < nch; ++c) {
+        float* cur = buf + (c & 1) * ENV_CHUNK;
+        float* oth = buf + ((c + 1) & 1) * ENV_CHUNK;
+        const int cnt = min(ENV_CHUNK, n - c * ENV_CHUNK);
+        if (tid == 0) {
+            int j = 0;
+            if (cnt >= 16) {
+                float4 q0 = reinterpret_cast<const float4*>(cur)[0], q1 = reinterpret_cast<const float4*>(cur)[1];
+                float4 q2 = reinterpret_cast<const float4*>(cur)[2], q3 = reinterpret_cast<const float4*>(cur)[3];
+                for (; j + 16 <= cnt; j += 16) {
+                    float4 v[4] = {q0, q1, q2, q3};
+                    if (j + 32 <= cnt) {
+                        const float4* nx = reinterpret_cast<const float4*>(cur + j + 16);
+                        q0 = nx[0]; q1 = nx[1]; q2 = nx[2]; q3 = nx[3];
+                    }
+                    for (int k = 0; k < 4; ++k) {
+                        run = run + v[k].x; v[k].x = run;
+                        run = run + v[k].y; v[k].y = run;
+                        run = run + v[k].z; v[k].z = run;
+                        run = run + v[k].w; v[k].w = run;
+                    }
+                    float4* o = reinterpret_cast<float4*>(cur + j);
+                    o[0] = v[0]; o[1] = v[1]; o[2] = v[2]; o[3] = v[3];
+                }
+            }
+            for (; j < cnt; ++j) {
+                run = run + cur[j];
+                cur[j] = run;
+            }
+        } else if (tid >= 64) {
+            // Whole float4s, every load of a thread issued before its LDS writes.
+            constexpr int LT = ENV_SCAN_THREADS - 64, Q = ENV_CHUNK / 4, PER = (Q + LT - 1) / LT;
+            const int t = tid - 64;
+            if (c > 0) {
+                float4* dst = reinterpret_cast<float4*>(cdf + static_cast<size_t>(c - 1) * ENV_CHUNK);
+                const float4* src = reinterpret_cast<const float4*>(oth);
+#pragma unroll
+                for (int k = 0; k < PER; ++k)
+                    if (t + k * LT < Q) dst[t + k * LT] = src[t + k * LT];
+            }
+            if (c + 1 < nch) {
+                const size_t nb = static_cast<size_t>(c + 1) * ENV_CHUNK;
+                const int ncnt = min(ENV_CHUNK, n - (c + 1) * ENV_CHUNK), nq = ncnt / 4;
+                const float4* src = reinterpret_cast<const float4*>(cdf + nb);
+                float4 r[PER];
+#pragma unroll
+                for (int k = 0; k < PER; ++k)
+                    if (t + k * LT < nq) r[k] = src[t + k * LT];
+                float4* dst = reinterpret_cast<float4*>(oth);
+#pragma unroll
+                for (int k = 0; k < PER; ++k)
+                    if (t + k * LT < nq) dst[t + k * LT] = r[k];
+                // The last partial float4 goes to the thread that stored that
+                // float4 of the previous chunk above, after its store.
+                if (t == nq % LT)
+                    for (int j = 4 * nq; j < ncnt; ++j) oth[j] = cdf[nb + j];
+            }
+        }
+        __syncthreads();
     }
-    *total = n > 0 ? run : 0.0f;
+    if (nch > 0) {
+        const int last = nch - 1;
+        const float* cur = buf + (last & 1) * ENV_CHUNK;
+        for (int j = tid; j < n - last * ENV_CHUNK; j += ENV_SCAN_THREADS) cdf[last * ENV_CHUNK + j] = cur[j];
+    }
+    if (tid == 0) *total = n > 0 ? run : 0.0f;
 }
 
 __global__ void k_fill_int(int* p, int n, int v) {
@@ -1153,7 +1233,8 @@ void launch_accum(hipStream_t s, const DevOpts& op, const PathBuf& pb, float4* a
 }
 
 void launch_env_cdf(hipStream_t s, const float* img, int n, float* cdf, float* total) {
-    hipLaunchKernelGGL(k_env_cdf, dim3(1), dim3(64), 0, s, img, n, cdf, total);
+    if (n > 0) hipLaunchKernelGGL(k_env_lum, dim3((n + 255) / 256), dim3(256), 0, s, img, n, cdf);
+    hipLaunchKernelGGL(k_env_scan, dim3(1), dim3(ENV_SCAN_THREADS), 2 * ENV_CHUNK * sizeof(float), s, cdf, n, total);
 }
 
 void launch_fill(hipStream_t s, int* p, int n, int v) {

@@ -15,6 +15,7 @@
 #include <cstring>
 #include <exception>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "../../../include/glslpt.h"
@@ -99,6 +100,7 @@ struct pt_renderer_s {
     int top_level_index = 0;
     // per-instance BLAS root reference and material id (TLAS leaves)
     std::vector<int> inst_blas, inst_mat;
+    std::unordered_map<int, int> root_ref;  // BLAS roots of the last full re-layout
     // device TLAS rebuild: inputs, scratch, and the TLAS placement
     bool tlas_rebuildable = false;
     bool tlas_on_device = false;  // the device TLAS was rebuilt after the last host node upload
@@ -317,6 +319,7 @@ struct Relayout {
     int root = 0;
     std::vector<int> inst_blas_ref, inst_material;
     std::vector<float> inst_local;  // box of each instance's BLAS root: min xyz, max xyz
+    std::unordered_map<int, int> root_ref;  // flat index of each BLAS root a TLAS leaf names -> its reference
     // The TLAS as the device rebuild writes it: one instance per leaf, every
     // instance once, internal nodes numbered in preorder from tlas_base to the
     // end of the node array.
@@ -395,6 +398,7 @@ Relayout relayout(const pt_bvh_node* flat, int num_nodes, int top, int num_tris,
             int inst = -as_int(flat[i].LRLeaf[2]) - 1;
             const int root = as_int(flat[i].LRLeaf[0]);
             R.inst_blas_ref[inst] = ref_of(root);
+            R.root_ref[root] = R.inst_blas_ref[inst];
             R.inst_material[inst] = as_int(flat[i].LRLeaf[1]);
             for (int j = 0; j < 3; ++j) {
                 R.inst_local[6 * inst + j] = flat[root].bboxmin[j];
@@ -431,6 +435,74 @@ Relayout relayout(const pt_bvh_node* flat, int num_nodes, int top, int num_tris,
         R.tlas_base = base;
     }
     return R;
+}
+
+// TLAS-only re-layout for an update of the TLAS node range [top, nflat) with
+// the BLASes unchanged (the reference uploads exactly that range,
+// src/core/Renderer.cpp:659-663): the n - 1 internal nodes, numbered in
+// preorder from base, plus each instance's BLAS root reference and material.
+// Returns false when the new range is not a one-instance-per-leaf preorder
+// tree or names a BLAS root that the last full re-layout did not convert;
+// the caller then re-lays the whole array out.
+struct TlasRelayout {
+    std::vector<GNode> nodes;
+    std::vector<int> blas, mat, roots;
+};
+bool relayout_tlas(const pt_bvh_node* tl, int top, int nflat, int n, int base,
+                   const std::unordered_map<int, int>& root_ref, TlasRelayout& T) {
+    auto at = [&](int i) -> const pt_bvh_node& { return tl[i - top]; };
+    auto kind = [&](int i) { const float z = at(i).LRLeaf[2]; return z > 0.f ? 1 : (z < 0.f ? -1 : 0); };
+    auto as_int = [](float f) { return static_cast<int>(f); };
+    T.nodes.assign(std::max(n - 1, 0), GNode{});
+    T.blas.assign(n, 0);
+    T.mat.assign(n, 0);
+    T.roots.assign(n, -1);
+    std::vector<char> seen(n, 0);
+    struct Item { int i, parent, side; };
+    std::vector<Item> st{{top, -1, 0}};
+    int internal = 0, leaves = 0;
+    while (!st.empty()) {
+        const Item it = st.back();
+        st.pop_back();
+        if (it.i < top || it.i >= nflat) return false;
+        int ref;
+        const int k = kind(it.i);
+        if (k == 0) {
+            if (internal >= n - 1) return false;
+            const int idx = internal++;
+            ref = base + idx;
+            const int l = as_int(at(it.i).LRLeaf[0]), r = as_int(at(it.i).LRLeaf[1]);
+            if (l < top || l >= nflat || r < top || r >= nflat) return false;
+            const float* lmn = at(l).bboxmin; const float* lmx = at(l).bboxmax;
+            const float* rmn = at(r).bboxmin; const float* rmx = at(r).bboxmax;
+            GNode& g = T.nodes[idx];
+            g.a = make_float4(lmn[0], lmn[1], lmn[2], lmx[0]);
+            g.b = make_float4(lmx[1], lmx[2], rmn[0], rmn[1]);
+            g.c = make_float4(rmn[2], rmx[0], rmx[1], rmx[2]);
+            g.d = make_int4(0, 0, 0, 0);
+            st.push_back({r, idx, 1});
+            st.push_back({l, idx, 0});
+        } else if (k < 0) {
+            const int inst = -as_int(at(it.i).LRLeaf[2]) - 1;
+            if (inst < 0 || inst >= n || seen[inst]) return false;
+            seen[inst] = 1;
+            ++leaves;
+            const int root = as_int(at(it.i).LRLeaf[0]);
+            const auto f = root_ref.find(root);
+            if (f == root_ref.end()) return false;
+            T.blas[inst] = f->second;
+            T.mat[inst] = as_int(at(it.i).LRLeaf[1]);
+            T.roots[inst] = root;
+            ref = -1 - inst;
+        } else {
+            return false;
+        }
+        if (it.parent >= 0) {
+            if (it.side == 0) T.nodes[it.parent].d.x = ref;
+            else T.nodes[it.parent].d.y = ref;
+        }
+    }
+    return internal == n - 1 && leaves == n;
 }
 
 GLight make_light(const pt_light& l) {
@@ -1016,6 +1088,7 @@ int pt_upload_scene(pt_handle h, const pt_scene_view* v) {
         h->top_level_index = v->top_level_index;
         h->inst_blas = R.inst_blas_ref;
         h->inst_mat = R.inst_material;
+        h->root_ref = std::move(R.root_ref);
         h->tlas_rebuildable = R.tlas_rebuildable;
         h->tlas_on_device = false;
         h->tlas_base = R.tlas_base;
@@ -1045,7 +1118,34 @@ int pt_update_instances(pt_handle h, const float* transforms, int32_t num_instan
         HIPCHK(hipSetDevice(h->device));
         HIPCHK(hipStreamSynchronize(h->stream));
         std::vector<GInst> insts(num_instances);
-        if (nodes) {
+        TlasRelayout T;
+        const bool tlas_only = nodes && h->tlas_rebuildable && first_node == h->top_level_index &&
+                               first_node + num_nodes == nflat &&
+                               relayout_tlas(nodes, first_node, nflat, num_instances, h->tlas_base, h->root_ref, T);
+        if (tlas_only) {
+            // The TLAS range only: n - 1 nodes and the instance records cross the bus.
+            for (int i = 0; i < num_instances; ++i)
+                if (T.mat[i] < 0 || T.mat[i] >= h->num_materials)
+                    return Fail(h, PT_ERR_INVALID_ARG, "TLAS leaf material out of range");
+            for (int i = 0; i < num_instances; ++i)
+                insts[i] = make_inst(transforms + 16 * static_cast<size_t>(i), T.blas[i], T.mat[i]);
+            std::vector<float> local(6 * static_cast<size_t>(num_instances));
+            for (int i = 0; i < num_instances; ++i)
+                for (int j = 0; j < 3; ++j) {
+                    local[6 * i + j] = h->flat[T.roots[i]].bboxmin[j];
+                    local[6 * i + 3 + j] = h->flat[T.roots[i]].bboxmax[j];
+                }
+            if (num_instances > 1)
+                HIPCHK(hipMemcpyAsync(h->nodes.p + h->tlas_base, T.nodes.data(), T.nodes.size() * sizeof(GNode),
+                                      hipMemcpyHostToDevice, h->stream));
+            h->t_local.upload(local.data(), local.size(), h->stream);
+            h->t_blas.upload(T.blas.data(), T.blas.size(), h->stream);
+            h->t_mat.upload(T.mat.data(), T.mat.size(), h->stream);
+            std::copy(nodes, nodes + num_nodes, h->flat.begin() + first_node);
+            h->inst_blas = T.blas;
+            h->inst_mat = T.mat;
+            h->tlas_on_device = false;
+        } else if (nodes) {
             // Everything is checked before any state changes.
             std::vector<pt_bvh_node> flat = h->flat;
             std::copy(nodes, nodes + num_nodes, flat.begin() + first_node);
@@ -1062,6 +1162,7 @@ int pt_update_instances(pt_handle h, const float* transforms, int32_t num_instan
             h->flat.swap(flat);
             h->inst_blas = R.inst_blas_ref;
             h->inst_mat = R.inst_material;
+            h->root_ref = std::move(R.root_ref);
             h->tlas_rebuildable = R.tlas_rebuildable;
             h->tlas_base = R.tlas_base;
             h->tlas_on_device = false;

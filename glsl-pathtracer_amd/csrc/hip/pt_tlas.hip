@@ -236,58 +236,87 @@ __global__ __launch_bounds__(kBuildThreads) void k_tlas_build(TlasBuild a) {
         }
         __syncthreads();
 
-        // Move the elements and reduce the child boxes.
-        for (int p = tid; p < n; p += nthr) {
-            const int s = a.pseg[p];
-            if (s < 0) {
-                ord2[p] = ord[p];
-                continue;
-            }
-            const TlasSeg& g = cur[s];
-            const int st = g.start, e = st + g.count;
-            int src = p;
-            int c0 = -1, r0 = 0, c1 = -1, r1 = 0;  // (child, rank) of up to two contributions
-            if (g.mode == 0) {
-                if (p < g.csplit) { c0 = 0; r0 = p - st; }
-                else { c0 = 1; r0 = p - g.csplit; }
-            } else if (g.mode == 1) {
-                c0 = 1; r0 = (p == st) ? 0 : e - p;
-                if (p < g.csplit) { c1 = 0; r1 = p - st; }
-            } else if (g.mode == 2) {
-                c0 = 0; r0 = p - st;
-                if (p >= g.csplit) { c1 = 1; r1 = p - g.csplit; }
-            } else {
-                const int f = a.pre[p + 1] - a.pre[p];
-                if (p < g.split) {
-                    if (!f) src = a.bpos[st + (p - st) - (a.pre[p] - a.pre[st])];
-                    c0 = 0; r0 = p - st;
+        // Move the elements and reduce the child boxes. A wave whose active
+        // lanes all lie in one node reduces the keys across its lanes first and
+        // sends one atomic per key (at the top levels every lane of the block
+        // targets the same 24 keys).
+        for (int base = 0; base < n; base += nthr) {
+            const int p = base + tid;
+            const int s = p < n ? a.pseg[p] : -1;
+            int rk[2] = {-1, -1};  // grow-order rank of this lane's element in each child, -1: none
+            int el = 0;
+            if (p < n && s < 0) ord2[p] = ord[p];
+            if (s >= 0) {
+                const TlasSeg& g = cur[s];
+                const int st = g.start, e = st + g.count;
+                int src = p;
+                if (g.mode == 0) {
+                    if (p < g.csplit) rk[0] = p - st;
+                    else rk[1] = p - g.csplit;
+                } else if (g.mode == 1) {
+                    rk[1] = (p == st) ? 0 : e - p;
+                    if (p < g.csplit) rk[0] = p - st;
+                } else if (g.mode == 2) {
+                    rk[0] = p - st;
+                    if (p >= g.csplit) rk[1] = p - g.csplit;
                 } else {
-                    c0 = 1;
-                    if (f) {
+                    const int f = a.pre[p + 1] - a.pre[p];
+                    if (p < g.split) {
+                        if (!f) src = a.bpos[st + (p - st) - (a.pre[p] - a.pre[st])];
+                        rk[0] = p - st;
+                    } else if (f) {
                         const int r = g.m - 1 - (a.pre[p] - a.pre[g.split]);
                         src = a.apos[st + r];
-                        r0 = e - (r == 0 ? e : a.bpos[st + r - 1]);
+                        rk[1] = e - (r == 0 ? e : a.bpos[st + r - 1]);
                     } else if (p == g.split) {
-                        r0 = e - (g.m == 0 ? e : a.bpos[st + g.m - 1]);
+                        rk[1] = e - (g.m == 0 ? e : a.bpos[st + g.m - 1]);
                     } else {
-                        r0 = e - p;
+                        rk[1] = e - p;
                     }
                 }
+                el = ord[src];
+                ord2[p] = el;
             }
-            const int el = ord[src];
-            ord2[p] = el;
-            const float* b = a.box + 6 * static_cast<size_t>(el);
-            const float* ce = a.cen + 3 * static_cast<size_t>(el);
-            for (int q = 0; q < 2; ++q) {
-                const int c = q == 0 ? c0 : c1;
-                if (c < 0) continue;
-                const unsigned rank = static_cast<unsigned>(q == 0 ? r0 : r1);
-                unsigned long long* k = keys + 24 * static_cast<size_t>(s) + 12 * c;
-                for (int j = 0; j < 3; ++j) {
-                    atomicMin(k + j, key_min(b[j], rank));
-                    atomicMax(k + 3 + j, key_max(b[3 + j], rank));
-                    atomicMin(k + 6 + j, key_min(ce[j], rank));
-                    atomicMax(k + 9 + j, key_max(ce[j], rank));
+            const unsigned long long act = __ballot(s >= 0);
+            if (act == 0ull) continue;
+            const int lead = __ffsll(static_cast<long long>(act)) - 1;
+            const int s_lead = __shfl(s, lead);
+            const bool uniform = __ballot(s >= 0 && s != s_lead) == 0ull;
+            float v[9];  // box min xyz, box max xyz, centre xyz
+            if (s >= 0) {
+                const float* b = a.box + 6 * static_cast<size_t>(el);
+                const float* ce = a.cen + 3 * static_cast<size_t>(el);
+                for (int j = 0; j < 6; ++j) v[j] = b[j];
+                for (int j = 0; j < 3; ++j) v[6 + j] = ce[j];
+            }
+            for (int c = 0; c < 2; ++c) {
+                const bool mine = rk[c] >= 0;
+                unsigned long long* k = keys + 24 * static_cast<size_t>(uniform ? s_lead : (s < 0 ? 0 : s)) + 12 * c;
+                if (uniform) {
+                    if (__ballot(mine) == 0ull) continue;
+                    for (int j = 0; j < 12; ++j) {
+                        // j: 0-2 box min, 3-5 box max, 6-8 centre min, 9-11 centre max
+                        const bool is_min = j < 3 || (j >= 6 && j < 9);
+                        const float x = v[j < 6 ? j : (j < 9 ? j : j - 3)];
+                        unsigned long long q = mine ? (is_min ? key_min(x, rk[c]) : key_max(x, rk[c]))
+                                                    : (is_min ? ~0ull : 0ull);
+                        for (int o = 32; o > 0; o >>= 1) {
+                            const unsigned long long t = __shfl_xor(q, o);
+                            q = is_min ? (t < q ? t : q) : (t > q ? t : q);
+                        }
+                        if (lane == lead) {
+                            if (is_min) atomicMin(k + j, q);
+                            else atomicMax(k + j, q);
+                        }
+                    }
+                } else if (mine) {
+                    const unsigned rank = static_cast<unsigned>(rk[c]);
+                    for (int j = 0; j < 3; ++j) {
+                        atomicMin(k + j, key_min(v[j], rank));
+                        atomicMax(k + 3 + j, key_max(v[3 + j], rank));
+                        atomicMin(k + 6 + j, key_min(v[6 + j], rank));
+                        atomicMax(k + 9 + j, key_max(v[6 + j], rank));
+                    }
                 }
             }
         }

@@ -166,3 +166,21 @@ def test_update_without_nodes_keeps_tlas(renderer):
     rc = renderer._lib.pt_update_instances(renderer._h, api.fptr(xf), n, None, 0,
                                            C.cast(C.byref(v.nodes.contents, 0), C.POINTER(api.PtBvhNode)), 0, 1)
     assert rc == -3  # PT_ERR_STATE
+
+
+def test_tlas_range_update_equals_full(renderer):
+    """pt_update_instances with exactly the TLAS node range re-lays out and
+    uploads the TLAS only; the device nodes equal those of a whole-array update."""
+    n = 300
+    ms = tlas_cases.meshes("clustered", 8)
+    insts, xf1 = tlas_cases.instances("clustered", n, 8)
+    sc = tlas_cases.build_scene(ms, insts)
+    renderer.upload_scene(sc)
+    for i in range(n):
+        sc.SetInstanceTransform(i, xf1[i])
+    sc.RebuildInstances()
+    renderer.update_instances(sc)
+    part = renderer.read_tlas(n)
+    renderer.update_instances(sc, first_node=0)
+    full = renderer.read_tlas(n)
+    assert same(part, full)
