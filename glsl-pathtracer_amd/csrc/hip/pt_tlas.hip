@@ -111,11 +111,14 @@ __global__ __launch_bounds__(kBuildThreads) void k_tlas_build(TlasBuild a) {
     TlasSeg* cur = a.seg0;
     TlasSeg* nxt = a.seg1;
     unsigned long long* keys = a.keys;
+    // Key j of level segment s at keys[j * ks + s]: the 24 keys of one node lie
+    // in 24 cache lines (atomics to one line are serialised in L2).
+    const size_t ks = static_cast<size_t>(n / 2 + 1);
 
     // Root: all elements in index order; centroid bounds grown in index order
     // (bvh.cpp:370-376).
     if (tid == 0) {
-        for (int j = 0; j < 3; ++j) { keys[j] = ~0ull; keys[3 + j] = 0ull; }
+        for (int j = 0; j < 3; ++j) { keys[j * ks] = ~0ull; keys[(3 + j) * ks] = 0ull; }
         lds[L_NSEG] = 1;
         lds[L_NEXT] = 0;
     }
@@ -124,8 +127,8 @@ __global__ __launch_bounds__(kBuildThreads) void k_tlas_build(TlasBuild a) {
     for (int p = tid; p < n; p += nthr) {
         const float* c = a.cen + 3 * static_cast<size_t>(p);
         for (int j = 0; j < 3; ++j) {
-            atomicMin(keys + j, key_min(c[j], p));
-            atomicMax(keys + 3 + j, key_max(c[j], p));
+            atomicMin(keys + j * ks, key_min(c[j], p));
+            atomicMax(keys + (3 + j) * ks, key_max(c[j], p));
         }
     }
     __syncthreads();
@@ -134,7 +137,7 @@ __global__ __launch_bounds__(kBuildThreads) void k_tlas_build(TlasBuild a) {
         g.start = 0;
         g.count = n;
         g.node = a.base;
-        for (int j = 0; j < 6; ++j) g.cb[j] = key_value(keys[j]);
+        for (int j = 0; j < 6; ++j) g.cb[j] = key_value(keys[j * ks]);
     }
     __syncthreads();
 
@@ -155,14 +158,10 @@ __global__ __launch_bounds__(kBuildThreads) void k_tlas_build(TlasBuild a) {
             g.border = (g.cb[3 + axis] + g.cb[axis]) * 0.5f;
             g.mode = ext > 0.f ? 3 : 0;
             g.near2far = (g.count + g.start) & 1;
-            unsigned long long* k = keys + 24 * static_cast<size_t>(s);
-            for (int c = 0; c < 2; ++c)
-                for (int j = 0; j < 3; ++j) {
-                    k[12 * c + j] = ~0ull;
-                    k[12 * c + 3 + j] = 0ull;
-                    k[12 * c + 6 + j] = ~0ull;
-                    k[12 * c + 9 + j] = 0ull;
-                }
+            for (int j = 0; j < 24; ++j) {
+                const int r = j % 12;  // 0-2, 6-8 minima; 3-5, 9-11 maxima
+                keys[j * ks + s] = (r < 3 || (r >= 6 && r < 9)) ? ~0ull : 0ull;
+            }
         }
         if (tid == 0) lds[L_CARRY] = 0;
         __syncthreads();
@@ -236,10 +235,9 @@ __global__ __launch_bounds__(kBuildThreads) void k_tlas_build(TlasBuild a) {
         }
         __syncthreads();
 
-        // Move the elements and reduce the child boxes. A wave whose active
-        // lanes all lie in one node reduces the keys across its lanes first and
-        // sends one atomic per key (at the top levels every lane of the block
-        // targets the same 24 keys).
+        // Move the elements and reduce the child boxes: within a wave first, then
+        // one store, or one atomic when the child spans several waves, per key
+        // and run of lanes.
         for (int base = 0; base < n; base += nthr) {
             const int p = base + tid;
             const int s = p < n ? a.pseg[p] : -1;
@@ -277,11 +275,7 @@ __global__ __launch_bounds__(kBuildThreads) void k_tlas_build(TlasBuild a) {
                 el = ord[src];
                 ord2[p] = el;
             }
-            const unsigned long long act = __ballot(s >= 0);
-            if (act == 0ull) continue;
-            const int lead = __ffsll(static_cast<long long>(act)) - 1;
-            const int s_lead = __shfl(s, lead);
-            const bool uniform = __ballot(s >= 0 && s != s_lead) == 0ull;
+            if (__ballot(s >= 0) == 0ull) continue;
             float v[9];  // box min xyz, box max xyz, centre xyz
             if (s >= 0) {
                 const float* b = a.box + 6 * static_cast<size_t>(el);
@@ -290,32 +284,58 @@ __global__ __launch_bounds__(kBuildThreads) void k_tlas_build(TlasBuild a) {
                 for (int j = 0; j < 3; ++j) v[6 + j] = ce[j];
             }
             for (int c = 0; c < 2; ++c) {
+                // The lanes adding to one child of one node are consecutive
+                // positions, so a segmented inclusive scan over the wave leaves
+                // each run's reduction in its last lane, which sends the atomics.
                 const bool mine = rk[c] >= 0;
-                unsigned long long* k = keys + 24 * static_cast<size_t>(uniform ? s_lead : (s < 0 ? 0 : s)) + 12 * c;
-                if (uniform) {
-                    if (__ballot(mine) == 0ull) continue;
-                    for (int j = 0; j < 12; ++j) {
-                        // j: 0-2 box min, 3-5 box max, 6-8 centre min, 9-11 centre max
-                        const bool is_min = j < 3 || (j >= 6 && j < 9);
-                        const float x = v[j < 6 ? j : (j < 9 ? j : j - 3)];
-                        unsigned long long q = mine ? (is_min ? key_min(x, rk[c]) : key_max(x, rk[c]))
-                                                    : (is_min ? ~0ull : 0ull);
-                        for (int o = 32; o > 0; o >>= 1) {
-                            const unsigned long long t = __shfl_xor(q, o);
-                            q = is_min ? (t < q ? t : q) : (t > q ? t : q);
-                        }
-                        if (lane == lead) {
-                            if (is_min) atomicMin(k + j, q);
-                            else atomicMax(k + j, q);
+                if (__ballot(mine) == 0ull) continue;
+                const int grp = mine ? s : -1;
+                // keys 0-2 box min, 3-5 box max, 6-8 centre min, 9-11 centre max
+                unsigned long long q[12];
+#pragma unroll
+                for (int j = 0; j < 3; ++j) {
+                    q[j] = mine ? key_min(v[j], rk[c]) : ~0ull;
+                    q[3 + j] = mine ? key_max(v[3 + j], rk[c]) : 0ull;
+                    q[6 + j] = mine ? key_min(v[6 + j], rk[c]) : ~0ull;
+                    q[9 + j] = mine ? key_max(v[6 + j], rk[c]) : 0ull;
+                }
+#pragma unroll
+                for (int d = 1; d < 64; d <<= 1) {
+                    const int gs = __shfl_up(grp, d);
+                    const bool take = lane >= d && gs == grp;
+                    unsigned long long t[12];
+#pragma unroll
+                    for (int j = 0; j < 12; ++j) t[j] = __shfl_up(q[j], d);
+                    if (take) {
+#pragma unroll
+                        for (int j = 0; j < 3; ++j) {
+                            q[j] = t[j] < q[j] ? t[j] : q[j];
+                            q[3 + j] = t[3 + j] > q[3 + j] ? t[3 + j] : q[3 + j];
+                            q[6 + j] = t[6 + j] < q[6 + j] ? t[6 + j] : q[6 + j];
+                            q[9 + j] = t[9 + j] > q[9 + j] ? t[9 + j] : q[9 + j];
                         }
                     }
-                } else if (mine) {
-                    const unsigned rank = static_cast<unsigned>(rk[c]);
-                    for (int j = 0; j < 3; ++j) {
-                        atomicMin(k + j, key_min(v[j], rank));
-                        atomicMax(k + 3 + j, key_max(v[3 + j], rank));
-                        atomicMin(k + 6 + j, key_min(v[6 + j], rank));
-                        atomicMax(k + 9 + j, key_max(v[6 + j], rank));
+                }
+                const int gn = __shfl_down(grp, 1);
+                if (mine && (lane == 63 || gn != grp)) {
+                    // The child's positions [lo, hi): when they all lie in this
+                    // wave the run is the whole reduction and is stored as is.
+                    const TlasSeg& g = cur[s];
+                    const int e = g.start + g.count;
+                    const int lo = (c == 1 && g.mode != 1) ? g.csplit : g.start;
+                    const int hi = (c == 0 && g.mode != 2) ? g.csplit : e;
+                    unsigned long long* k = keys + static_cast<size_t>(12 * c) * ks + s;
+                    if (lo >= p - lane && p == hi - 1) {
+#pragma unroll
+                        for (int j = 0; j < 12; ++j) k[j * ks] = q[j];
+                    } else {
+#pragma unroll
+                        for (int j = 0; j < 3; ++j) {
+                            atomicMin(k + j * ks, q[j]);
+                            atomicMax(k + (3 + j) * ks, q[3 + j]);
+                            atomicMin(k + (6 + j) * ks, q[6 + j]);
+                            atomicMax(k + (9 + j) * ks, q[9 + j]);
+                        }
                     }
                 }
             }
@@ -325,9 +345,8 @@ __global__ __launch_bounds__(kBuildThreads) void k_tlas_build(TlasBuild a) {
         // Node records (bvh_translator.cpp:52-80) and the next level.
         for (int s = tid; s < nseg; s += nthr) {
             TlasSeg& g = cur[s];
-            const unsigned long long* k = keys + 24 * static_cast<size_t>(s);
             float v[24];
-            for (int j = 0; j < 24; ++j) v[j] = key_value(k[j]);
+            for (int j = 0; j < 24; ++j) v[j] = key_value(keys[j * ks + s]);
             const int st = g.start, e = st + g.count, cs = g.csplit;
             const int nl = cs - st, nr = e - cs;
             GNode node;

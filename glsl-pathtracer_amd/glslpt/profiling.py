@@ -2,6 +2,7 @@
 
     python -m glslpt.profiling copy NBYTES
     python -m glslpt.profiling heavy [SPP]
+    python -m glslpt.profiling edits
 
 copy: copies NBYTES device-to-device with hipMemcpy (one blit-kernel dispatch
 that reads and writes NBYTES). Profiled with `rocprofv3 --pmc FETCH_SIZE`
@@ -13,6 +14,11 @@ scene at 3840x2160, depth 12, then an instrumented frame of the same samples
 (k_extend<true, ...> kernels, so a kernel filter on "k_extend<false" selects
 the plain frame). Prints k_extend's time and algorithmic bytes (64 B per node,
 36 B per triangle, 52 B per instance, 60 B per light record, 56 B per ray).
+
+edits: the device edit paths on the heavy mixed scene: pt_set_env_map of its
+2048x1024 sky (k_env_lum, k_env_scan) four times, then twenty
+pt_rebuild_instances of its 1003 instances with alternating transforms
+(k_tlas_prep, k_tlas_build). Prints the wall time of each call.
 """
 import ctypes as C
 import sys
@@ -69,10 +75,43 @@ def heavy(spp):
     print(json.dumps(out))
 
 
+def edits():
+    import json
+    import time
+
+    import numpy as np
+
+    from .renderer import Renderer
+    from .scenes import heavy_mixed
+    sc, rs = heavy_mixed()
+    a = sc.arrays()
+    r = Renderer(0)
+    r.upload_scene(sc)
+    sky = np.ascontiguousarray(a["env_img"], np.float32).reshape(a["env_size"][1], a["env_size"][0], 3)
+    env_ms = []
+    for _ in range(4):
+        t0 = time.perf_counter()
+        r.set_env_map(sky)
+        env_ms.append(1e3 * (time.perf_counter() - t0))
+    xf0 = np.ascontiguousarray(a["transforms"], np.float32).reshape(-1, 16)
+    xf1 = xf0.copy()
+    xf1[:, 12:15] += np.random.default_rng(0).uniform(-0.5, 0.5, (xf0.shape[0], 3)).astype(np.float32)
+    tlas_ms = []
+    for k in range(20):
+        t0 = time.perf_counter()
+        r.rebuild_instances(xf1 if k % 2 == 0 else xf0)
+        tlas_ms.append(1e3 * (time.perf_counter() - t0))
+    r.close()
+    print(json.dumps({"env": {"texels": int(sky.shape[0] * sky.shape[1]), "set_env_map_ms": env_ms},
+                      "tlas": {"instances": int(xf0.shape[0]), "rebuild_instances_ms": tlas_ms}}))
+
+
 if __name__ == "__main__":
     if len(sys.argv) == 3 and sys.argv[1] == "copy":
         copy(int(sys.argv[2]))
     elif len(sys.argv) in (2, 3) and sys.argv[1] == "heavy":
         heavy(int(sys.argv[2]) if len(sys.argv) == 3 else 4)
+    elif len(sys.argv) == 2 and sys.argv[1] == "edits":
+        edits()
     else:
-        sys.exit("usage: python -m glslpt.profiling copy NBYTES | heavy [SPP]")
+        sys.exit("usage: python -m glslpt.profiling copy NBYTES | heavy [SPP] | edits")
