@@ -941,13 +941,208 @@ __global__ void __launch_bounds__(256) k_accum_batch(PathBuf pb, BatchBuf bb) {
 constexpr int ENV_CHUNK = 4096;
 constexpr int ENV_SCAN_THREADS = 256;
 
-__global__ void k_env_lum(const float* __restrict__ img, int n, float* __restrict__ w) {
+// Luminance of every texel; clears *ok when one is negative, -0 or not finite
+// (the binade scan below needs finite values >= +0).
+__global__ void k_env_lum(const float* __restrict__ img, int n, float* __restrict__ w, int* __restrict__ ok) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) w[i] = 0.212671f * img[3 * static_cast<size_t>(i)] + 0.715160f * img[3 * static_cast<size_t>(i) + 1] +
-                      0.072169f * img[3 * static_cast<size_t>(i) + 2];
+    if (i >= n) return;
+    const float y = 0.212671f * img[3 * static_cast<size_t>(i)] + 0.715160f * img[3 * static_cast<size_t>(i) + 1] +
+                    0.072169f * img[3 * static_cast<size_t>(i) + 2];
+    w[i] = y;
+    if (!(y >= 0.0f) || (__float_as_uint(y) >> 31) || !(y <= 3.40282347e38f)) *ok = 0;
 }
 
-__global__ void __launch_bounds__(ENV_SCAN_THREADS) k_env_scan(float* cdf, int n, float* __restrict__ total) {
+// Exact parallel form of the same running sum, for finite luminances >= +0.
+// Between two crossings of a power of two by the running sum r, every add
+// rounds on one grid of spacing u (the ulp of r), so with r = R*u (R < 2^24)
+// each add increases R by an integer: floor(w/u), plus one when the fraction
+// of w/u is above one half, or, at exactly one half, when R + floor(w/u) is
+// odd (ties to even). An add is a function of the parity of R only, so a scan
+// of (increment, parity) pairs over a chunk gives every R at once. The first
+// add whose exact sum leaves the grid (R + floor(w/u) >= 2^24) is done as a
+// float add, and the scan restarts on the new grid from the element after it;
+// an add that rounds up onto 2^24 ends the grid too. The values equal the
+// one-lane chain bit for bit.
+constexpr int ZS_THREADS = 1024, ZS_PER = 8, ZS_CHUNK = ZS_THREADS * ZS_PER;
+constexpr int ZS_TOP = 1 << 24, ZS_SAT = 1 << 26;
+constexpr int ZL_POS = 0, ZL_K = 1, ZL_R = 2, ZL_EXIT = 3, ZL_INF = 4, ZL_TOT = 8, ZL_PRE = 8 + 4 * 16;
+
+struct ZFun {  // increments of R and output parities for input parity 0 and 1
+    int a, b, pa, pb;
+};
+__device__ __forceinline__ ZFun zf_identity() { return ZFun{0, 0, 0, 1}; }
+__device__ __forceinline__ ZFun zf_then(ZFun f, ZFun g) {  // f, then g
+    ZFun h;
+    h.a = min(f.a + (f.pa ? g.b : g.a), ZS_SAT);
+    h.pa = f.pa ? g.pb : g.pa;
+    h.b = min(f.b + (f.pb ? g.b : g.a), ZS_SAT);
+    h.pb = f.pb ? g.pb : g.pa;
+    return h;
+}
+__device__ __forceinline__ int zf_inc(int F, int cls, int parity) {
+    return F + (cls == 2 ? 1 : (cls == 1 ? ((parity + F) & 1) : 0));
+}
+__device__ __forceinline__ ZFun zf_elem(int F, int cls) {
+    const int a = zf_inc(F, cls, 0), b = zf_inc(F, cls, 1);
+    return ZFun{a, b, a & 1, (1 + b) & 1};
+}
+// Grid of a value r >= +0: spacing 2^k and r = R * 2^k.
+__device__ __forceinline__ void zone_of(float r, int& k, int& R) {
+    const unsigned bits = __float_as_uint(r);
+    const int e = static_cast<int>((bits >> 23) & 0xffu);
+    k = e == 0 ? -149 : e - 150;
+    R = static_cast<int>(e == 0 ? (bits & 0x7fffffu) : ((bits & 0x7fffffu) | 0x800000u));
+}
+
+__global__ void __launch_bounds__(ZS_THREADS) k_env_zone_scan(float* cdf, int n, float* __restrict__ total,
+                                                              const int* __restrict__ ok) {
+    if (*ok == 0) return;
+    extern __shared__ int lds[];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int nwaves = ZS_THREADS / 64;
+    if (tid == 0) {
+        const float r0 = n > 0 ? cdf[0] : 0.0f;  // cdf[0] = w[0]
+        int k, R;
+        zone_of(r0, k, R);
+        lds[ZL_POS] = 1;
+        lds[ZL_K] = k;
+        lds[ZL_R] = R;
+        lds[ZL_INF] = (__float_as_uint(r0) & 0x7fffffffu) == 0x7f800000u;
+    }
+    __syncthreads();
+    for (;;) {
+        const int pos = lds[ZL_POS];
+        if (pos >= n) break;
+        if (lds[ZL_INF]) {
+            for (int i = pos + tid; i < n; i += ZS_THREADS) cdf[i] = __uint_as_float(0x7f800000u);
+            break;
+        }
+        const int k = lds[ZL_K], R0 = lds[ZL_R];
+        const int base = pos + tid * ZS_PER;
+        float w[ZS_PER];
+        int F[ZS_PER], cls[ZS_PER];
+        ZFun L = zf_identity();
+#pragma unroll
+        for (int j = 0; j < ZS_PER; ++j) {
+            w[j] = 0.0f;
+            F[j] = 0;
+            cls[j] = 0;
+            if (base + j < n) {
+                w[j] = cdf[base + j];
+                const float q = ldexpf(w[j], -k);
+                if (!(q < 16777216.0f)) {
+                    F[j] = ZS_TOP;
+                } else {
+                    const float fq = floorf(q);
+                    const float d = q - fq;
+                    F[j] = static_cast<int>(fq);
+                    cls[j] = d < 0.5f ? 0 : (d == 0.5f ? 1 : 2);
+                }
+                L = zf_then(L, zf_elem(F[j], cls[j]));
+            }
+        }
+        // inclusive scan over the wave, then over the waves
+        ZFun I = L;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            ZFun o;
+            o.a = __shfl_up(I.a, d);
+            o.b = __shfl_up(I.b, d);
+            o.pa = __shfl_up(I.pa, d);
+            o.pb = __shfl_up(I.pb, d);
+            if (lane >= d) I = zf_then(o, I);
+        }
+        if (lane == 63) {
+            lds[ZL_TOT + 4 * wave] = I.a;
+            lds[ZL_TOT + 4 * wave + 1] = I.b;
+            lds[ZL_TOT + 4 * wave + 2] = I.pa;
+            lds[ZL_TOT + 4 * wave + 3] = I.pb;
+        }
+        if (tid == 0) lds[ZL_EXIT] = 0x7fffffff;
+        __syncthreads();
+        if (tid == 0) {
+            ZFun acc = zf_identity();
+            for (int v = 0; v < nwaves; ++v) {
+                lds[ZL_PRE + 4 * v] = acc.a;
+                lds[ZL_PRE + 4 * v + 1] = acc.b;
+                lds[ZL_PRE + 4 * v + 2] = acc.pa;
+                lds[ZL_PRE + 4 * v + 3] = acc.pb;
+                acc = zf_then(acc, ZFun{lds[ZL_TOT + 4 * v], lds[ZL_TOT + 4 * v + 1], lds[ZL_TOT + 4 * v + 2],
+                                        lds[ZL_TOT + 4 * v + 3]});
+            }
+        }
+        __syncthreads();
+        ZFun X;  // everything before this thread's first element
+        X.a = __shfl_up(I.a, 1);
+        X.b = __shfl_up(I.b, 1);
+        X.pa = __shfl_up(I.pa, 1);
+        X.pb = __shfl_up(I.pb, 1);
+        if (lane == 0) X = zf_identity();
+        X = zf_then(ZFun{lds[ZL_PRE + 4 * wave], lds[ZL_PRE + 4 * wave + 1], lds[ZL_PRE + 4 * wave + 2],
+                         lds[ZL_PRE + 4 * wave + 3]}, X);
+        const int Rstart = min(R0 + ((R0 & 1) ? X.b : X.a), ZS_SAT);
+        // exact walk over this thread's elements
+        int Rv[ZS_PER];
+        int R = Rstart, ex = -1, exA = 0;
+#pragma unroll
+        for (int j = 0; j < ZS_PER; ++j) {
+            Rv[j] = R;
+            if (ex < 0 && base + j < n) {
+                if (R + F[j] >= ZS_TOP) {
+                    ex = j;
+                    exA = 1;
+                } else {
+                    R += zf_inc(F[j], cls[j], R & 1);
+                    Rv[j] = R;
+                    if (R >= ZS_TOP) ex = j;
+                }
+            }
+        }
+        if (ex >= 0) atomicMin(&lds[ZL_EXIT], base + ex);
+        __syncthreads();
+        const int estar = lds[ZL_EXIT];
+#pragma unroll
+        for (int j = 0; j < ZS_PER; ++j)
+            if (base + j < n && base + j < estar) cdf[base + j] = ldexpf(static_cast<float>(Rv[j]), k);
+        if (estar != 0x7fffffff) {
+            if (estar >= base && estar < base + ZS_PER) {
+                float rv = 0.0f;
+#pragma unroll
+                for (int j = 0; j < ZS_PER; ++j) {
+                    if (base + j != estar) continue;
+                    if (exA) {
+                        const int Rp = j == 0 ? Rstart : Rv[j > 0 ? j - 1 : 0];
+                        rv = ldexpf(static_cast<float>(Rp), k) + w[j];
+                    } else {
+                        rv = ldexpf(static_cast<float>(Rv[j]), k);
+                    }
+                }
+                cdf[estar] = rv;
+                int nk, nR;
+                zone_of(rv, nk, nR);
+                lds[ZL_POS] = estar + 1;
+                lds[ZL_K] = nk;
+                lds[ZL_R] = nR;
+                lds[ZL_INF] = (__float_as_uint(rv) & 0x7fffffffu) == 0x7f800000u;
+            }
+        } else {
+            const int last = min(pos + ZS_CHUNK, n) - 1;
+            if (last >= base && last < base + ZS_PER) {
+#pragma unroll
+                for (int j = 0; j < ZS_PER; ++j)
+                    if (base + j == last) lds[ZL_R] = Rv[j];
+                lds[ZL_POS] = last + 1;
+            }
+        }
+        __syncthreads();
+    }
+    __syncthreads();
+    if (tid == 0) *total = n > 0 ? cdf[n - 1] : 0.0f;
+}
+
+__global__ void __launch_bounds__(ENV_SCAN_THREADS) k_env_scan(float* cdf, int n, float* __restrict__ total,
+                                                               const int* __restrict__ ok) {
+    if (*ok != 0) return;
     extern __shared__ int lds[];
     float* buf = reinterpret_cast<float*>(lds);  // two chunks
     const int tid = threadIdx.x;
@@ -1232,9 +1427,12 @@ void launch_accum(hipStream_t s, const DevOpts& op, const PathBuf& pb, float4* a
     hipLaunchKernelGGL(k_accum, dim3((npix + 255) / 256), dim3(256), 0, s, op, pb, accum, npix, nsamp);
 }
 
-void launch_env_cdf(hipStream_t s, const float* img, int n, float* cdf, float* total) {
-    if (n > 0) hipLaunchKernelGGL(k_env_lum, dim3((n + 255) / 256), dim3(256), 0, s, img, n, cdf);
-    hipLaunchKernelGGL(k_env_scan, dim3(1), dim3(ENV_SCAN_THREADS), 2 * ENV_CHUNK * sizeof(float), s, cdf, n, total);
+void launch_env_cdf(hipStream_t s, const float* img, int n, float* cdf, float* total, int* ok) {
+    launch_fill(s, ok, 1, 1);
+    if (n > 0) hipLaunchKernelGGL(k_env_lum, dim3((n + 255) / 256), dim3(256), 0, s, img, n, cdf, ok);
+    // exactly one of the two scans does the work, as *ok says
+    hipLaunchKernelGGL(k_env_zone_scan, dim3(1), dim3(ZS_THREADS), (ZL_PRE + 4 * 16) * sizeof(int), s, cdf, n, total, ok);
+    hipLaunchKernelGGL(k_env_scan, dim3(1), dim3(ENV_SCAN_THREADS), 2 * ENV_CHUNK * sizeof(float), s, cdf, n, total, ok);
 }
 
 void launch_fill(hipStream_t s, int* p, int n, int v) {

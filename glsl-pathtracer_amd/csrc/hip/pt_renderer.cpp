@@ -90,6 +90,7 @@ struct pt_renderer_s {
     DevBuf<GLight> lights;
     DevBuf<uint8_t> tex;
     DevBuf<float> env_img, env_cdf, env_total;
+    DevBuf<int> env_ok;
     DevScene sc{};
     bool has_scene = false;
     int num_tris = 0, num_instances = 0, num_materials = 0;
@@ -161,7 +162,7 @@ struct pt_renderer_s {
         DevBuf<int>* ib[] = {&qa, &qb, &hq, &counters, &spill, &ovf};
         for (auto* b : ib) b->free();
         nodes.free(); tris.free(); vidx.free(); verts.free(); norms.free(); mats.free(); inst.free();
-        lights.free(); tex.free(); env_img.free(); env_cdf.free(); env_total.free(); accum.free();
+        lights.free(); tex.free(); env_img.free(); env_cdf.free(); env_total.free(); env_ok.free(); accum.free();
         p_ro.free(); p_rd.free(); p_thr.free(); p_rad.free(); p_aux.free(); p_med.free(); p_hit.free();
         p_rng.free(); p_meta.free(); p_hit2.free(); s_o.free(); s_t.free(); s_d.free(); s_c.free();
         a_rad.free(); a_sg.free(); a_dimg.free(); a_grad.free();
@@ -1000,7 +1001,8 @@ static float build_env(pt_renderer_s* h, const float* rgb, int width, int height
     h->env_img.upload(rgb, 3 * n, h->stream);
     h->env_cdf.ensure(n);
     h->env_total.ensure(1);
-    ptk::launch_env_cdf(h->stream, h->env_img.p, static_cast<int>(n), h->env_cdf.p, h->env_total.p);
+    h->env_ok.ensure(1);
+    ptk::launch_env_cdf(h->stream, h->env_img.p, static_cast<int>(n), h->env_cdf.p, h->env_total.p, h->env_ok.p);
     HIPCHK(hipGetLastError());
     float total = 0.f;
     HIPCHK(hipMemcpyAsync(&total, h->env_total.p, sizeof(float), hipMemcpyDeviceToHost, h->stream));

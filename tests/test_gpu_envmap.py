@@ -96,3 +96,61 @@ def test_set_env_map_equals_host_upload(renderer):
     r = renderer.read_accum()
     assert np.array_equal(g, r)
     assert np.abs(g).max() > 0
+
+
+def _green_for(target):
+    """A green value whose float32 luminance (red = blue = 0) is exactly target."""
+    def lum(g):
+        return np.float32(0.715160) * np.float32(g)
+    g = np.float32(target / 0.715160)
+    for _ in range(4000):
+        v = lum(g)
+        if v == np.float32(target):
+            return g
+        g = np.nextafter(g, np.float32(np.inf) if v < target else np.float32(-np.inf))
+    raise AssertionError(f"no green value for {target}")
+
+
+def special_image(kind, w=4099, h=3):
+    rng = np.random.default_rng(7)
+    img = np.zeros((h, w, 3), np.float32)
+    if kind == "constant":
+        img[:] = 1.0
+    elif kind == "bright_then_dim":
+        img[:] = rng.random((h, w, 3)) * 1e-3
+        img[0, 0] = 1e6
+    elif kind == "subnormal":
+        img[:] = rng.random((h, w, 3)) * 1e-40
+    elif kind == "overflow":
+        img[:] = 3e38
+    elif kind == "ties":
+        # running sum at 2^23 + 1 (grid spacing 1), then halves: every add is a tie
+        img[..., 1] = _green_for(0.5)
+        img[0, 0, 1] = _green_for(2.0 ** 23 + 1)
+    elif kind == "negative":
+        img[:] = rng.random((h, w, 3))
+        img[1, 5, 0] = -3.0
+    elif kind == "neg_zero":
+        img[:] = np.float32(-0.0)
+    elif kind == "inf":
+        img[:] = rng.random((h, w, 3))
+        img[2, 7, 2] = np.inf
+    return img
+
+
+@pytest.mark.parametrize("kind", ["constant", "bright_then_dim", "subnormal", "overflow", "ties", "negative",
+                                  "neg_zero", "inf"])
+def test_device_cdf_special_inputs(renderer, kind):
+    """Ties to even, long runs of adds below half a grid step, subnormal sums,
+    overflow to inf, and (negative, -0, inf luminance) the one-lane fallback:
+    all bit-identical to the sequential float32 sum."""
+    sc, rs = scenes.feature_scene("envmap")
+    renderer.upload_scene(sc)
+    img = special_image(kind)
+    renderer.set_env_map(img)
+    h, w = img.shape[:2]
+    cdf, total = renderer.read_env_cdf(w * h)
+    with np.errstate(over="ignore", invalid="ignore"):
+        want = host_cdf(img)
+    assert np.array_equal(cdf.view(np.uint32), want.view(np.uint32)), kind
+    assert np.float32(total).view(np.uint32) == want[-1].view(np.uint32)
