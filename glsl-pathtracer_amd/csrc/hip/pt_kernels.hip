@@ -963,7 +963,7 @@ __global__ void k_env_lum(const float* __restrict__ img, int n, float* __restric
 // float add, and the scan restarts on the new grid from the element after it;
 // an add that rounds up onto 2^24 ends the grid too. The values equal the
 // one-lane chain bit for bit.
-constexpr int ZS_THREADS = 1024, ZS_PER = 8, ZS_CHUNK = ZS_THREADS * ZS_PER;
+constexpr int ZS_THREADS = 1024, ZS_PER = 16, ZS_CHUNK = ZS_THREADS * ZS_PER;
 constexpr int ZS_TOP = 1 << 24, ZS_SAT = 1 << 26;
 constexpr int ZL_POS = 0, ZL_K = 1, ZL_R = 2, ZL_EXIT = 3, ZL_INF = 4, ZL_TOT = 8, ZL_PRE = 8 + 4 * 16;
 
@@ -1060,15 +1060,31 @@ __global__ void __launch_bounds__(ZS_THREADS) k_env_zone_scan(float* cdf, int n,
         }
         if (tid == 0) lds[ZL_EXIT] = 0x7fffffff;
         __syncthreads();
-        if (tid == 0) {
-            ZFun acc = zf_identity();
-            for (int v = 0; v < nwaves; ++v) {
-                lds[ZL_PRE + 4 * v] = acc.a;
-                lds[ZL_PRE + 4 * v + 1] = acc.b;
-                lds[ZL_PRE + 4 * v + 2] = acc.pa;
-                lds[ZL_PRE + 4 * v + 3] = acc.pb;
-                acc = zf_then(acc, ZFun{lds[ZL_TOT + 4 * v], lds[ZL_TOT + 4 * v + 1], lds[ZL_TOT + 4 * v + 2],
-                                        lds[ZL_TOT + 4 * v + 3]});
+        if (wave == 0) {  // exclusive prefix of the wave totals, one lane per wave
+            ZFun t = zf_identity();
+            if (lane < nwaves)
+                t = ZFun{lds[ZL_TOT + 4 * lane], lds[ZL_TOT + 4 * lane + 1], lds[ZL_TOT + 4 * lane + 2],
+                         lds[ZL_TOT + 4 * lane + 3]};
+#pragma unroll
+            for (int d = 1; d < nwaves; d <<= 1) {
+                ZFun o;
+                o.a = __shfl_up(t.a, d);
+                o.b = __shfl_up(t.b, d);
+                o.pa = __shfl_up(t.pa, d);
+                o.pb = __shfl_up(t.pb, d);
+                if (lane >= d) t = zf_then(o, t);
+            }
+            ZFun e;
+            e.a = __shfl_up(t.a, 1);
+            e.b = __shfl_up(t.b, 1);
+            e.pa = __shfl_up(t.pa, 1);
+            e.pb = __shfl_up(t.pb, 1);
+            if (lane == 0) e = zf_identity();
+            if (lane < nwaves) {
+                lds[ZL_PRE + 4 * lane] = e.a;
+                lds[ZL_PRE + 4 * lane + 1] = e.b;
+                lds[ZL_PRE + 4 * lane + 2] = e.pa;
+                lds[ZL_PRE + 4 * lane + 3] = e.pb;
             }
         }
         __syncthreads();
