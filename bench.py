@@ -42,6 +42,8 @@ heavy.tlas_rebuild: an edit that moves all 1000 instances, as the device
 rebuild (pt_rebuild_instances) against the host path it replaces (the scene
 builder's RebuildInstances + pt_update_instances with the TLAS range), wall
 time per call.
+heavy.env_cdf: replacing the heavy scene's 2048x1024 sky, pt_set_env_map (image
+upload + CDF built on the GPU) against the host CDF build of the scene library.
 
 cpu_baseline: the CPU oracle (C++ restatement of the reference shaders,
 OpenMP) on the host cores, rendering a bounded sample of the same frame
@@ -245,8 +247,33 @@ def heavy_throughput(steps, warmup, count=True):
                                               "instances": cw["instances_entered"] / max(1, cw["rays_closest"])},
                                   "rays_per_sample": cw["rays_closest"] / (rs.width * rs.height)}
     out["tlas_rebuild"] = tlas_rebuild_times(r, sc, a)
+    out["env_cdf"] = env_cdf_times(r, a)
     r.close()
     return out
+
+
+def env_cdf_times(r, a, reps=4):
+    """Environment-map change of the heavy scene's 2048x1024 sky: pt_set_env_map
+    (24 MB image up, CDF built on the GPU, bit-identical to BuildCDF) against the
+    host build of the scene library (EnvironmentMap::BuildCDF's running sum on one
+    core, plus the image copy), which a host path would then upload with the
+    8 MB CDF. Wall time per call, best of `reps`."""
+    import numpy as np
+
+    from glslpt.scene import Scene
+    sky = np.ascontiguousarray(a["env_img"], np.float32).reshape(a["env_size"][1], a["env_size"][0], 3)
+    dev, host = [], []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        r.set_env_map(sky)
+        dev.append(time.perf_counter() - t0)
+    s = Scene()
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        s.AddEnvMap(sky)
+        host.append(time.perf_counter() - t0)
+    return {"texels": int(sky.shape[0] * sky.shape[1]), "device_set_env_map_ms": 1e3 * min(dev),
+            "host_build_ms": 1e3 * min(host)}
 
 
 def tlas_rebuild_times(r, sc, a, reps=5):
