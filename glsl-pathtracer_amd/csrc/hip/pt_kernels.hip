@@ -572,7 +572,7 @@ __device__ __forceinline__ DevOpts job_opts(const DevOpts& op, const JobDev& J) 
 // (bounce 0 of a batched pass) takes the slot count and each slot's job,
 // sample and pixel from the device-side pass state.
 template <bool COUNT, bool PRIMARY, bool BATCH>
-__global__ void __launch_bounds__(256) k_extend(DevScene sc, DevOpts op, PathBuf pb, const int* __restrict__ qin,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) k_extend(DevScene sc, DevOpts op, PathBuf pb, const int* __restrict__ qin,
                                                 const int* count_in, int npaths, int npix, int first_sample,
                                                 int* hitq, int* hit_count, int* work, int S, int* spill,
                                                 unsigned long long* counters, int* ovf_flag, BatchBuf bb) {
