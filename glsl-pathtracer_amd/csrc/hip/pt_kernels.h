@@ -101,8 +101,9 @@ void launch_shade_batch(hipStream_t s, int blocks, int bs, const ptd::DevScene& 
                         bool compact);
 void launch_accum_batch(hipStream_t s, int blocks, const ptd::PathBuf& pb, const ptd::BatchBuf& bb);
 // Environment-map CDF and total luminance of an RGB32F image of n texels
-// (ok: one int of scratch).
-void launch_env_cdf(hipStream_t s, const float* img, int n, float* cdf, float* total, int* ok);
+// (ok: one int of scratch; scratch: env_cdf_scratch_bytes(n) bytes, 8-byte aligned).
+size_t env_cdf_scratch_bytes(int n);
+void launch_env_cdf(hipStream_t s, const float* img, int n, float* cdf, float* total, int* ok, void* scratch);
 void launch_fill(hipStream_t s, int* p, int n, int v);
 void launch_query_closest(hipStream_t s, const ptd::DevScene& sc, const ptd::DevOpts& op, const float* o,
                           const float* d, int n, float* t, int* inst, int* tri, int* light, float* uv, int S,

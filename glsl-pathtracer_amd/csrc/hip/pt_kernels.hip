@@ -965,7 +965,8 @@ __global__ void k_env_lum(const float* __restrict__ img, int n, float* __restric
 // one-lane chain bit for bit.
 constexpr int ZS_THREADS = 1024, ZS_PER = 16, ZS_CHUNK = ZS_THREADS * ZS_PER;
 constexpr int ZS_TOP = 1 << 24, ZS_SAT = 1 << 26;
-constexpr int ZL_POS = 0, ZL_K = 1, ZL_R = 2, ZL_EXIT = 3, ZL_INF = 4, ZL_TOT = 8, ZL_PRE = 8 + 4 * 16;
+constexpr int ZL_POS = 0, ZL_K = 1, ZL_R = 2, ZL_EXIT = 3, ZL_INF = 4, ZL_CUR = 5, ZL_TOT = 8, ZL_PRE = 8 + 4 * 16;
+constexpr int ZL_WORDS = ZL_PRE + 4 * 17;
 
 struct ZFun {  // increments of R and output parities for input parity 0 and 1
     int a, b, pa, pb;
@@ -994,108 +995,125 @@ __device__ __forceinline__ void zone_of(float r, int& k, int& R) {
     R = static_cast<int>(e == 0 ? (bits & 0x7fffffu) : ((bits & 0x7fffffu) | 0x800000u));
 }
 
-__global__ void __launch_bounds__(ZS_THREADS) k_env_zone_scan(float* cdf, int n, float* __restrict__ total,
-                                                              const int* __restrict__ ok) {
-    if (*ok == 0) return;
-    extern __shared__ int lds[];
+// Texels [base, base + ZS_PER) below `end` on the grid 2^k: the input values,
+// their integer parts F and fraction classes, and their composed function.
+__device__ __forceinline__ ZFun zs_load(const float* cdf, int base, int end, int k, float (&w)[ZS_PER],
+                                        int (&F)[ZS_PER], int (&cls)[ZS_PER]) {
+    ZFun L = zf_identity();
+#pragma unroll
+    for (int j = 0; j < ZS_PER; ++j) {
+        w[j] = 0.0f;
+        F[j] = 0;
+        cls[j] = 0;
+        if (base + j < end) {
+            w[j] = cdf[base + j];
+            const float q = ldexpf(w[j], -k);
+            if (!(q < 16777216.0f)) {
+                F[j] = ZS_TOP;
+            } else {
+                const float fq = floorf(q);
+                const float d = q - fq;
+                F[j] = static_cast<int>(fq);
+                cls[j] = d < 0.5f ? 0 : (d == 0.5f ? 1 : 2);
+            }
+            L = zf_then(L, zf_elem(F[j], cls[j]));
+        }
+    }
+    return L;
+}
+
+// Block scan of the threads' functions (every thread calls it): returns the
+// composition of all earlier threads' functions; `tot` gets the whole block's.
+__device__ __forceinline__ ZFun zs_block_scan(ZFun L, int* lds, ZFun& tot) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int nwaves = ZS_THREADS / 64;
-    if (tid == 0) {
-        const float r0 = n > 0 ? cdf[0] : 0.0f;  // cdf[0] = w[0]
-        int k, R;
-        zone_of(r0, k, R);
-        lds[ZL_POS] = 1;
-        lds[ZL_K] = k;
-        lds[ZL_R] = R;
-        lds[ZL_INF] = (__float_as_uint(r0) & 0x7fffffffu) == 0x7f800000u;
+    constexpr int nwaves = ZS_THREADS / 64;
+    ZFun I = L;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        ZFun o;
+        o.a = __shfl_up(I.a, d);
+        o.b = __shfl_up(I.b, d);
+        o.pa = __shfl_up(I.pa, d);
+        o.pb = __shfl_up(I.pb, d);
+        if (lane >= d) I = zf_then(o, I);
+    }
+    if (lane == 63) {
+        lds[ZL_TOT + 4 * wave] = I.a;
+        lds[ZL_TOT + 4 * wave + 1] = I.b;
+        lds[ZL_TOT + 4 * wave + 2] = I.pa;
+        lds[ZL_TOT + 4 * wave + 3] = I.pb;
     }
     __syncthreads();
+    if (wave == 0) {  // exclusive prefix of the wave totals, one lane per wave
+        ZFun t = zf_identity();
+        if (lane < nwaves)
+            t = ZFun{lds[ZL_TOT + 4 * lane], lds[ZL_TOT + 4 * lane + 1], lds[ZL_TOT + 4 * lane + 2],
+                     lds[ZL_TOT + 4 * lane + 3]};
+#pragma unroll
+        for (int d = 1; d < nwaves; d <<= 1) {
+            ZFun o;
+            o.a = __shfl_up(t.a, d);
+            o.b = __shfl_up(t.b, d);
+            o.pa = __shfl_up(t.pa, d);
+            o.pb = __shfl_up(t.pb, d);
+            if (lane >= d) t = zf_then(o, t);
+        }
+        ZFun e;
+        e.a = __shfl_up(t.a, 1);
+        e.b = __shfl_up(t.b, 1);
+        e.pa = __shfl_up(t.pa, 1);
+        e.pb = __shfl_up(t.pb, 1);
+        if (lane == 0) e = zf_identity();
+        if (lane < nwaves) {
+            lds[ZL_PRE + 4 * lane] = e.a;
+            lds[ZL_PRE + 4 * lane + 1] = e.b;
+            lds[ZL_PRE + 4 * lane + 2] = e.pa;
+            lds[ZL_PRE + 4 * lane + 3] = e.pb;
+        }
+        if (lane == nwaves - 1) {
+            lds[ZL_PRE + 4 * nwaves] = t.a;
+            lds[ZL_PRE + 4 * nwaves + 1] = t.b;
+            lds[ZL_PRE + 4 * nwaves + 2] = t.pa;
+            lds[ZL_PRE + 4 * nwaves + 3] = t.pb;
+        }
+    }
+    __syncthreads();
+    ZFun X;
+    X.a = __shfl_up(I.a, 1);
+    X.b = __shfl_up(I.b, 1);
+    X.pa = __shfl_up(I.pa, 1);
+    X.pb = __shfl_up(I.pb, 1);
+    if (lane == 0) X = zf_identity();
+    X = zf_then(ZFun{lds[ZL_PRE + 4 * wave], lds[ZL_PRE + 4 * wave + 1], lds[ZL_PRE + 4 * wave + 2],
+                     lds[ZL_PRE + 4 * wave + 3]}, X);
+    tot = ZFun{lds[ZL_PRE + 4 * nwaves], lds[ZL_PRE + 4 * nwaves + 1], lds[ZL_PRE + 4 * nwaves + 2],
+               lds[ZL_PRE + 4 * nwaves + 3]};
+    return X;
+}
+
+// The exact running sum of texels [pos, end) from the state in LDS (ZL_POS,
+// ZL_K, ZL_R, ZL_INF), leaving the state after texel end - 1 (every thread of
+// the block calls it).
+__device__ void zone_walk(float* cdf, int end, int* lds) {
+    const int tid = threadIdx.x;
     for (;;) {
         const int pos = lds[ZL_POS];
-        if (pos >= n) break;
+        if (pos >= end) break;
         if (lds[ZL_INF]) {
-            for (int i = pos + tid; i < n; i += ZS_THREADS) cdf[i] = __uint_as_float(0x7f800000u);
+            for (int i = pos + tid; i < end; i += ZS_THREADS) cdf[i] = __uint_as_float(0x7f800000u);
+            __syncthreads();
+            if (tid == 0) lds[ZL_POS] = end;
+            __syncthreads();
             break;
         }
         const int k = lds[ZL_K], R0 = lds[ZL_R];
         const int base = pos + tid * ZS_PER;
         float w[ZS_PER];
         int F[ZS_PER], cls[ZS_PER];
-        ZFun L = zf_identity();
-#pragma unroll
-        for (int j = 0; j < ZS_PER; ++j) {
-            w[j] = 0.0f;
-            F[j] = 0;
-            cls[j] = 0;
-            if (base + j < n) {
-                w[j] = cdf[base + j];
-                const float q = ldexpf(w[j], -k);
-                if (!(q < 16777216.0f)) {
-                    F[j] = ZS_TOP;
-                } else {
-                    const float fq = floorf(q);
-                    const float d = q - fq;
-                    F[j] = static_cast<int>(fq);
-                    cls[j] = d < 0.5f ? 0 : (d == 0.5f ? 1 : 2);
-                }
-                L = zf_then(L, zf_elem(F[j], cls[j]));
-            }
-        }
-        // inclusive scan over the wave, then over the waves
-        ZFun I = L;
-#pragma unroll
-        for (int d = 1; d < 64; d <<= 1) {
-            ZFun o;
-            o.a = __shfl_up(I.a, d);
-            o.b = __shfl_up(I.b, d);
-            o.pa = __shfl_up(I.pa, d);
-            o.pb = __shfl_up(I.pb, d);
-            if (lane >= d) I = zf_then(o, I);
-        }
-        if (lane == 63) {
-            lds[ZL_TOT + 4 * wave] = I.a;
-            lds[ZL_TOT + 4 * wave + 1] = I.b;
-            lds[ZL_TOT + 4 * wave + 2] = I.pa;
-            lds[ZL_TOT + 4 * wave + 3] = I.pb;
-        }
+        const ZFun L = zs_load(cdf, base, end, k, w, F, cls);
         if (tid == 0) lds[ZL_EXIT] = 0x7fffffff;
-        __syncthreads();
-        if (wave == 0) {  // exclusive prefix of the wave totals, one lane per wave
-            ZFun t = zf_identity();
-            if (lane < nwaves)
-                t = ZFun{lds[ZL_TOT + 4 * lane], lds[ZL_TOT + 4 * lane + 1], lds[ZL_TOT + 4 * lane + 2],
-                         lds[ZL_TOT + 4 * lane + 3]};
-#pragma unroll
-            for (int d = 1; d < nwaves; d <<= 1) {
-                ZFun o;
-                o.a = __shfl_up(t.a, d);
-                o.b = __shfl_up(t.b, d);
-                o.pa = __shfl_up(t.pa, d);
-                o.pb = __shfl_up(t.pb, d);
-                if (lane >= d) t = zf_then(o, t);
-            }
-            ZFun e;
-            e.a = __shfl_up(t.a, 1);
-            e.b = __shfl_up(t.b, 1);
-            e.pa = __shfl_up(t.pa, 1);
-            e.pb = __shfl_up(t.pb, 1);
-            if (lane == 0) e = zf_identity();
-            if (lane < nwaves) {
-                lds[ZL_PRE + 4 * lane] = e.a;
-                lds[ZL_PRE + 4 * lane + 1] = e.b;
-                lds[ZL_PRE + 4 * lane + 2] = e.pa;
-                lds[ZL_PRE + 4 * lane + 3] = e.pb;
-            }
-        }
-        __syncthreads();
-        ZFun X;  // everything before this thread's first element
-        X.a = __shfl_up(I.a, 1);
-        X.b = __shfl_up(I.b, 1);
-        X.pa = __shfl_up(I.pa, 1);
-        X.pb = __shfl_up(I.pb, 1);
-        if (lane == 0) X = zf_identity();
-        X = zf_then(ZFun{lds[ZL_PRE + 4 * wave], lds[ZL_PRE + 4 * wave + 1], lds[ZL_PRE + 4 * wave + 2],
-                         lds[ZL_PRE + 4 * wave + 3]}, X);
+        ZFun tot;
+        const ZFun X = zs_block_scan(L, lds, tot);
         const int Rstart = min(R0 + ((R0 & 1) ? X.b : X.a), ZS_SAT);
         // exact walk over this thread's elements
         int Rv[ZS_PER];
@@ -1103,7 +1121,7 @@ __global__ void __launch_bounds__(ZS_THREADS) k_env_zone_scan(float* cdf, int n,
 #pragma unroll
         for (int j = 0; j < ZS_PER; ++j) {
             Rv[j] = R;
-            if (ex < 0 && base + j < n) {
+            if (ex < 0 && base + j < end) {
                 if (R + F[j] >= ZS_TOP) {
                     ex = j;
                     exA = 1;
@@ -1119,7 +1137,7 @@ __global__ void __launch_bounds__(ZS_THREADS) k_env_zone_scan(float* cdf, int n,
         const int estar = lds[ZL_EXIT];
 #pragma unroll
         for (int j = 0; j < ZS_PER; ++j)
-            if (base + j < n && base + j < estar) cdf[base + j] = ldexpf(static_cast<float>(Rv[j]), k);
+            if (base + j < end && base + j < estar) cdf[base + j] = ldexpf(static_cast<float>(Rv[j]), k);
         if (estar != 0x7fffffff) {
             if (estar >= base && estar < base + ZS_PER) {
                 float rv = 0.0f;
@@ -1142,7 +1160,7 @@ __global__ void __launch_bounds__(ZS_THREADS) k_env_zone_scan(float* cdf, int n,
                 lds[ZL_INF] = (__float_as_uint(rv) & 0x7fffffffu) == 0x7f800000u;
             }
         } else {
-            const int last = min(pos + ZS_CHUNK, n) - 1;
+            const int last = min(pos + ZS_CHUNK, end) - 1;
             if (last >= base && last < base + ZS_PER) {
 #pragma unroll
                 for (int j = 0; j < ZS_PER; ++j)
@@ -1152,8 +1170,138 @@ __global__ void __launch_bounds__(ZS_THREADS) k_env_zone_scan(float* cdf, int n,
         }
         __syncthreads();
     }
+}
+
+// The same sum spread over the grid. Texels 1..n-1 form chunks of ZS_CHUNK.
+// (1) Each chunk's sum in double gives an estimate of the running sum before
+// it, hence a guess of its grid. (2) Each chunk's function on its guessed grid.
+// (3) One workgroup carries the exact state from chunk to chunk: a chunk whose
+// grid was guessed right and that stays on it is applied as a whole; any other
+// chunk is summed exactly by zone_walk. (4) The chunks applied as a whole write
+// their values from the state carried into them.
+// Scratch per chunk: double sum, guessed grid, function (4 ints), carried R and
+// a flag (the chunk was applied as a whole).
+__global__ void __launch_bounds__(256) k_env_chunk_sum(const float* __restrict__ cdf, int n, double* __restrict__ csum,
+                                                       const int* __restrict__ ok) {
+    if (*ok == 0) return;
+    extern __shared__ int lds[];
+    double* part = reinterpret_cast<double*>(lds);
+    const int c = blockIdx.x;
+    const int lo = 1 + c * ZS_CHUNK, hi = min(lo + ZS_CHUNK, n);
+    double acc = 0.0;
+    for (int i = lo + static_cast<int>(threadIdx.x); i < hi; i += 256) acc += static_cast<double>(cdf[i]);
+    part[threadIdx.x] = acc;
     __syncthreads();
-    if (tid == 0) *total = n > 0 ? cdf[n - 1] : 0.0f;
+    for (int h = 128; h > 0; h >>= 1) {
+        if (static_cast<int>(threadIdx.x) < h) part[threadIdx.x] += part[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) csum[c] = part[0];
+}
+
+__global__ void k_env_guess(const float* __restrict__ cdf, int nchunks, const double* __restrict__ csum,
+                            int* __restrict__ kguess, const int* __restrict__ ok) {
+    if (*ok == 0 || threadIdx.x != 0) return;
+    double run = static_cast<double>(cdf[0]);
+    for (int c = 0; c < nchunks; ++c) {
+        int k, R;
+        zone_of(static_cast<float>(run), k, R);
+        kguess[c] = k;
+        run += csum[c];
+    }
+}
+
+__global__ void __launch_bounds__(ZS_THREADS) k_env_chunk_fun(const float* __restrict__ cdf, int n,
+                                                              const int* __restrict__ kguess, int4* __restrict__ fun,
+                                                              const int* __restrict__ ok) {
+    if (*ok == 0) return;
+    extern __shared__ int lds[];
+    const int c = blockIdx.x;
+    const int lo = 1 + c * ZS_CHUNK, hi = min(lo + ZS_CHUNK, n);
+    float w[ZS_PER];
+    int F[ZS_PER], cls[ZS_PER];
+    const ZFun L = zs_load(cdf, lo + static_cast<int>(threadIdx.x) * ZS_PER, hi, kguess[c], w, F, cls);
+    ZFun tot;
+    (void)zs_block_scan(L, lds, tot);
+    if (threadIdx.x == 0) fun[c] = make_int4(tot.a, tot.b, tot.pa, tot.pb);
+}
+
+__global__ void __launch_bounds__(ZS_THREADS) k_env_carry(float* cdf, int n, int nchunks, const int* __restrict__ kguess,
+                                                          const int4* __restrict__ fun, int2* __restrict__ cstate,
+                                                          float* __restrict__ total, const int* __restrict__ ok) {
+    if (*ok == 0) return;
+    extern __shared__ int lds[];
+    const int tid = threadIdx.x;
+    if (tid == 0) {
+        const float r0 = n > 0 ? cdf[0] : 0.0f;  // cdf[0] = w[0]
+        int k, R;
+        zone_of(r0, k, R);
+        lds[ZL_POS] = 1;
+        lds[ZL_K] = k;
+        lds[ZL_R] = R;
+        lds[ZL_INF] = (__float_as_uint(r0) & 0x7fffffffu) == 0x7f800000u;
+        lds[ZL_CUR] = 0;
+    }
+    __syncthreads();
+    for (;;) {
+        if (tid == 0) {  // chunks that stay on their guessed grid: O(1) each
+            int c = lds[ZL_CUR], R = lds[ZL_R], pos = lds[ZL_POS];
+            const int k = lds[ZL_K];
+            while (c < nchunks && !lds[ZL_INF]) {
+                const int lo = 1 + c * ZS_CHUNK;
+                if (pos != lo || k != kguess[c]) break;
+                const int4 f = fun[c];
+                const int inc = (R & 1) ? f.y : f.x;
+                if (R + inc >= ZS_TOP) break;
+                cstate[c] = make_int2(R, 1);
+                R += inc;
+                pos = min(lo + ZS_CHUNK, n);
+                ++c;
+            }
+            lds[ZL_CUR] = c;
+            lds[ZL_R] = R;
+            lds[ZL_POS] = pos;
+        }
+        __syncthreads();
+        const int c = lds[ZL_CUR];
+        if (c >= nchunks) break;
+        // the next chunk leaves its grid, or the grid was guessed wrong: exactly
+        if (tid == 0) cstate[c] = make_int2(0, 0);
+        zone_walk(cdf, min(1 + (c + 1) * ZS_CHUNK, n), lds);
+        if (tid == 0) lds[ZL_CUR] = c + 1;
+        __syncthreads();
+    }
+    if (tid == 0)
+        *total = n <= 0 ? 0.0f : (lds[ZL_INF] ? __uint_as_float(0x7f800000u) : ldexpf(static_cast<float>(lds[ZL_R]), lds[ZL_K]));
+}
+
+__global__ void __launch_bounds__(ZS_THREADS) k_env_chunk_write(float* cdf, int n, const int* __restrict__ kguess,
+                                                                const int2* __restrict__ cstate,
+                                                                const int* __restrict__ ok) {
+    if (*ok == 0) return;
+    const int c = blockIdx.x;
+    const int2 cs = cstate[c];
+    if (cs.y == 0) return;  // summed exactly by k_env_carry
+    extern __shared__ int lds[];
+    const int k = kguess[c];
+    const int lo = 1 + c * ZS_CHUNK, hi = min(lo + ZS_CHUNK, n);
+    const int base = lo + static_cast<int>(threadIdx.x) * ZS_PER;
+    float w[ZS_PER];
+    int F[ZS_PER], cls[ZS_PER];
+    const ZFun L = zs_load(cdf, base, hi, k, w, F, cls);
+    ZFun tot;
+    const ZFun X = zs_block_scan(L, lds, tot);
+    int R = cs.x + ((cs.x & 1) ? X.b : X.a);
+    float out[ZS_PER];
+#pragma unroll
+    for (int j = 0; j < ZS_PER; ++j) {
+        R += zf_inc(F[j], cls[j], R & 1);
+        out[j] = ldexpf(static_cast<float>(R), k);
+    }
+    __syncthreads();  // every thread has read its inputs before any writes
+#pragma unroll
+    for (int j = 0; j < ZS_PER; ++j)
+        if (base + j < hi) cdf[base + j] = out[j];
 }
 
 __global__ void __launch_bounds__(ENV_SCAN_THREADS) k_env_scan(float* cdf, int n, float* __restrict__ total,
@@ -1443,11 +1591,32 @@ void launch_accum(hipStream_t s, const DevOpts& op, const PathBuf& pb, float4* a
     hipLaunchKernelGGL(k_accum, dim3((npix + 255) / 256), dim3(256), 0, s, op, pb, accum, npix, nsamp);
 }
 
-void launch_env_cdf(hipStream_t s, const float* img, int n, float* cdf, float* total, int* ok) {
+size_t env_cdf_scratch_bytes(int n) {
+    const size_t nchunks = n > 1 ? (static_cast<size_t>(n) - 2) / ZS_CHUNK + 1 : 0;
+    return nchunks * (sizeof(double) + sizeof(int) + sizeof(int4) + sizeof(int2)) + 64;
+}
+
+void launch_env_cdf(hipStream_t s, const float* img, int n, float* cdf, float* total, int* ok, void* scratch) {
     launch_fill(s, ok, 1, 1);
     if (n > 0) hipLaunchKernelGGL(k_env_lum, dim3((n + 255) / 256), dim3(256), 0, s, img, n, cdf, ok);
-    // exactly one of the two scans does the work, as *ok says
-    hipLaunchKernelGGL(k_env_zone_scan, dim3(1), dim3(ZS_THREADS), (ZL_PRE + 4 * 16) * sizeof(int), s, cdf, n, total, ok);
+    const int nchunks = n > 1 ? (n - 2) / ZS_CHUNK + 1 : 0;
+    // scratch: int4 first (16-byte aligned), then double, int2, int
+    int4* fun = static_cast<int4*>(scratch);
+    double* csum = reinterpret_cast<double*>(fun + nchunks);
+    int2* cstate = reinterpret_cast<int2*>(csum + nchunks);
+    int* kguess = reinterpret_cast<int*>(cstate + nchunks);
+    // exactly one of the two paths does the work, as *ok says
+    if (nchunks > 0) {
+        hipLaunchKernelGGL(k_env_chunk_sum, dim3(nchunks), dim3(256), 256 * sizeof(double), s, cdf, n, csum, ok);
+        hipLaunchKernelGGL(k_env_guess, dim3(1), dim3(64), 0, s, cdf, nchunks, csum, kguess, ok);
+        hipLaunchKernelGGL(k_env_chunk_fun, dim3(nchunks), dim3(ZS_THREADS), ZL_WORDS * sizeof(int), s, cdf, n, kguess,
+                           fun, ok);
+    }
+    hipLaunchKernelGGL(k_env_carry, dim3(1), dim3(ZS_THREADS), ZL_WORDS * sizeof(int), s, cdf, n, nchunks, kguess, fun,
+                       cstate, total, ok);
+    if (nchunks > 0)
+        hipLaunchKernelGGL(k_env_chunk_write, dim3(nchunks), dim3(ZS_THREADS), ZL_WORDS * sizeof(int), s, cdf, n,
+                           kguess, cstate, ok);
     hipLaunchKernelGGL(k_env_scan, dim3(1), dim3(ENV_SCAN_THREADS), 2 * ENV_CHUNK * sizeof(float), s, cdf, n, total, ok);
 }
 

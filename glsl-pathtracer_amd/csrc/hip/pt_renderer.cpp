@@ -91,6 +91,7 @@ struct pt_renderer_s {
     DevBuf<uint8_t> tex;
     DevBuf<float> env_img, env_cdf, env_total;
     DevBuf<int> env_ok;
+    DevBuf<double> env_scratch;  // chunk sums, grids, functions and states of the CDF scan
     DevScene sc{};
     bool has_scene = false;
     int num_tris = 0, num_instances = 0, num_materials = 0;
@@ -162,7 +163,7 @@ struct pt_renderer_s {
         DevBuf<int>* ib[] = {&qa, &qb, &hq, &counters, &spill, &ovf};
         for (auto* b : ib) b->free();
         nodes.free(); tris.free(); vidx.free(); verts.free(); norms.free(); mats.free(); inst.free();
-        lights.free(); tex.free(); env_img.free(); env_cdf.free(); env_total.free(); env_ok.free(); accum.free();
+        lights.free(); tex.free(); env_img.free(); env_cdf.free(); env_total.free(); env_ok.free(); env_scratch.free(); accum.free();
         p_ro.free(); p_rd.free(); p_thr.free(); p_rad.free(); p_aux.free(); p_med.free(); p_hit.free();
         p_rng.free(); p_meta.free(); p_hit2.free(); s_o.free(); s_t.free(); s_d.free(); s_c.free();
         a_rad.free(); a_sg.free(); a_dimg.free(); a_grad.free();
@@ -1002,7 +1003,9 @@ static float build_env(pt_renderer_s* h, const float* rgb, int width, int height
     h->env_cdf.ensure(n);
     h->env_total.ensure(1);
     h->env_ok.ensure(1);
-    ptk::launch_env_cdf(h->stream, h->env_img.p, static_cast<int>(n), h->env_cdf.p, h->env_total.p, h->env_ok.p);
+    h->env_scratch.ensure((ptk::env_cdf_scratch_bytes(static_cast<int>(n)) + 7) / 8);
+    ptk::launch_env_cdf(h->stream, h->env_img.p, static_cast<int>(n), h->env_cdf.p, h->env_total.p, h->env_ok.p,
+                        h->env_scratch.p);
     HIPCHK(hipGetLastError());
     float total = 0.f;
     HIPCHK(hipMemcpyAsync(&total, h->env_total.p, sizeof(float), hipMemcpyDeviceToHost, h->stream));

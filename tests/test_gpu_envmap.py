@@ -34,14 +34,14 @@ def ragged_image(w, h, seed):
     return img
 
 
-@pytest.mark.parametrize("size", [(1, 1), (7, 5), (64, 32), (4099, 3), (2048, 1024)])
+@pytest.mark.parametrize("size", [(1, 1), (7, 5), (64, 32), (4099, 3), (4099, 12), (512, 256), (2048, 1024)])
 def test_device_cdf_equals_host(renderer, size):
     """Single texel, a tail shorter than one step, one chunk, several chunks with
-    a short last one, and the C5 sky size."""
+    a short last one (ragged and sky), and the C5 sky size."""
     sc, rs = scenes.feature_scene("envmap")
     renderer.upload_scene(sc)
     w, h = size
-    img = scenes.sky_envmap(w, h) if size in ((64, 32), (2048, 1024)) else ragged_image(w, h, 3)
+    img = scenes.sky_envmap(w, h) if size in ((64, 32), (512, 256), (2048, 1024)) else ragged_image(w, h, 3)
     renderer.set_env_map(img)
     cdf, total = renderer.read_env_cdf(w * h)
     want = host_cdf(img)
@@ -111,7 +111,7 @@ def _green_for(target):
     raise AssertionError(f"no green value for {target}")
 
 
-def special_image(kind, w=4099, h=3):
+def special_image(kind, w=4099, h=9):
     rng = np.random.default_rng(7)
     img = np.zeros((h, w, 3), np.float32)
     if kind == "constant":
