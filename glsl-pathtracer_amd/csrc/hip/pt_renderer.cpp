@@ -109,7 +109,7 @@ struct pt_renderer_s {
     int tlas_base = 0;
     DevBuf<float> t_xf, t_local, t_box, t_cen;
     DevBuf<int> t_blas, t_mat, t_ord, t_pre, t_pos, t_pseg;
-    DevBuf<ptd::TlasSeg> t_seg;
+    DevBuf<int> t_seg;
     DevBuf<unsigned long long> t_keys;
 
     // render state
@@ -1217,7 +1217,7 @@ int pt_rebuild_instances(pt_handle h, const float* transforms, int32_t num_insta
         h->t_pre.ensure(nn + 1);
         h->t_pos.ensure(2 * nn);
         h->t_pseg.ensure(nn);
-        h->t_seg.ensure(2 * half);
+        h->t_seg.ensure(ptd::kTlasSegWords * half);
         h->t_keys.ensure(24 * half);
         ptd::TlasBuild a{};
         a.n = n;
@@ -1236,8 +1236,7 @@ int pt_rebuild_instances(pt_handle h, const float* transforms, int32_t num_insta
         a.apos = h->t_pos.p;
         a.bpos = h->t_pos.p + nn;
         a.pseg = h->t_pseg.p;
-        a.seg0 = h->t_seg.p;
-        a.seg1 = h->t_seg.p + half;
+        a.seg = h->t_seg.p;
         a.keys = h->t_keys.p;
         ptk::launch_tlas_rebuild(s, a);
         HIPCHK(hipGetLastError());

@@ -45,20 +45,20 @@ __host__ __device__ __forceinline__ bool inst_record(const float* m, int blas_re
     return true;
 }
 
-// One node of a TLAS level under construction: elements [start, start+count)
-// of the order array, its internal node index, its split plane and the
-// outcome of the split.
-struct TlasSeg {
-    int start, count, node;
-    int mode;        // 0 middle split, no partition; 1 all right; 2 all left; 3 partitioned
-    int axis, near2far;
-    int split;       // partition split (modes 1..3)
-    int csplit;      // first element of the right child
-    int m;           // swapped pairs (mode 3)
-    int child[2];    // next-level segment of each child, -1 for a leaf
-    float border;
-    float cb[6];     // centroid bounds: min xyz, max xyz
-};
+// Segment table of the build: one segment per node of the level under
+// construction (elements [start, start + count) of the order array), as word
+// arrays of h = n / 2 + 1 entries each, in this order:
+//   start[2], count[2], node[2]  (two buffers: current and next level)
+//   cb[6 h]     centroid bounds, min xyz, max xyz per segment (read when the
+//               level starts, written for the next level's segments later)
+//   border      split plane (float)
+//   info        mode | axis << 2 | near2far << 4; mode 0 middle split without
+//               partition, 1 all right, 2 all left, 3 partitioned
+//   split       partition split (modes 1..3)
+//   csplit      first element of the right child
+//   m           swapped pairs (mode 3)
+//   child[2]    next-level segment of each child, -1 for a leaf
+constexpr int kTlasSegWords = 19;
 
 // Buffers of one rebuild over n instances (device pointers).
 struct TlasBuild {
@@ -78,8 +78,7 @@ struct TlasBuild {
     int* apos;              // [n] positions of the swapped pairs, per node
     int* bpos;
     int* pseg;              // [n] level segment of each position, -1 when in a leaf
-    TlasSeg* seg0;          // [n / 2 + 1] segments of the current and the next level
-    TlasSeg* seg1;
+    int* seg;               // [(n / 2 + 1) * kTlasSegWords] segment table
     unsigned long long* keys;  // [(n / 2 + 1) * 24] child box reduction keys per segment
 };
 

@@ -40,12 +40,17 @@ namespace {
 using ptd::GInst;
 using ptd::GNode;
 using ptd::TlasBuild;
-using ptd::TlasSeg;
+using ptd::kTlasSegWords;
 
 constexpr int kBuildThreads = 1024;
 // LDS words of the build kernel.
 constexpr int L_CARRY = 0, L_NSEG = 1, L_NEXT = 2, L_WTOT = 4;  // L_WTOT: one word per wave
 constexpr int kBuildLdsWords = L_WTOT + kBuildThreads / 64;
+// Up to this many instances the per-position arrays (two orders, segment of
+// each position, the two swap-position tables, the side-flag prefix count:
+// 6n + 1 words) and the segment table live in LDS after the words above
+// (63,648 bytes at 1024); beyond it in global memory.
+constexpr int kLdsElems = 1024;
 
 // std::min / std::max of the reference's Vec3::Min / Max (Vec3.h:119-135):
 // the first argument wins ties.
@@ -105,11 +110,29 @@ __global__ __launch_bounds__(kBuildThreads) void k_tlas_build(TlasBuild a) {
     extern __shared__ int lds[];
     const int n = a.n;
     const int tid = threadIdx.x, nthr = blockDim.x;
-    const int lane = tid & 63, wave = tid >> 6, nwaves = (nthr + 63) >> 6;
-    int* ord = a.ord0;
-    int* ord2 = a.ord1;
-    TlasSeg* cur = a.seg0;
-    TlasSeg* nxt = a.seg1;
+    const int lane = tid & 63, wave = tid >> 6;
+    const bool in_lds = n <= kLdsElems;
+    int* const arr = lds + kBuildLdsWords;
+    int* ord = in_lds ? arr : a.ord0;
+    int* ord2 = in_lds ? arr + n : a.ord1;
+    int* const pseg = in_lds ? arr + 2 * n : a.pseg;
+    int* const apos = in_lds ? arr + 3 * n : a.apos;
+    int* const bpos = in_lds ? arr + 4 * n : a.bpos;
+    int* const pre = in_lds ? arr + 5 * n : a.pre;  // n + 1 words
+    // Segment table (layout in pt_tlas.h), after the per-position arrays.
+    const int hs = n / 2 + 1;
+    int* const sw = in_lds ? arr + 6 * n + 1 : a.seg;
+    int* const sstart = sw;  // start, count, node: [2 hs], current level at lv * hs
+    int* const scount = sw + 2 * hs;
+    int* const snode = sw + 4 * hs;
+    float* const scb = reinterpret_cast<float*>(sw + 6 * hs);
+    float* const sborder = reinterpret_cast<float*>(sw + 12 * hs);
+    int* const sinfo = sw + 13 * hs;
+    int* const ssplit = sw + 14 * hs;
+    int* const scsplit = sw + 15 * hs;
+    int* const sm = sw + 16 * hs;
+    int* const schild = sw + 17 * hs;  // child c of segment s at c * hs + s
+    int lv = 0;
     unsigned long long* keys = a.keys;
     // Key j of level segment s at keys[j * ks + s]: the 24 keys of one node lie
     // in 24 cache lines (atomics to one line are serialised in L2).
@@ -122,22 +145,41 @@ __global__ __launch_bounds__(kBuildThreads) void k_tlas_build(TlasBuild a) {
         lds[L_NSEG] = 1;
         lds[L_NEXT] = 0;
     }
-    for (int p = tid; p < n; p += nthr) { ord[p] = p; a.pseg[p] = 0; }
+    for (int p = tid; p < n; p += nthr) { ord[p] = p; pseg[p] = 0; }
     __syncthreads();
-    for (int p = tid; p < n; p += nthr) {
-        const float* c = a.cen + 3 * static_cast<size_t>(p);
-        for (int j = 0; j < 3; ++j) {
-            atomicMin(keys + j * ks, key_min(c[j], p));
-            atomicMax(keys + (3 + j) * ks, key_max(c[j], p));
+    {
+        // Reduced per thread, then over the wave; one atomic per wave and key.
+        unsigned long long q[6] = {~0ull, ~0ull, ~0ull, 0ull, 0ull, 0ull};
+        for (int p = tid; p < n; p += nthr) {
+            const float* c = a.cen + 3 * static_cast<size_t>(p);
+            for (int j = 0; j < 3; ++j) {
+                const unsigned long long kn = key_min(c[j], p), kx = key_max(c[j], p);
+                q[j] = kn < q[j] ? kn : q[j];
+                q[3 + j] = kx > q[3 + j] ? kx : q[3 + j];
+            }
+        }
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                const unsigned long long kn = __shfl_xor(q[j], d), kx = __shfl_xor(q[3 + j], d);
+                q[j] = kn < q[j] ? kn : q[j];
+                q[3 + j] = kx > q[3 + j] ? kx : q[3 + j];
+            }
+        }
+        if (lane == 0) {
+            for (int j = 0; j < 3; ++j) {
+                atomicMin(keys + j * ks, q[j]);
+                atomicMax(keys + (3 + j) * ks, q[3 + j]);
+            }
         }
     }
     __syncthreads();
     if (tid == 0) {
-        TlasSeg& g = cur[0];
-        g.start = 0;
-        g.count = n;
-        g.node = a.base;
-        for (int j = 0; j < 6; ++j) g.cb[j] = key_value(keys[j * ks]);
+        sstart[0] = 0;
+        scount[0] = n;
+        snode[0] = a.base;
+        for (int j = 0; j < 6; ++j) scb[j] = key_value(keys[j * ks]);
     }
     __syncthreads();
 
@@ -147,17 +189,15 @@ __global__ __launch_bounds__(kBuildThreads) void k_tlas_build(TlasBuild a) {
 
         // Split plane of every node of the level (bvh.cpp:94-96, :125-129).
         for (int s = tid; s < nseg; s += nthr) {
-            TlasSeg& g = cur[s];
-            const float ex = g.cb[3] - g.cb[0], ey = g.cb[4] - g.cb[1], ez = g.cb[5] - g.cb[2];
+            const float* cb = scb + 6 * s;
+            const float ex = cb[3] - cb[0], ey = cb[4] - cb[1], ez = cb[5] - cb[2];
             int axis = 0;  // bbox::maxdim (bbox.h:70-83)
             if (ex >= ey && ex >= ez) axis = 0;
             else if (ey >= ex && ey >= ez) axis = 1;
             else if (ez >= ex && ez >= ey) axis = 2;
             const float ext = axis == 0 ? ex : (axis == 1 ? ey : ez);
-            g.axis = axis;
-            g.border = (g.cb[3 + axis] + g.cb[axis]) * 0.5f;
-            g.mode = ext > 0.f ? 3 : 0;
-            g.near2far = (g.count + g.start) & 1;
+            sborder[s] = (cb[3 + axis] + cb[axis]) * 0.5f;
+            sinfo[s] = (ext > 0.f ? 3 : 0) | (axis << 2) | (((scount[lv * hs + s] + sstart[lv * hs + s]) & 1) << 4);
             for (int j = 0; j < 24; ++j) {
                 const int r = j % 12;  // 0-2, 6-8 minima; 3-5, 9-11 maxima
                 keys[j * ks + s] = (r < 3 || (r >= 6 && r < 9)) ? ~0ull : 0ull;
@@ -172,11 +212,11 @@ __global__ __launch_bounds__(kBuildThreads) void k_tlas_build(TlasBuild a) {
             const int p = base + tid;
             int f = 0;
             if (p < n) {
-                const int s = a.pseg[p];
-                if (s >= 0 && cur[s].mode == 3) {
-                    const TlasSeg& g = cur[s];
-                    const float c = a.cen[3 * static_cast<size_t>(ord[p]) + g.axis];
-                    f = g.near2far ? (c < g.border) : (c >= g.border);
+                const int s = pseg[p];
+                const int in = s >= 0 ? sinfo[s] : 0;
+                if ((in & 3) == 3) {
+                    const float c = a.cen[3 * static_cast<size_t>(ord[p]) + ((in >> 2) & 3)];
+                    f = (in & 16) ? (c < sborder[s]) : (c >= sborder[s]);
                 }
             }
             const unsigned long long mask = __ballot(f);
@@ -184,53 +224,52 @@ __global__ __launch_bounds__(kBuildThreads) void k_tlas_build(TlasBuild a) {
                 static_cast<unsigned>(mask >> 32), __builtin_amdgcn_mbcnt_lo(static_cast<unsigned>(mask), 0u)));
             if (lane == 0) lds[L_WTOT + wave] = __popcll(mask);
             __syncthreads();
-            if (tid == 0) {
-                int c = lds[L_CARRY];
-                for (int w = 0; w < nwaves; ++w) {
-                    const int t = lds[L_WTOT + w];
-                    lds[L_WTOT + w] = c;
-                    c += t;
-                }
-                lds[L_CARRY] = c;
+            // Every wave adds up the counts of the waves before it.
+            int off = lds[L_CARRY], tot = 0;
+#pragma unroll
+            for (int w = 0; w < kBuildThreads / 64; ++w) {
+                const int t = lds[L_WTOT + w];
+                off += w < wave ? t : 0;
+                tot += t;
             }
+            if (p < n) pre[p] = off + below;
             __syncthreads();
-            if (p < n) a.pre[p] = lds[L_WTOT + wave] + below;
-            __syncthreads();
+            if (tid == 0) lds[L_CARRY] += tot;
         }
-        if (tid == 0) a.pre[n] = lds[L_CARRY];
+        if (tid == 0) pre[n] = lds[L_CARRY];
         __syncthreads();
 
         // Split of every node; fallback to the middle (bvh.cpp:198-212).
         for (int s = tid; s < nseg; s += nthr) {
-            TlasSeg& g = cur[s];
-            const int st = g.start, e = st + g.count;
-            int csplit = st + (g.count >> 1);
-            g.m = 0;
-            if (g.mode == 3) {
-                const int split = st + (a.pre[e] - a.pre[st]);
-                g.split = split;
-                if (split == st) g.mode = 1;
-                else if (split == e) g.mode = 2;
+            const int st = sstart[lv * hs + s], cnt = scount[lv * hs + s], e = st + cnt;
+            int csplit = st + (cnt >> 1), mm = 0, in = sinfo[s];
+            if ((in & 3) == 3) {
+                const int split = st + (pre[e] - pre[st]);
+                ssplit[s] = split;
+                if (split == st) in = (in & ~3) | 1;
+                else if (split == e) in = (in & ~3) | 2;
                 else {
                     csplit = split;
-                    g.m = (split - st) - (a.pre[split] - a.pre[st]);
+                    mm = (split - st) - (pre[split] - pre[st]);
                 }
             }
-            g.csplit = csplit;
+            sinfo[s] = in;
+            sm[s] = mm;
+            scsplit[s] = csplit;
         }
         __syncthreads();
 
         // Positions of the swapped pairs, by rank: a = right-going below the
         // split (from the left), b = left-going at or above it (from the right).
         for (int p = tid; p < n; p += nthr) {
-            const int s = a.pseg[p];
-            if (s < 0 || cur[s].mode != 3) continue;
-            const TlasSeg& g = cur[s];
-            const int f = a.pre[p + 1] - a.pre[p];
-            if (p < g.split) {
-                if (!f) a.apos[g.start + (p - g.start) - (a.pre[p] - a.pre[g.start])] = p;
+            const int s = pseg[p];
+            if (s < 0 || (sinfo[s] & 3) != 3) continue;
+            const int st = sstart[lv * hs + s], sp = ssplit[s];
+            const int f = pre[p + 1] - pre[p];
+            if (p < sp) {
+                if (!f) apos[st + (p - st) - (pre[p] - pre[st])] = p;
             } else if (f) {
-                a.bpos[g.start + g.m - 1 - (a.pre[p] - a.pre[g.split])] = p;
+                bpos[st + sm[s] - 1 - (pre[p] - pre[sp])] = p;
             }
         }
         __syncthreads();
@@ -240,34 +279,37 @@ __global__ __launch_bounds__(kBuildThreads) void k_tlas_build(TlasBuild a) {
         // and run of lanes.
         for (int base = 0; base < n; base += nthr) {
             const int p = base + tid;
-            const int s = p < n ? a.pseg[p] : -1;
+            const int s = p < n ? pseg[p] : -1;
             int rk[2] = {-1, -1};  // grow-order rank of this lane's element in each child, -1: none
-            int el = 0;
+            int el = 0, st = 0, e = 0, cs = 0, mode = 0;
             if (p < n && s < 0) ord2[p] = ord[p];
             if (s >= 0) {
-                const TlasSeg& g = cur[s];
-                const int st = g.start, e = st + g.count;
+                st = sstart[lv * hs + s];
+                e = st + scount[lv * hs + s];
+                cs = scsplit[s];
+                mode = sinfo[s] & 3;
                 int src = p;
-                if (g.mode == 0) {
-                    if (p < g.csplit) rk[0] = p - st;
-                    else rk[1] = p - g.csplit;
-                } else if (g.mode == 1) {
+                if (mode == 0) {
+                    if (p < cs) rk[0] = p - st;
+                    else rk[1] = p - cs;
+                } else if (mode == 1) {
                     rk[1] = (p == st) ? 0 : e - p;
-                    if (p < g.csplit) rk[0] = p - st;
-                } else if (g.mode == 2) {
+                    if (p < cs) rk[0] = p - st;
+                } else if (mode == 2) {
                     rk[0] = p - st;
-                    if (p >= g.csplit) rk[1] = p - g.csplit;
+                    if (p >= cs) rk[1] = p - cs;
                 } else {
-                    const int f = a.pre[p + 1] - a.pre[p];
-                    if (p < g.split) {
-                        if (!f) src = a.bpos[st + (p - st) - (a.pre[p] - a.pre[st])];
+                    const int sp = ssplit[s], mm = sm[s];
+                    const int f = pre[p + 1] - pre[p];
+                    if (p < sp) {
+                        if (!f) src = bpos[st + (p - st) - (pre[p] - pre[st])];
                         rk[0] = p - st;
                     } else if (f) {
-                        const int r = g.m - 1 - (a.pre[p] - a.pre[g.split]);
-                        src = a.apos[st + r];
-                        rk[1] = e - (r == 0 ? e : a.bpos[st + r - 1]);
-                    } else if (p == g.split) {
-                        rk[1] = e - (g.m == 0 ? e : a.bpos[st + g.m - 1]);
+                        const int r = mm - 1 - (pre[p] - pre[sp]);
+                        src = apos[st + r];
+                        rk[1] = e - (r == 0 ? e : bpos[st + r - 1]);
+                    } else if (p == sp) {
+                        rk[1] = e - (mm == 0 ? e : bpos[st + mm - 1]);
                     } else {
                         rk[1] = e - p;
                     }
@@ -283,13 +325,20 @@ __global__ __launch_bounds__(kBuildThreads) void k_tlas_build(TlasBuild a) {
                 for (int j = 0; j < 6; ++j) v[j] = b[j];
                 for (int j = 0; j < 3; ++j) v[6 + j] = ce[j];
             }
-            for (int c = 0; c < 2; ++c) {
-                // The lanes adding to one child of one node are consecutive
-                // positions, so a segmented inclusive scan over the wave leaves
-                // each run's reduction in its last lane, which sends the atomics.
-                const bool mine = rk[c] >= 0;
+            // A position adds to one child, except in the fallback splits, where
+            // the side that received everything spans the whole node and the
+            // other half adds to the other child as well. The first pass takes
+            // the spanning child (else the only one), the second the other.
+            const int cp = mode == 1 ? 1 : (mode == 2 ? 0 : (rk[0] >= 0 ? 0 : 1));
+            for (int pass = 0; pass < 2; ++pass) {
+                const int c = pass == 0 ? cp : 1 - cp;
+                const bool mine = s >= 0 && rk[c] >= 0;
                 if (__ballot(mine) == 0ull) continue;
-                const int grp = mine ? s : -1;
+                // The child's positions [lo, hi) are consecutive lanes, so a
+                // segmented inclusive scan over the wave leaves the reduction of
+                // the run in its last lane; the scan stops at the longest run.
+                const int lo = (c == 1 && mode != 1) ? cs : st;
+                const int hi = (c == 0 && mode != 2) ? cs : e;
                 // keys 0-2 box min, 3-5 box max, 6-8 centre min, 9-11 centre max
                 unsigned long long q[12];
 #pragma unroll
@@ -299,10 +348,9 @@ __global__ __launch_bounds__(kBuildThreads) void k_tlas_build(TlasBuild a) {
                     q[6 + j] = mine ? key_min(v[6 + j], rk[c]) : ~0ull;
                     q[9 + j] = mine ? key_max(v[6 + j], rk[c]) : 0ull;
                 }
-#pragma unroll
                 for (int d = 1; d < 64; d <<= 1) {
-                    const int gs = __shfl_up(grp, d);
-                    const bool take = lane >= d && gs == grp;
+                    const bool take = mine && lane >= d && p - d >= lo;
+                    if (__ballot(take) == 0ull) break;
                     unsigned long long t[12];
 #pragma unroll
                     for (int j = 0; j < 12; ++j) t[j] = __shfl_up(q[j], d);
@@ -316,14 +364,9 @@ __global__ __launch_bounds__(kBuildThreads) void k_tlas_build(TlasBuild a) {
                         }
                     }
                 }
-                const int gn = __shfl_down(grp, 1);
-                if (mine && (lane == 63 || gn != grp)) {
-                    // The child's positions [lo, hi): when they all lie in this
-                    // wave the run is the whole reduction and is stored as is.
-                    const TlasSeg& g = cur[s];
-                    const int e = g.start + g.count;
-                    const int lo = (c == 1 && g.mode != 1) ? g.csplit : g.start;
-                    const int hi = (c == 0 && g.mode != 2) ? g.csplit : e;
+                if (mine && (lane == 63 || p == hi - 1)) {
+                    // When the whole child lies in this wave the run is the whole
+                    // reduction and is stored as is.
                     unsigned long long* k = keys + static_cast<size_t>(12 * c) * ks + s;
                     if (lo >= p - lane && p == hi - 1) {
 #pragma unroll
@@ -342,43 +385,72 @@ __global__ __launch_bounds__(kBuildThreads) void k_tlas_build(TlasBuild a) {
         }
         __syncthreads();
 
-        // Node records (bvh_translator.cpp:52-80) and the next level.
-        for (int s = tid; s < nseg; s += nthr) {
-            TlasSeg& g = cur[s];
+        // Node records (bvh_translator.cpp:52-80) and the next level. The
+        // children that are inner nodes are numbered in position order (a
+        // prefix count over the level's segments, themselves in position
+        // order), so the runs of one wave reduce into neighbouring key slots.
+        for (int sb = 0; sb < nseg; sb += nthr) {
+            const int s = sb + tid;
+            const bool live = s < nseg;
+            int st = 0, cs = 0, gn = 0, nl = 0, nr = 0;
             float v[24];
-            for (int j = 0; j < 24; ++j) v[j] = key_value(keys[j * ks + s]);
-            const int st = g.start, e = st + g.count, cs = g.csplit;
-            const int nl = cs - st, nr = e - cs;
-            GNode node;
-            node.a = make_float4(v[0], v[1], v[2], v[3]);
-            node.b = make_float4(v[4], v[5], v[12], v[13]);
-            node.c = make_float4(v[14], v[15], v[16], v[17]);
-            node.d = make_int4(nl == 1 ? -1 - ord2[st] : g.node + 1, nr == 1 ? -1 - ord2[cs] : g.node + nl, 0, 0);
-            a.nodes[g.node] = node;
-            g.child[0] = g.child[1] = -1;
-            for (int c = 0; c < 2; ++c) {
-                const int cnt = c == 0 ? nl : nr;
-                if (cnt < 2) continue;
-                const int id = atomicAdd(&lds[L_NEXT], 1);
-                TlasSeg& ch = nxt[id];
-                ch.start = c == 0 ? st : cs;
-                ch.count = cnt;
-                ch.node = c == 0 ? g.node + 1 : g.node + nl;
-                for (int j = 0; j < 6; ++j) ch.cb[j] = v[12 * c + 6 + j];
-                g.child[c] = id;
+            if (live) {
+                for (int j = 0; j < 24; ++j) v[j] = key_value(keys[j * ks + s]);
+                st = sstart[lv * hs + s];
+                cs = scsplit[s];
+                gn = snode[lv * hs + s];
+                nl = cs - st;
+                nr = st + scount[lv * hs + s] - cs;
+                GNode node;
+                node.a = make_float4(v[0], v[1], v[2], v[3]);
+                node.b = make_float4(v[4], v[5], v[12], v[13]);
+                node.c = make_float4(v[14], v[15], v[16], v[17]);
+                node.d = make_int4(nl == 1 ? -1 - ord2[st] : gn + 1, nr == 1 ? -1 - ord2[cs] : gn + nl, 0, 0);
+                a.nodes[gn] = node;
             }
+            const unsigned long long ml = __ballot(live && nl >= 2), mr = __ballot(live && nr >= 2);
+            const int below = static_cast<int>(
+                __builtin_amdgcn_mbcnt_hi(static_cast<unsigned>(ml >> 32), __builtin_amdgcn_mbcnt_lo(static_cast<unsigned>(ml), 0u)) +
+                __builtin_amdgcn_mbcnt_hi(static_cast<unsigned>(mr >> 32), __builtin_amdgcn_mbcnt_lo(static_cast<unsigned>(mr), 0u)));
+            if (lane == 0) lds[L_WTOT + wave] = __popcll(ml) + __popcll(mr);
+            __syncthreads();
+            int id = lds[L_NEXT], tot = 0;
+#pragma unroll
+            for (int w = 0; w < kBuildThreads / 64; ++w) {
+                const int t = lds[L_WTOT + w];
+                id += w < wave ? t : 0;
+                tot += t;
+            }
+            id += below;
+            if (live) {
+                const int nx = (lv ^ 1) * hs;
+                for (int c = 0; c < 2; ++c) {
+                    const int cnt = c == 0 ? nl : nr;
+                    int cid = -1;
+                    if (cnt >= 2) {
+                        cid = id++;
+                        sstart[nx + cid] = c == 0 ? st : cs;
+                        scount[nx + cid] = cnt;
+                        snode[nx + cid] = c == 0 ? gn + 1 : gn + nl;
+                        // Only the start of a level reads the centroid bounds.
+                        for (int j = 0; j < 6; ++j) scb[6 * cid + j] = v[12 * c + 6 + j];
+                    }
+                    schild[c * hs + s] = cid;
+                }
+            }
+            __syncthreads();
+            if (tid == 0) lds[L_NEXT] += tot;
         }
         __syncthreads();
 
         for (int p = tid; p < n; p += nthr) {
-            const int s = a.pseg[p];
+            const int s = pseg[p];
             if (s < 0) continue;
-            const TlasSeg& g = cur[s];
-            a.pseg[p] = p < g.csplit ? g.child[0] : g.child[1];
+            pseg[p] = p < scsplit[s] ? schild[s] : schild[hs + s];
         }
         {
             int* t = ord; ord = ord2; ord2 = t;
-            TlasSeg* u = cur; cur = nxt; nxt = u;
+            lv ^= 1;
         }
         __syncthreads();
         if (tid == 0) {
@@ -391,8 +463,10 @@ __global__ __launch_bounds__(kBuildThreads) void k_tlas_build(TlasBuild a) {
 
 void launch_tlas_rebuild(hipStream_t s, const TlasBuild& a) {
     hipLaunchKernelGGL(k_tlas_prep, dim3((a.n + 255) / 256), dim3(256), 0, s, a);
-    if (a.n >= 2)
-        hipLaunchKernelGGL(k_tlas_build, dim3(1), dim3(kBuildThreads), kBuildLdsWords * sizeof(int), s, a);
+    if (a.n >= 2) {
+        const int words = kBuildLdsWords + (a.n <= kLdsElems ? 6 * a.n + 1 + kTlasSegWords * (a.n / 2 + 1) : 0);
+        hipLaunchKernelGGL(k_tlas_build, dim3(1), dim3(kBuildThreads), words * sizeof(int), s, a);
+    }
 }
 
 }  // namespace ptk

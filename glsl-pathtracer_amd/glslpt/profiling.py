@@ -16,9 +16,11 @@ the plain frame). Prints k_extend's time and algorithmic bytes (64 B per node,
 36 B per triangle, 52 B per instance, 60 B per light record, 56 B per ray).
 
 edits: the device edit paths on the heavy mixed scene: pt_set_env_map of its
-2048x1024 sky (k_env_lum, k_env_scan) four times, then twenty
-pt_rebuild_instances of its 1003 instances with alternating transforms
-(k_tlas_prep, k_tlas_build). Prints the wall time of each call.
+2048x1024 sky (k_env_lum, the chunked CDF kernels k_env_chunk_sum, k_env_guess,
+k_env_chunk_fun, k_env_carry, k_env_chunk_write, and the k_env_scan fallback)
+four times, then twenty pt_rebuild_instances of its 1003 instances with
+alternating transforms (k_tlas_prep, k_tlas_build). Prints the wall time of
+each call.
 """
 import ctypes as C
 import sys
