@@ -1214,7 +1214,7 @@ int pt_rebuild_instances(pt_handle h, const float* transforms, int32_t num_insta
         h->t_box.ensure(6 * nn);
         h->t_cen.ensure(3 * nn);
         h->t_ord.ensure(2 * nn);
-        h->t_pre.ensure(nn + 1);
+        h->t_pre.ensure(nn + 2);
         h->t_pos.ensure(2 * nn);
         h->t_pseg.ensure(nn);
         h->t_seg.ensure(ptd::kTlasSegWords * half);
@@ -1238,6 +1238,8 @@ int pt_rebuild_instances(pt_handle h, const float* transforms, int32_t num_insta
         a.pseg = h->t_pseg.p;
         a.seg = h->t_seg.p;
         a.keys = h->t_keys.p;
+        a.negz = h->t_pre.p + nn + 1;
+        HIPCHK(hipMemsetAsync(a.negz, 0, sizeof(int), s));
         ptk::launch_tlas_rebuild(s, a);
         HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(s));

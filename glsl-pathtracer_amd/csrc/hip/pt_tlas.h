@@ -80,6 +80,7 @@ struct TlasBuild {
     int* pseg;              // [n] level segment of each position, -1 when in a leaf
     int* seg;               // [(n / 2 + 1) * kTlasSegWords] segment table
     unsigned long long* keys;  // [(n / 2 + 1) * 24] child box reduction keys per segment
+    int* negz;              // [1] set when a world box or centre coordinate is -0.0
 };
 
 }  // namespace ptd

@@ -78,6 +78,23 @@ __device__ __forceinline__ float key_value(unsigned long long k) {
     return __uint_as_float(u);
 }
 
+// The rank only decides the sign of a zero extreme. Without any -0.0 among
+// the world boxes and centres every zero is +0.0, and the value word alone
+// (32-bit keys) gives the same bounds.
+template <typename K> struct Key;
+template <> struct Key<unsigned long long> {
+    static __device__ __forceinline__ unsigned long long lo(float v, unsigned r) { return key_min(v, r); }
+    static __device__ __forceinline__ unsigned long long hi(float v, unsigned r) { return key_max(v, r); }
+    static __device__ __forceinline__ float value(unsigned long long k) { return key_value(k); }
+};
+template <> struct Key<unsigned> {
+    static __device__ __forceinline__ unsigned lo(float v, unsigned) { return ord_bits(v); }
+    static __device__ __forceinline__ unsigned hi(float v, unsigned) { return ord_bits(v); }
+    static __device__ __forceinline__ float value(unsigned o) {
+        return __uint_as_float((o & 0x80000000u) ? (o & 0x7FFFFFFFu) : ~o);
+    }
+};
+
 }  // namespace
 
 // Per instance: world box (Scene.cpp:155-178), its centre (bbox.cpp:28) and
@@ -89,6 +106,7 @@ __global__ void k_tlas_prep(TlasBuild a) {
     const float* lb = a.local + 6 * static_cast<size_t>(i);
     float* box = a.box + 6 * static_cast<size_t>(i);
     float* cen = a.cen + 3 * static_cast<size_t>(i);
+    bool negz = false;
     for (int r = 0; r < 3; ++r) {
         const float xa = m[r] * lb[0], xb = m[r] * lb[3];
         const float ya = m[4 + r] * lb[1], yb = m[4 + r] * lb[4];
@@ -98,16 +116,20 @@ __global__ void k_tlas_prep(TlasBuild a) {
         box[r] = mn;
         box[3 + r] = mx;
         cen[r] = (mx + mn) * 0.5f;
+        negz = negz || __float_as_uint(mn) == 0x80000000u || __float_as_uint(mx) == 0x80000000u ||
+               __float_as_uint(cen[r]) == 0x80000000u;
     }
+    if (negz) a.negz[0] = 1;
     GInst g;
     // The host validated every transform; a singular one cannot reach here.
     (void)ptd::inst_record(m, a.blas_ref[i], a.mat[i], &g);
     a.inst[i] = g;
 }
 
-// Level-synchronous top-down build in one workgroup (n >= 2).
-__global__ __launch_bounds__(kBuildThreads) void k_tlas_build(TlasBuild a) {
-    extern __shared__ int lds[];
+// Level-synchronous top-down build in one workgroup (n >= 2), K: key type.
+template <typename K>
+__device__ __forceinline__ void tlas_build(const TlasBuild& a, int* lds) {
+    constexpr K kNone = static_cast<K>(~static_cast<K>(0));
     const int n = a.n;
     const int tid = threadIdx.x, nthr = blockDim.x;
     const int lane = tid & 63, wave = tid >> 6;
@@ -133,7 +155,7 @@ __global__ __launch_bounds__(kBuildThreads) void k_tlas_build(TlasBuild a) {
     int* const sm = sw + 16 * hs;
     int* const schild = sw + 17 * hs;  // child c of segment s at c * hs + s
     int lv = 0;
-    unsigned long long* keys = a.keys;
+    K* const keys = reinterpret_cast<K*>(a.keys);
     // Key j of level segment s at keys[j * ks + s]: the 24 keys of one node lie
     // in 24 cache lines (atomics to one line are serialised in L2).
     const size_t ks = static_cast<size_t>(n / 2 + 1);
@@ -141,7 +163,7 @@ __global__ __launch_bounds__(kBuildThreads) void k_tlas_build(TlasBuild a) {
     // Root: all elements in index order; centroid bounds grown in index order
     // (bvh.cpp:370-376).
     if (tid == 0) {
-        for (int j = 0; j < 3; ++j) { keys[j * ks] = ~0ull; keys[(3 + j) * ks] = 0ull; }
+        for (int j = 0; j < 3; ++j) { keys[j * ks] = kNone; keys[(3 + j) * ks] = 0; }
         lds[L_NSEG] = 1;
         lds[L_NEXT] = 0;
     }
@@ -149,11 +171,11 @@ __global__ __launch_bounds__(kBuildThreads) void k_tlas_build(TlasBuild a) {
     __syncthreads();
     {
         // Reduced per thread, then over the wave; one atomic per wave and key.
-        unsigned long long q[6] = {~0ull, ~0ull, ~0ull, 0ull, 0ull, 0ull};
+        K q[6] = {kNone, kNone, kNone, 0, 0, 0};
         for (int p = tid; p < n; p += nthr) {
             const float* c = a.cen + 3 * static_cast<size_t>(p);
             for (int j = 0; j < 3; ++j) {
-                const unsigned long long kn = key_min(c[j], p), kx = key_max(c[j], p);
+                const K kn = Key<K>::lo(c[j], p), kx = Key<K>::hi(c[j], p);
                 q[j] = kn < q[j] ? kn : q[j];
                 q[3 + j] = kx > q[3 + j] ? kx : q[3 + j];
             }
@@ -162,7 +184,7 @@ __global__ __launch_bounds__(kBuildThreads) void k_tlas_build(TlasBuild a) {
         for (int d = 1; d < 64; d <<= 1) {
 #pragma unroll
             for (int j = 0; j < 3; ++j) {
-                const unsigned long long kn = __shfl_xor(q[j], d), kx = __shfl_xor(q[3 + j], d);
+                const K kn = __shfl_xor(q[j], d), kx = __shfl_xor(q[3 + j], d);
                 q[j] = kn < q[j] ? kn : q[j];
                 q[3 + j] = kx > q[3 + j] ? kx : q[3 + j];
             }
@@ -179,7 +201,7 @@ __global__ __launch_bounds__(kBuildThreads) void k_tlas_build(TlasBuild a) {
         sstart[0] = 0;
         scount[0] = n;
         snode[0] = a.base;
-        for (int j = 0; j < 6; ++j) scb[j] = key_value(keys[j * ks]);
+        for (int j = 0; j < 6; ++j) scb[j] = Key<K>::value(keys[j * ks]);
     }
     __syncthreads();
 
@@ -200,7 +222,7 @@ __global__ __launch_bounds__(kBuildThreads) void k_tlas_build(TlasBuild a) {
             sinfo[s] = (ext > 0.f ? 3 : 0) | (axis << 2) | (((scount[lv * hs + s] + sstart[lv * hs + s]) & 1) << 4);
             for (int j = 0; j < 24; ++j) {
                 const int r = j % 12;  // 0-2, 6-8 minima; 3-5, 9-11 maxima
-                keys[j * ks + s] = (r < 3 || (r >= 6 && r < 9)) ? ~0ull : 0ull;
+                keys[j * ks + s] = (r < 3 || (r >= 6 && r < 9)) ? kNone : 0;
             }
         }
         if (tid == 0) lds[L_CARRY] = 0;
@@ -340,18 +362,18 @@ __global__ __launch_bounds__(kBuildThreads) void k_tlas_build(TlasBuild a) {
                 const int lo = (c == 1 && mode != 1) ? cs : st;
                 const int hi = (c == 0 && mode != 2) ? cs : e;
                 // keys 0-2 box min, 3-5 box max, 6-8 centre min, 9-11 centre max
-                unsigned long long q[12];
+                K q[12];
 #pragma unroll
                 for (int j = 0; j < 3; ++j) {
-                    q[j] = mine ? key_min(v[j], rk[c]) : ~0ull;
-                    q[3 + j] = mine ? key_max(v[3 + j], rk[c]) : 0ull;
-                    q[6 + j] = mine ? key_min(v[6 + j], rk[c]) : ~0ull;
-                    q[9 + j] = mine ? key_max(v[6 + j], rk[c]) : 0ull;
+                    q[j] = mine ? Key<K>::lo(v[j], rk[c]) : kNone;
+                    q[3 + j] = mine ? Key<K>::hi(v[3 + j], rk[c]) : 0;
+                    q[6 + j] = mine ? Key<K>::lo(v[6 + j], rk[c]) : kNone;
+                    q[9 + j] = mine ? Key<K>::hi(v[6 + j], rk[c]) : 0;
                 }
                 for (int d = 1; d < 64; d <<= 1) {
                     const bool take = mine && lane >= d && p - d >= lo;
                     if (__ballot(take) == 0ull) break;
-                    unsigned long long t[12];
+                    K t[12];
 #pragma unroll
                     for (int j = 0; j < 12; ++j) t[j] = __shfl_up(q[j], d);
                     if (take) {
@@ -367,7 +389,7 @@ __global__ __launch_bounds__(kBuildThreads) void k_tlas_build(TlasBuild a) {
                 if (mine && (lane == 63 || p == hi - 1)) {
                     // When the whole child lies in this wave the run is the whole
                     // reduction and is stored as is.
-                    unsigned long long* k = keys + static_cast<size_t>(12 * c) * ks + s;
+                    K* k = keys + static_cast<size_t>(12 * c) * ks + s;
                     if (lo >= p - lane && p == hi - 1) {
 #pragma unroll
                         for (int j = 0; j < 12; ++j) k[j * ks] = q[j];
@@ -395,7 +417,7 @@ __global__ __launch_bounds__(kBuildThreads) void k_tlas_build(TlasBuild a) {
             int st = 0, cs = 0, gn = 0, nl = 0, nr = 0;
             float v[24];
             if (live) {
-                for (int j = 0; j < 24; ++j) v[j] = key_value(keys[j * ks + s]);
+                for (int j = 0; j < 24; ++j) v[j] = Key<K>::value(keys[j * ks + s]);
                 st = sstart[lv * hs + s];
                 cs = scsplit[s];
                 gn = snode[lv * hs + s];
@@ -459,6 +481,12 @@ __global__ __launch_bounds__(kBuildThreads) void k_tlas_build(TlasBuild a) {
         }
         __syncthreads();
     }
+}
+
+__global__ __launch_bounds__(kBuildThreads) void k_tlas_build(TlasBuild a) {
+    extern __shared__ int lds[];
+    if (a.negz[0]) tlas_build<unsigned long long>(a, lds);
+    else tlas_build<unsigned>(a, lds);
 }
 
 void launch_tlas_rebuild(hipStream_t s, const TlasBuild& a) {
