@@ -79,6 +79,7 @@ struct pt_renderer_s {
     int device = 0;
     hipStream_t stream = nullptr;
     int num_cu = 0;
+    int lds_bytes = 65536;  // LDS per workgroup (hipDeviceProp_t::sharedMemPerBlock)
     std::string err;
 
     // scene (GPU layout)
@@ -972,6 +973,7 @@ int pt_create(int32_t device, pt_handle* out) {
         hipDeviceProp_t prop;
         HIPCHK(hipGetDeviceProperties(&prop, device));
         h->num_cu = prop.multiProcessorCount;
+        h->lds_bytes = static_cast<int>(prop.sharedMemPerBlock);
         HIPCHK(hipEventCreate(&h->ev_r0));
         HIPCHK(hipEventCreate(&h->ev_r1));
         pt_default_options(&h->opts);
@@ -1239,6 +1241,7 @@ int pt_rebuild_instances(pt_handle h, const float* transforms, int32_t num_insta
         a.seg = h->t_seg.p;
         a.keys = h->t_keys.p;
         a.negz = h->t_pre.p + nn + 1;
+        a.lds_bytes = h->lds_bytes;
         HIPCHK(hipMemsetAsync(a.negz, 0, sizeof(int), s));
         ptk::launch_tlas_rebuild(s, a);
         HIPCHK(hipGetLastError());

@@ -155,9 +155,11 @@ __device__ __forceinline__ void tlas_build(const TlasBuild& a, int* lds) {
     int* const sm = sw + 16 * hs;
     int* const schild = sw + 17 * hs;  // child c of segment s at c * hs + s
     int lv = 0;
-    K* const keys = reinterpret_cast<K*>(a.keys);
-    // Key j of level segment s at keys[j * ks + s]: the 24 keys of one node lie
-    // in 24 cache lines (atomics to one line are serialised in L2).
+    // 32-bit keys after the segment table when the launcher reserved the LDS.
+    K* const keys = (sizeof(K) == 4 && a.key_lds) ? reinterpret_cast<K*>(sw + kTlasSegWords * hs)
+                                                  : reinterpret_cast<K*>(a.keys);
+    // Key j of level segment s at keys[j * ks + s]: the runs of one wave reduce
+    // into neighbouring slots. In HBM, the keys of one node lie in 24 cache lines.
     const size_t ks = static_cast<size_t>(n / 2 + 1);
 
     // Root: all elements in index order; centroid bounds grown in index order
@@ -492,8 +494,20 @@ __global__ __launch_bounds__(kBuildThreads) void k_tlas_build(TlasBuild a) {
 void launch_tlas_rebuild(hipStream_t s, const TlasBuild& a) {
     hipLaunchKernelGGL(k_tlas_prep, dim3((a.n + 255) / 256), dim3(256), 0, s, a);
     if (a.n >= 2) {
-        const int words = kBuildLdsWords + (a.n <= kLdsElems ? 6 * a.n + 1 + kTlasSegWords * (a.n / 2 + 1) : 0);
-        hipLaunchKernelGGL(k_tlas_build, dim3(1), dim3(kBuildThreads), words * sizeof(int), s, a);
+        const int hs = a.n / 2 + 1;
+        TlasBuild b = a;
+        b.key_lds = 0;
+        int words = kBuildLdsWords;
+        if (a.n <= kLdsElems) {
+            words += 6 * a.n + 1 + kTlasSegWords * hs;
+            // The 32-bit keys (24 per segment) too, where the workgroup's LDS allows:
+            // 112,896 bytes in all at 1024 instances.
+            if (static_cast<size_t>(words + 24 * hs) * sizeof(int) <= static_cast<size_t>(a.lds_bytes)) {
+                words += 24 * hs;
+                b.key_lds = 1;
+            }
+        }
+        hipLaunchKernelGGL(k_tlas_build, dim3(1), dim3(kBuildThreads), words * sizeof(int), s, b);
     }
 }
 
