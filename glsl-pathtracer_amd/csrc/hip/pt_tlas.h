@@ -82,7 +82,8 @@ struct TlasBuild {
     unsigned long long* keys;  // [(n / 2 + 1) * 24] child box reduction keys per segment
     int* negz;              // [1] set when a world box or centre coordinate is -0.0
     int lds_bytes;          // LDS one workgroup may use on this device
-    int key_lds;            // set by the launcher: 32-bit keys kept in LDS
+    int arr_lds;            // set by the launcher: order, prefix, pair and segment arrays in LDS
+    int key_lds;            // set by the launcher: 32-bit keys kept in LDS too
 };
 
 }  // namespace ptd

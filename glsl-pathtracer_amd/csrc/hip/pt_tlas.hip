@@ -46,11 +46,6 @@ constexpr int kBuildThreads = 1024;
 // LDS words of the build kernel.
 constexpr int L_CARRY = 0, L_NSEG = 1, L_NEXT = 2, L_WTOT = 4;  // L_WTOT: one word per wave
 constexpr int kBuildLdsWords = L_WTOT + kBuildThreads / 64;
-// Up to this many instances the per-position arrays (two orders, segment of
-// each position, the two swap-position tables, the side-flag prefix count:
-// 6n + 1 words) and the segment table live in LDS after the words above
-// (63,648 bytes at 1024); beyond it in global memory.
-constexpr int kLdsElems = 1024;
 
 // std::min / std::max of the reference's Vec3::Min / Max (Vec3.h:119-135):
 // the first argument wins ties.
@@ -133,7 +128,7 @@ __device__ __forceinline__ void tlas_build(const TlasBuild& a, int* lds) {
     const int n = a.n;
     const int tid = threadIdx.x, nthr = blockDim.x;
     const int lane = tid & 63, wave = tid >> 6;
-    const bool in_lds = n <= kLdsElems;
+    const bool in_lds = a.arr_lds != 0;
     int* const arr = lds + kBuildLdsWords;
     int* ord = in_lds ? arr : a.ord0;
     int* ord2 = in_lds ? arr + n : a.ord1;
@@ -494,15 +489,21 @@ __global__ __launch_bounds__(kBuildThreads) void k_tlas_build(TlasBuild a) {
 void launch_tlas_rebuild(hipStream_t s, const TlasBuild& a) {
     hipLaunchKernelGGL(k_tlas_prep, dim3((a.n + 255) / 256), dim3(256), 0, s, a);
     if (a.n >= 2) {
-        const int hs = a.n / 2 + 1;
+        // The per-position arrays (two orders, segment of each position, the two
+        // swap-position tables, the side-flag prefix count: 6n + 1 words) and the
+        // segment table live in LDS after the words above where the workgroup's
+        // LDS holds them, and the 32-bit keys (24 per segment) too where it holds
+        // those as well: on gfx950 (160 KB) up to 2640 and 1487 instances.
+        const size_t cap = static_cast<size_t>(a.lds_bytes) / sizeof(int);
+        const size_t hs = static_cast<size_t>(a.n) / 2 + 1;
+        const size_t arr = 6 * static_cast<size_t>(a.n) + 1 + kTlasSegWords * hs;
         TlasBuild b = a;
-        b.key_lds = 0;
-        int words = kBuildLdsWords;
-        if (a.n <= kLdsElems) {
-            words += 6 * a.n + 1 + kTlasSegWords * hs;
-            // The 32-bit keys (24 per segment) too, where the workgroup's LDS allows:
-            // 112,896 bytes in all at 1024 instances.
-            if (static_cast<size_t>(words + 24 * hs) * sizeof(int) <= static_cast<size_t>(a.lds_bytes)) {
+        b.arr_lds = b.key_lds = 0;
+        size_t words = kBuildLdsWords;
+        if (words + arr <= cap) {
+            words += arr;
+            b.arr_lds = 1;
+            if (words + 24 * hs <= cap) {
                 words += 24 * hs;
                 b.key_lds = 1;
             }

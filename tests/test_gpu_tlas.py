@@ -62,7 +62,7 @@ def test_rebuild_equals_host(renderer, case):
     check_case(renderer, case, SIZES[case], 7)
 
 
-@pytest.mark.parametrize("n", [1, 2, 3, 5, 64, 1024, 1025, 4097])
+@pytest.mark.parametrize("n", [1, 2, 3, 5, 64, 1024, 1025, 1500, 4097])
 def test_rebuild_sizes(renderer, n):
     print(f"random n={n}", flush=True)
     check_case(renderer, "random", n, 11)
