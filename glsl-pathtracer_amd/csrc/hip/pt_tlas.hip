@@ -121,14 +121,16 @@ __global__ void k_tlas_prep(TlasBuild a) {
     a.inst[i] = g;
 }
 
-// Level-synchronous top-down build in one workgroup (n >= 2), K: key type.
-template <typename K>
+// Level-synchronous top-down build in one workgroup (n >= 2). K: key type;
+// ArrLds, KeyLds: the arrays, and the keys, in LDS (fixed per instantiation, so
+// their accesses compile to LDS instructions).
+template <typename K, bool ArrLds, bool KeyLds>
 __device__ __forceinline__ void tlas_build(const TlasBuild& a, int* lds) {
     constexpr K kNone = static_cast<K>(~static_cast<K>(0));
     const int n = a.n;
     const int tid = threadIdx.x, nthr = blockDim.x;
     const int lane = tid & 63, wave = tid >> 6;
-    const bool in_lds = a.arr_lds != 0;
+    constexpr bool in_lds = ArrLds;
     int* const arr = lds + kBuildLdsWords;
     int* ord = in_lds ? arr : a.ord0;
     int* ord2 = in_lds ? arr + n : a.ord1;
@@ -151,8 +153,7 @@ __device__ __forceinline__ void tlas_build(const TlasBuild& a, int* lds) {
     int* const schild = sw + 17 * hs;  // child c of segment s at c * hs + s
     int lv = 0;
     // 32-bit keys after the segment table when the launcher reserved the LDS.
-    K* const keys = (sizeof(K) == 4 && a.key_lds) ? reinterpret_cast<K*>(sw + kTlasSegWords * hs)
-                                                  : reinterpret_cast<K*>(a.keys);
+    K* const keys = KeyLds ? reinterpret_cast<K*>(sw + kTlasSegWords * hs) : reinterpret_cast<K*>(a.keys);
     // Key j of level segment s at keys[j * ks + s]: the runs of one wave reduce
     // into neighbouring slots. In HBM, the keys of one node lie in 24 cache lines.
     const size_t ks = static_cast<size_t>(n / 2 + 1);
@@ -482,8 +483,18 @@ __device__ __forceinline__ void tlas_build(const TlasBuild& a, int* lds) {
 
 __global__ __launch_bounds__(kBuildThreads) void k_tlas_build(TlasBuild a) {
     extern __shared__ int lds[];
-    if (a.negz[0]) tlas_build<unsigned long long>(a, lds);
-    else tlas_build<unsigned>(a, lds);
+    using u64 = unsigned long long;
+    if (a.negz[0]) {
+        // The LDS reserved for 32-bit keys does not hold 64-bit ones.
+        if (a.arr_lds) tlas_build<u64, true, false>(a, lds);
+        else tlas_build<u64, false, false>(a, lds);
+    } else if (a.key_lds) {
+        tlas_build<unsigned, true, true>(a, lds);
+    } else if (a.arr_lds) {
+        tlas_build<unsigned, true, false>(a, lds);
+    } else {
+        tlas_build<unsigned, false, false>(a, lds);
+    }
 }
 
 void launch_tlas_rebuild(hipStream_t s, const TlasBuild& a) {
