@@ -502,8 +502,8 @@ void launch_tlas_rebuild(hipStream_t s, const TlasBuild& a) {
     if (a.n >= 2) {
         // The per-position arrays (two orders, segment of each position, the two
         // swap-position tables, the side-flag prefix count: 6n + 1 words) and the
-        // segment table live in LDS after the words above where the workgroup's
-        // LDS holds them, and the 32-bit keys (24 per segment) too where it holds
+        // segment table live in LDS after its first kBuildLdsWords words where the
+        // workgroup's LDS holds them, and the 32-bit keys (24 per segment) too where it holds
         // those as well: on gfx950 (160 KB) up to 2640 and 1487 instances.
         const size_t cap = static_cast<size_t>(a.lds_bytes) / sizeof(int);
         const size_t hs = static_cast<size_t>(a.n) / 2 + 1;
