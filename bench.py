@@ -25,8 +25,9 @@ differentiable-materials workload (256 boxes + 8 noise-displaced spheres,
 199,072 triangles, 64 Disney materials, 512x512, 16 spp, max depth 4): forward
 = one pt_render of the 16 samples, backward = one pt_adjoint of the same
 samples for an L2 loss against a target rendered from a second parameter seed
-(both device time between HIP events, median of --adjoint-reps). Measured only
-with --adjoint: the library is built with pt_adjoint switched off.
+(both device time between HIP events, median of --adjoint-reps). The backward
+pass is the whole pt_adjoint call: a primal pass (the forward render, recording
+NEE visibility) and the replay pass that accumulates the gradients.
 
 batch: the batched form on 32 uneven Cornell-box jobs (pt_batch_render,
 graph-captured passes), reported beside the headline.
@@ -159,7 +160,9 @@ def adjoint_ratio(reps, wave_spp):
         loss = float(np.mean(diff.astype(np.float64) ** 2))
         dl = (2.0 / diff.size) * diff
         g = r.adjoint(0, rs.spp, dl.astype(np.float32))
-        b_ms = r.stats()["ms_last_render"]
+        st = r.stats()
+        b_ms = st["ms_last_render"]
+        dropped = st["adjoint_dropped"]
         if i > 0:  # the first round is warm-up
             fwd.append(f_ms)
             bwd.append(b_ms)
@@ -169,7 +172,7 @@ def adjoint_ratio(reps, wave_spp):
                         f"spheres), {a['materials'].shape[0]} Disney materials x 7 parameters, "
                         f"{rs.width}x{rs.height}, {rs.spp} spp, max depth {rs.max_depth}",
             "forward_ms": f_med, "backward_ms": b_med, "backward_over_forward": b_med / f_med,
-            "reps": reps, "loss": loss, "grad_norm": float(np.linalg.norm(g)),
+            "reps": reps, "loss": loss, "grad_norm": float(np.linalg.norm(g)), "dropped_vertices": dropped,
             "launch": {"wave_spp": wave_spp}}
 
 
@@ -349,8 +352,7 @@ def main():
     ap.add_argument("--no-count", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--adjoint-reps", type=int, default=5)
-    ap.add_argument("--adjoint", action="store_true",
-                    help="time pt_adjoint (needs a library built with -DPT_ENABLE_ADJOINT)")
+    ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-heavy", action="store_true")
     ap.add_argument("--heavy-steps", type=int, default=2)
@@ -416,9 +418,7 @@ def main():
                     "stack_spills": cw["stack_spills"]}
     r.close()
 
-    adjoint = adjoint_ratio(args.adjoint_reps, 16) if args.adjoint else {
-        "status": "not measured", "reason": "pt_adjoint is switched off in this build: its replay kernels "
-                                            "faulted on gfx950 and the cause is not yet found"}
+    adjoint = None if args.no_adjoint else adjoint_ratio(args.adjoint_reps, 16)
     batch = None if args.no_batch else batch_throughput(3)
     heavy = None if args.no_heavy else heavy_throughput(args.heavy_steps, 1, count=not args.no_count)
 

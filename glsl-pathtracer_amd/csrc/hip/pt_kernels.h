@@ -23,19 +23,28 @@ struct PathBuf {
 // Shadow (NEE visibility) queue: one record per path with up to two rays
 // (k = 0 environment, 1 analytic light) from the same scatter point.
 struct ShadowBuf {
-    float4* o;  // [n]    origin, path slot (int bits)
-    float4* t;  // [n]    throughput at NEE time, mask of valid rays (int bits)
+    int2* idx;  // [n]    path slot, mask of valid rays (bit k: ray k)
+    float4* o;  // [n]    origin (w unused)
+    float4* t;  // [n]    throughput at NEE time (w unused)
     float4* d;  // [2n]   ray k: direction, max distance
     float4* c;  // [2n]   ray k: contribution without the throughput
 };
 
-// Adjoint (replay pass) inputs and outputs.
-struct AdjBuf {
+// Visibility record of the adjoint's primal pass: per path slot and bounce,
+// bit k of the bounce's two bits says NEE ray k (0 environment, 1 analytic
+// light) was valid and unoccluded. 16 bounces per 32-bit word; word array w
+// (max_paths words) covers bounces [16 w, 16 w + 16), at bit 2 (b mod 16).
+constexpr int VIS_BOUNCES_PER_WORD = 16;
+
+// Replay pass of the adjoint: inputs, the primal's records and the outputs.
+struct ReplayBuf {
     const float4* ltot;   // primal radiance of every path slot of the pass
+    const unsigned* vis;  // primal NEE visibility bits of the current bounce's word (pt_replay.h)
     const float* dimg;    // dLoss / d(mean image), RGB per pixel, rows bottom-up
     float inv_n;          // 1 / number of samples in the mean image
-    float* grad;          // [num_mats * 7] gradient sums
-    float4* sg;           // per shadow record and ray k: 2 float4 at 4*pos + 2*k = 7 gradient terms, material id (int bits)
+    float* grad;          // [num_mats * NPARAM] gradient sums
+    int* err;             // set to 1 when a vertex names a material outside [0, num_mats)
+    int* dropped;         // vertices whose gradient had a non-finite term
     int num_mats;
     int lds;              // 1: accumulate per block in LDS
 };
@@ -68,11 +77,11 @@ struct BatchBuf {
 }  // namespace ptd
 
 namespace ptk {
-enum { KIND_PLAIN = 0, KIND_EXTEND = 1, KIND_SHADE = 2, KIND_SHADOW = 3, KIND_SHADE_ADJ = 4, KIND_SHADOW_ADJ = 5,
-       KIND_SHADE_BATCH = 6 };
+enum { KIND_PLAIN = 0, KIND_EXTEND = 1, KIND_SHADE = 2, KIND_SHADOW = 3, KIND_SHADE_BATCH = 4,
+       KIND_SHADE_INLINE = 5, KIND_SHADE_BATCH_INLINE = 6, KIND_REPLAY = 7 };
 
 // Resident blocks for a persistent launch (occupancy x CUs); extra_lds is the
-// adjoint's per-block gradient table.
+// replay stage's per-block gradient table.
 int persistent_blocks(int kind, int bs, int S, bool count, int num_cu, size_t extra_lds = 0);
 void launch_plain(hipStream_t s, int blocks, int bs, const ptd::DevScene& sc, const ptd::DevOpts& op, float4* accum,
                   int sample, int num_items, int* work, int S, int* spill, unsigned long long* counters, int* ovf,
@@ -81,13 +90,25 @@ void launch_extend(hipStream_t s, int blocks, int bs, const ptd::DevScene& sc, c
                    const ptd::PathBuf& pb, bool primary, const int* qin, const int* count_in, int npaths, int npix,
                    int first_sample, int* hitq, int* hit_count, int* work, int S, int* spill,
                    unsigned long long* counters, int* ovf, bool count);
+// Shade stage; bb: the batched form's pass state (null: one frame); inline_nee:
+// NEE visibility resolved in the stage (stochastic alpha, volume MIS), else
+// queued as shadow records. S and spill: the traversal stack (inline_nee).
 void launch_shade(hipStream_t s, int blocks, int bs, const ptd::DevScene& sc, const ptd::DevOpts& op,
-                  const ptd::PathBuf& pb, const ptd::ShadowBuf& sb, bool primary, const int* hitq, const int* hit_count,
-                  int npix, int first_sample, int* qout, int* count_out, int* shadow_count, int* work, bool compact,
-                  const ptd::AdjBuf* ad = nullptr);
+                  const ptd::PathBuf& pb, const ptd::ShadowBuf& sb, const ptd::BatchBuf* bb, bool primary,
+                  bool inline_nee, const int* hitq, const int* hit_count, int npix, int first_sample, int* qout,
+                  int* count_out, int* shadow_count, int* work, bool compact, int S, int* spill,
+                  unsigned long long* counters, int* ovf);
+// Shadow stage; vis (the adjoint's primal pass, else null): the bounce's word
+// array of visibility bits, set at bit vshift + k for visible ray k.
 void launch_shadow(hipStream_t s, int blocks, int bs, const ptd::DevScene& sc, const ptd::DevOpts& op,
                    const ptd::PathBuf& pb, const ptd::ShadowBuf& sb, const int* count_in, int* work, int S,
-                   int* spill, unsigned long long* counters, int* ovf, bool count, const ptd::AdjBuf* ad = nullptr);
+                   int* spill, unsigned long long* counters, int* ovf, bool count, unsigned* vis = nullptr,
+                   int vshift = 0);
+// Replay stage of the adjoint (extend is the forward renderer's).
+void launch_replay_shade(hipStream_t s, int blocks, int bs, const ptd::DevScene& sc, const ptd::DevOpts& op,
+                         const ptd::PathBuf& pb, bool primary, const int* hitq, const int* hit_count, int npix,
+                         int first_sample, int* qout, int* count_out, int* work, bool compact,
+                         const ptd::ReplayBuf& rb, int vshift);
 void launch_accum(hipStream_t s, const ptd::DevOpts& op, const ptd::PathBuf& pb, float4* accum, int npix, int nsamp);
 // batched render: the stage kernels of one pass, parameterised by device-side pass state
 void launch_pass_begin(hipStream_t s, const ptd::BatchBuf& bb, int* counters, int ncounters);
@@ -95,10 +116,6 @@ void launch_pass_end(hipStream_t s, const ptd::BatchBuf& bb);
 void launch_extend_batch(hipStream_t s, int blocks, int bs, const ptd::DevScene& sc, const ptd::DevOpts& op,
                          const ptd::PathBuf& pb, const ptd::BatchBuf& bb, int* hitq, int* hit_count, int* work, int S,
                          int* spill, int* ovf);
-void launch_shade_batch(hipStream_t s, int blocks, int bs, const ptd::DevScene& sc, const ptd::DevOpts& op,
-                        const ptd::PathBuf& pb, const ptd::ShadowBuf& sb, const ptd::BatchBuf& bb, bool primary,
-                        const int* hitq, const int* hit_count, int* qout, int* count_out, int* shadow_count, int* work,
-                        bool compact);
 void launch_accum_batch(hipStream_t s, int blocks, const ptd::PathBuf& pb, const ptd::BatchBuf& bb);
 // Environment-map CDF and total luminance of an RGB32F image of n texels
 // (ok: one int of scratch; scratch: env_cdf_scratch_bytes(n) bytes, 8-byte aligned).

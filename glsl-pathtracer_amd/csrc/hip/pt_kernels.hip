@@ -1,12 +1,13 @@
 // Kernels of the path: plain one-launch-per-frame megakernel, wavefront
-// stages, ray queries, accumulation and tonemapping. gfx950, wave64.
+// stages, ray queries, accumulation and tonemapping. gfx950, wave64. The
+// per-path functions they share are in pt_path.h.
 //
 // Reference behaviour (file:line in knightcrawler25/GLSL-PathTracer):
-//   ray generation      src/shaders/tile.glsl:41-75
 //   bounce loop         src/shaders/common/pathtrace.glsl:285-476
-//   DirectLight (NEE)   src/shaders/common/pathtrace.glsl:158-283
 //   tonemap             src/shaders/tonemap.glsl:59-133
 #include "pt_kernels.h"
+#include "pt_path.h"
+#include "pt_replay.h"
 
 namespace ptd {
 
@@ -57,400 +58,6 @@ __device__ __forceinline__ void item_pixel(int i, int width, int& px, int& py) {
     const int tile = i >> 6, l = i & 63;
     px = (tile % tiles_x) * 8 + (l & 7);
     py = (tile / tiles_x) * 8 + (l >> 3);
-}
-
-// Camera ray of tile.glsl:41-69 for pixel (px, py).
-__device__ void camera_ray(const DevOpts& op, int px, int py, Rng& rng, vec3& ro, vec3& rd) {
-    const float resx = static_cast<float>(op.width), resy = static_cast<float>(op.height);
-    float cx = (static_cast<float>(px) + 0.5f) / resx;
-    float cy = (static_cast<float>(py) + 0.5f) / resy;
-    float r1 = 2.0f * rnd(rng);
-    float r2 = 2.0f * rnd(rng);
-    float jx = r1 < 1.0f ? sqrtf(r1) - 1.0f : 1.0f - sqrtf(2.0f - r1);
-    float jy = r2 < 1.0f ? sqrtf(r2) - 1.0f : 1.0f - sqrtf(2.0f - r2);
-    jx = jx / (resx * 0.5f);
-    jy = jy / (resy * 0.5f);
-    float dx = (cx * 2.0f - 1.0f) + jx;
-    float dy = (cy * 2.0f - 1.0f) + jy;
-    float scale = tanf(op.cam_fov * 0.5f);
-    dy = dy * (resy / resx * scale);
-    dx = dx * scale;
-    vec3 right = V(op.cam_right[0], op.cam_right[1], op.cam_right[2]);
-    vec3 up = V(op.cam_up[0], op.cam_up[1], op.cam_up[2]);
-    vec3 fwd = V(op.cam_fwd[0], op.cam_fwd[1], op.cam_fwd[2]);
-    vec3 rayDir = normalize(dx * right + dy * up + fwd);
-    vec3 focalPoint = op.cam_focal * rayDir;
-    float cam_r1 = rnd(rng) * TWO_PI;
-    float cam_r2 = rnd(rng) * op.cam_aperture;
-    vec3 ap = (cosf(cam_r1) * right + sinf(cam_r1) * up) * sqrtf(cam_r2);
-    rd = normalize(focalPoint - ap);
-    ro = V(op.cam_pos[0], op.cam_pos[1], op.cam_pos[2]) + ap;
-}
-
-// AnyHit (anyhit.glsl:25-215): lights, then the BVH; with OPT_ALPHA_TEST and
-// no media, alpha-tested candidates are skipped (BLEND draws from `rng`).
-template <bool COUNT>
-__device__ __forceinline__ bool occluded(const DevScene& sc, const DevOpts& op, vec3 ro, vec3 rd, float maxDist,
-                                         Stack& stk, TravCount& cnt, bool& ovf, Rng* rng = nullptr) {
-    if ((op.flags & OPT_LIGHTS) && any_light<COUNT>(sc, ro, rd, maxDist, cnt)) return true;
-    Hit h;
-    const bool alpha = (op.flags & OPT_ALPHA_TEST) && !(op.flags & OPT_MEDIUM);
-    return traverse<true, COUNT>(sc, ro, rd, maxDist, stk, h, cnt, ovf, alpha, rng);
-}
-
-template <bool COUNT>
-__device__ __forceinline__ Hit closest(const DevScene& sc, const DevOpts& op, vec3 ro, vec3 rd, int depth, Stack& stk,
-                                       TravCount& cnt, bool& ovf) {
-    Hit h;
-    h.t = INF; h.u = 0.f; h.v = 0.f; h.tri = -1; h.inst = -1; h.light = -1;
-    if ((op.flags & OPT_LIGHTS) && (!(op.flags & OPT_HIDE_EMITTERS) || depth > 0)) hit_lights<COUNT>(sc, ro, rd, h, cnt);
-    traverse<false, COUNT>(sc, ro, rd, h.t, stk, h, cnt, ovf);
-    return h;
-}
-
-// EvalTransmittance (pathtrace.glsl:118-156): product of medium
-// transmittances along a shadow ray, zero when an opaque surface blocks it.
-template <bool COUNT>
-__device__ vec3 eval_transmittance(const DevScene& sc, const DevOpts& op, vec3 ro, vec3 rd, Rng& rng, Stack& stk,
-                                   TravCount& cnt, bool& ovf) {
-    vec3 tr = V1(1.0f);
-    SurfState st;
-    st.matID = 0; st.tu = 0.f; st.tv = 0.f; st.normal = V1(0.f); st.ffnormal = V1(0.f);
-    st.tangent = V1(0.f); st.bitangent = V1(0.f); st.fhp = V1(0.f);
-    float prev_rough = 0.f;
-    for (int depth = 0; depth < op.max_depth; depth++) {
-        Hit h = closest<COUNT>(sc, op, ro, rd, 0, stk, cnt, ovf);
-        bool hit = h.t != INF;
-        if (!hit || h.tri < 0) break;
-        surface_attributes(sc, h, ro, rd, st);
-        float eta;
-        Material m = GetMaterial(sc, op, st, rd, 0, prev_rough, eta);
-        prev_rough = m.roughness;
-        bool alphatest = (m.alphaMode == ALPHA_MODE_MASK && m.opacity < m.alphaCutoff) ||
-                         (m.alphaMode == ALPHA_MODE_BLEND && rnd(rng) > m.opacity);
-        bool refractive = (1.0f - m.metallic) * m.specTrans > 0.0f;
-        if (hit && !(alphatest || refractive)) return V1(0.0f);
-        if (dot(rd, st.normal) > 0.0f && m.medium.type != MEDIUM_NONE) {
-            vec3 color = m.medium.type == MEDIUM_ABSORB ? V1(1.0f) - m.medium.color : V1(1.0f);
-            tr = tr * vexp(-color * m.medium.density * h.t);
-        }
-        ro = st.fhp + rd * EPS;
-    }
-    return tr;
-}
-
-// Per-path state carried between bounces (the locals of PathTrace).
-struct PathRegs {
-    vec3 ro, rd, thr, rad;
-    float alpha, prev_pdf, prev_rough;
-    int depth, pass, prev_mat;
-    bool in_medium, surface_scatter;
-    float tu, tv;
-    Medium med;
-    Rng rng;
-};
-
-// Result of DirectLight's sampling step; the visibility test is separate so
-// the wavefront form can defer it to the shadow stage.
-struct NeeRay { vec3 o, d, contrib, w; float maxd; bool valid, tested; };
-
-// The sampling and BSDF part of DirectLight for one light kind. kind 0 =
-// environment (SampleEnvMap), 1 = analytic light. RNG draws happen in the
-// reference order; contrib excludes the final `* throughput`; w is contrib
-// without the BSDF value (the factor the adjoint multiplies df/dparam by).
-// `tested`: the reference runs AnyHit for this ray (always for the
-// environment, for a light when it faces the point) even where the BSDF
-// gives no contribution, which matters when AnyHit draws random numbers.
-__device__ NeeRay nee_setup(const DevScene& sc, const DevOpts& op, int kind, vec3 rd_in, const SurfState& st,
-                            const Material& m, float eta, Rng& rng) {
-    NeeRay r;
-    r.valid = false;
-    r.tested = kind == 0;
-    r.contrib = V1(0.0f);
-    r.w = V1(0.0f);
-    vec3 scatterPos = st.fhp + st.normal * EPS;
-    r.o = scatterPos;
-    if (kind == 0) {
-        vec3 Li;
-        float4 dp = SampleEnvMap(sc, op, rng, Li);
-        vec3 lightDir = xyz(dp);
-        float lightPdf = dp.w;
-        r.d = lightDir;
-        r.maxd = INF - EPS;
-        float pdf;
-        vec3 f = DisneyEval(m, eta, -rd_in, st.ffnormal, lightDir, pdf);
-        if (pdf > 0.0f) {
-            float mis = PowerHeuristic(lightPdf, pdf);
-            if (mis > 0.0f) {
-                r.contrib = mis * Li * f * op.env_intensity / lightPdf;
-                r.w = mis * Li * op.env_intensity / lightPdf;
-                r.valid = true;
-            }
-        }
-        return r;
-    }
-    int index = static_cast<int>(rnd(rng) * static_cast<float>(sc.num_lights));
-    if (index > sc.num_lights - 1) index = sc.num_lights - 1;
-    const GLight L = sc.lights[index];
-    LightSample ls;
-    SampleOneLight(L, scatterPos, sc.num_lights, rng, ls);
-    vec3 Li = ls.emission;
-    r.d = ls.direction;
-    r.maxd = ls.dist - EPS;
-    if (dot(ls.direction, ls.normal) < 0.0f) {
-        r.tested = true;
-        float pdf;
-        vec3 f = DisneyEval(m, eta, -rd_in, st.ffnormal, ls.direction, pdf);
-        float mis = 1.0f;
-        if (L.e.w > 0.0f) mis = PowerHeuristic(ls.pdf, pdf);
-        if (pdf > 0.0f) {
-            r.contrib = mis * Li * f / ls.pdf;
-            r.w = mis * Li / ls.pdf;
-            r.valid = true;
-        }
-    }
-    return r;
-}
-
-// DirectLight with the volume-MIS variant (transmittance instead of a binary
-// test); used by the plain kernel only.
-template <bool COUNT>
-__device__ vec3 direct_light_volmis(const DevScene& sc, const DevOpts& op, vec3 rd_in, const SurfState& st,
-                                    const Material& m, float eta, const Medium& med, bool isSurface, Rng& rng,
-                                    Stack& stk, TravCount& cnt, bool& ovf) {
-    vec3 Ld = V1(0.0f);
-    vec3 scatterPos = st.fhp + st.normal * EPS;
-    if ((op.flags & OPT_ENVMAP) && !(op.flags & OPT_UNIFORM_LIGHT)) {
-        vec3 Li;
-        float4 dp = SampleEnvMap(sc, op, rng, Li);
-        vec3 lightDir = xyz(dp);
-        float lightPdf = dp.w;
-        Li = Li * eval_transmittance<COUNT>(sc, op, scatterPos, lightDir, rng, stk, cnt, ovf);
-        vec3 f;
-        float pdf;
-        if (isSurface) f = DisneyEval(m, eta, -rd_in, st.ffnormal, lightDir, pdf);
-        else { float p = PhaseHG(dot(-rd_in, lightDir), med.anisotropy); f = V1(p); pdf = p; }
-        if (pdf > 0.0f) {
-            float mis = PowerHeuristic(lightPdf, pdf);
-            if (mis > 0.0f) Ld = Ld + mis * Li * f * op.env_intensity / lightPdf;
-        }
-    }
-    if (op.flags & OPT_LIGHTS) {
-        int index = static_cast<int>(rnd(rng) * static_cast<float>(sc.num_lights));
-        if (index > sc.num_lights - 1) index = sc.num_lights - 1;
-        const GLight L = sc.lights[index];
-        LightSample ls;
-        SampleOneLight(L, scatterPos, sc.num_lights, rng, ls);
-        vec3 Li = ls.emission;
-        if (dot(ls.direction, ls.normal) < 0.0f) {
-            Li = Li * eval_transmittance<COUNT>(sc, op, scatterPos, ls.direction, rng, stk, cnt, ovf);
-            vec3 f;
-            float pdf;
-            if (isSurface) f = DisneyEval(m, eta, -rd_in, st.ffnormal, ls.direction, pdf);
-            else { float p = PhaseHG(dot(-rd_in, ls.direction), med.anisotropy); f = V1(p); pdf = p; }
-            float mis = 1.0f;
-            if (L.e.w > 0.0f) mis = PowerHeuristic(ls.pdf, pdf);
-            if (pdf > 0.0f) Ld = Ld + mis * f * Li / ls.pdf;
-        }
-    }
-    return Ld;
-}
-
-// DirectLight with binary visibility (pathtrace.glsl:158-283, non-VOL_MIS).
-// With stochastic alpha in AnyHit every ray the reference tests is traced,
-// so that the path's random-number stream advances as the reference's does;
-// otherwise only rays that can contribute are.
-template <bool COUNT>
-__device__ vec3 direct_light(const DevScene& sc, const DevOpts& op, vec3 rd_in, const SurfState& st,
-                             const Material& m, float eta, Rng& rng, Stack& stk, TravCount& cnt, bool& ovf) {
-    vec3 Ld = V1(0.0f);
-    const bool rng_in_anyhit = (op.flags & OPT_ALPHA_TEST) && !(op.flags & OPT_MEDIUM);
-    for (int kind = 0; kind < 2; ++kind) {
-        if (kind == 0 && !((op.flags & OPT_ENVMAP) && !(op.flags & OPT_UNIFORM_LIGHT))) continue;
-        if (kind == 1 && !(op.flags & OPT_LIGHTS)) continue;
-        NeeRay r = nee_setup(sc, op, kind, rd_in, st, m, eta, rng);
-        if (r.valid || (rng_in_anyhit && r.tested)) {
-            const bool occ = occluded<COUNT>(sc, op, r.o, r.d, r.maxd, stk, cnt, ovf, &rng);
-            if (r.valid && !occ) Ld = Ld + r.contrib;
-        }
-    }
-    return Ld;
-}
-
-// Event of one bounce after the closest hit.
-enum BounceEvent { EV_END = 0, EV_SURFACE = 1, EV_PASS = 2, EV_MEDIUM = 3 };
-
-// Miss branch of PathTrace (pathtrace.glsl:305-339).
-__device__ void on_miss(const DevScene& sc, const DevOpts& op, PathRegs& p) {
-    if ((op.flags & (OPT_BACKGROUND | OPT_TRANSPARENT_BACKGROUND)) && p.depth == 0) p.alpha = 0.0f;
-    if (!(op.flags & OPT_HIDE_EMITTERS) || p.depth > 0) {
-        if (op.flags & OPT_UNIFORM_LIGHT) {
-            p.rad = p.rad + V(op.ucol_x, op.ucol_y, op.ucol_z) * p.thr;
-        } else if (op.flags & OPT_ENVMAP) {
-            float4 e = EvalEnvMap(sc, op, p.rd);
-            float mis = 1.0f;
-            if (p.depth > 0) mis = PowerHeuristic(p.prev_pdf, e.w);
-            if ((op.flags & OPT_MEDIUM) && !(op.flags & OPT_VOL_MIS) && !p.surface_scatter) mis = 1.0f;
-            if (mis > 0.0f) p.rad = p.rad + mis * xyz(e) * p.thr * op.env_intensity;
-        }
-    }
-}
-
-// Bounce body from the hit up to next-event estimation (pathtrace.glsl:341-425):
-// material, emission, emitter MIS, depth limit, medium events, alpha test.
-// `st` keeps the previous bounce's attributes when a light is hit, as the
-// shader's State does. Returns the event that decides what follows.
-__device__ int bounce_pre(const DevScene& sc, const DevOpts& op, const Hit& h, PathRegs& p, SurfState& st,
-                          Material& m, float& eta) {
-    const bool is_emitter = h.tri < 0;
-    float light_pdf = 0.f;
-    vec3 light_em = V1(0.f);
-    if (is_emitter) {
-        light_pdf = light_hit_pdf(sc, h.light, p.ro, p.rd, h.t);
-        light_em = xyz(sc.lights[h.light].e);
-        st.fhp = p.ro + p.rd * h.t;
-        st.matID = p.prev_mat;
-        st.tu = p.tu; st.tv = p.tv;
-    } else {
-        surface_attributes(sc, h, p.ro, p.rd, st);
-    }
-    m = GetMaterial(sc, op, st, p.rd, p.depth, p.prev_rough, eta);
-    p.prev_rough = m.roughness;
-    p.prev_mat = st.matID;
-    p.tu = st.tu; p.tv = st.tv;
-    p.rad = p.rad + m.emission * p.thr;
-    if ((op.flags & OPT_LIGHTS) && is_emitter) {
-        float mis = 1.0f;
-        if (p.depth > 0) mis = PowerHeuristic(p.prev_pdf, light_pdf);
-        if ((op.flags & OPT_MEDIUM) && !(op.flags & OPT_VOL_MIS) && !p.surface_scatter) mis = 1.0f;
-        p.rad = p.rad + mis * light_em * p.thr;
-        return EV_END;
-    }
-    if (p.depth == op.max_depth) return EV_END;
-    if (op.flags & OPT_MEDIUM) {
-        p.surface_scatter = false;
-        if (p.in_medium) {
-            if (p.med.type == MEDIUM_ABSORB) {
-                p.thr = p.thr * vexp(-(V1(1.0f) - p.med.color) * h.t * p.med.density);
-            } else if (p.med.type == MEDIUM_EMISSIVE) {
-                p.rad = p.rad + p.med.color * h.t * p.med.density * p.thr;
-            } else {
-                float scatterDist = gmin(-logf(rnd(p.rng)) / p.med.density, h.t);
-                if (scatterDist < h.t) {
-                    p.thr = p.thr * p.med.color;
-                    p.ro = p.ro + p.rd * scatterDist;
-                    st.fhp = p.ro;
-                    return EV_MEDIUM;
-                }
-            }
-        }
-    }
-    if (op.flags & OPT_ALPHA_TEST) {
-        bool passed = (m.alphaMode == ALPHA_MODE_MASK && m.opacity < m.alphaCutoff) ||
-                      (m.alphaMode == ALPHA_MODE_BLEND && rnd(p.rng) > m.opacity);
-        if (passed) {
-            p.depth--;
-            p.pass++;
-            return EV_PASS;
-        }
-    }
-    p.surface_scatter = true;
-    return EV_SURFACE;
-}
-
-// BSDF sample of a SURFACE event, reported to the adjoint.
-struct BsdfOut { vec3 f, L; float pdf; };
-
-// Bounce body after NEE: BSDF or phase sampling, medium bookkeeping, Russian
-// roulette (pathtrace.glsl:392-471). Returns false when the path ends.
-__device__ bool bounce_post(const DevOpts& op, PathRegs& p, const SurfState& st, const Material& m, float eta, int ev,
-                            BsdfOut* bo = nullptr) {
-    if (ev == EV_MEDIUM) {
-        float ra = rnd(p.rng);
-        float rb = rnd(p.rng);
-        vec3 scatterDir = SampleHG(-p.rd, p.med.anisotropy, ra, rb);
-        p.prev_pdf = PhaseHG(dot(-p.rd, scatterDir), p.med.anisotropy);
-        p.rd = scatterDir;
-    } else {
-        vec3 L;
-        if (ev == EV_PASS) {
-            if (p.pass > 1024) return false;
-            L = p.rd;
-        } else {
-            float pdf;
-            vec3 f = DisneySample(m, eta, -p.rd, st.ffnormal, p.rng, L, pdf);
-            if (bo) { bo->f = f; bo->L = L; bo->pdf = pdf; }
-            p.prev_pdf = pdf;
-            if (pdf > 0.0f) p.thr = p.thr * (f / pdf);
-            else return false;
-        }
-        p.rd = L;
-        p.ro = st.fhp + p.rd * EPS;
-        if (op.flags & OPT_MEDIUM) {
-            if (dot(p.rd, st.normal) < 0.0f && m.medium.type != MEDIUM_NONE) {
-                p.in_medium = true;
-                p.med = m.medium;
-            } else if (m.medium.type != MEDIUM_NONE) {
-                p.in_medium = false;
-            }
-        }
-    }
-    if ((op.flags & OPT_RR) && p.depth >= op.rr_depth) {
-        float q = gmin(gmax(p.thr.x, gmax(p.thr.y, p.thr.z)) + 0.001f, 0.95f);
-        if (rnd(p.rng) > q) return false;
-        p.thr = p.thr / q;
-    }
-    return true;
-}
-
-__device__ __forceinline__ void init_path(PathRegs& p) {
-    p.thr = V1(1.0f); p.rad = V1(0.0f);
-    p.alpha = 1.0f; p.prev_pdf = 0.0f; p.prev_rough = 0.0f;
-    p.depth = 0; p.pass = 0; p.prev_mat = 0;
-    p.in_medium = false; p.surface_scatter = false;
-    p.tu = 0.f; p.tv = 0.f;
-    p.med.type = 0; p.med.density = 0.f; p.med.color = V1(0.f); p.med.anisotropy = 0.f;
-}
-
-__device__ __forceinline__ void init_surf(SurfState& st) {
-    st.fhp = V1(0.f); st.normal = V1(0.f); st.ffnormal = V1(0.f); st.tangent = V1(0.f); st.bitangent = V1(0.f);
-    st.tu = 0.f; st.tv = 0.f; st.matID = 0;
-}
-
-// NEE rays of one SURFACE or MEDIUM event in the reference's draw order
-// (environment first, then one analytic light).
-__device__ __forceinline__ void nee_rays(const DevScene& sc, const DevOpts& op, const PathRegs& p0, Rng& rng,
-                                         const SurfState& st, const Material& m, float eta, NeeRay nee[2]) {
-    nee[0].valid = nee[1].valid = false;
-    if ((op.flags & OPT_ENVMAP) && !(op.flags & OPT_UNIFORM_LIGHT)) nee[0] = nee_setup(sc, op, 0, p0.rd, st, m, eta, rng);
-    if (op.flags & OPT_LIGHTS) nee[1] = nee_setup(sc, op, 1, p0.rd, st, m, eta, rng);
-}
-
-// Full PathTrace for one sample (megakernel form).
-template <bool COUNT>
-__device__ float4 path_trace(const DevScene& sc, const DevOpts& op, PathRegs& p, Stack& stk, TravCount& cnt,
-                             bool& ovf) {
-    SurfState st;
-    init_surf(st);
-    for (;; p.depth++) {
-        Hit h = closest<COUNT>(sc, op, p.ro, p.rd, p.depth, stk, cnt, ovf);
-        if (h.t == INF) { on_miss(sc, op, p); break; }
-        Material m;
-        float eta;
-        const int ev = bounce_pre(sc, op, h, p, st, m, eta);
-        if (ev == EV_END) break;
-        if (ev != EV_PASS) {
-            vec3 Ld = V1(0.0f);
-            if ((op.flags & OPT_MEDIUM) && (op.flags & OPT_VOL_MIS)) {
-                Ld = direct_light_volmis<COUNT>(sc, op, p.rd, st, m, eta, p.med, ev == EV_SURFACE, p.rng, stk, cnt, ovf);
-            } else {
-                Ld = direct_light<COUNT>(sc, op, p.rd, st, m, eta, p.rng, stk, cnt, ovf);
-            }
-            p.rad = p.rad + Ld * p.thr;
-        }
-        if (!bounce_post(op, p, st, m, eta, ev)) break;
-    }
-    return make_float4(p.rad.x, p.rad.y, p.rad.z, p.alpha);
 }
 
 // ------------------------------------------------------------------ plain kernel
@@ -636,92 +243,27 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))
     if (ovf) atomicOr(ovf_flag, 1);
 }
 
-// Gradient accumulators of one block: per-material parameter sums in LDS
-// (flushed with one global atomic per non-zero entry at the end), or direct
-// global atomics when the material table does not fit.
-struct GradAcc {
-    float* lds;   // null: global atomics
-    float* glob;
-    int n;        // num_mats * NPARAM
-    __device__ __forceinline__ void init() {
-        if (!lds) return;
-        for (int i = threadIdx.x; i < n; i += blockDim.x) lds[i] = 0.0f;
-        __syncthreads();
-    }
-    __device__ __forceinline__ void add(int mat, const float g[NPARAM]) {
-        if (lds) {
-#pragma unroll
-            for (int k = 0; k < NPARAM; ++k)
-                if (g[k] != 0.0f)
-                    __hip_atomic_fetch_add(lds + mat * NPARAM + k, g[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        } else {
-#pragma unroll
-            for (int k = 0; k < NPARAM; ++k)
-                if (g[k] != 0.0f) atomicAdd(glob + mat * NPARAM + k, g[k]);
-        }
-    }
-    __device__ __forceinline__ void flush() {
-        if (!lds) return;
-        __syncthreads();
-        for (int i = threadIdx.x; i < n; i += blockDim.x)
-            if (lds[i] != 0.0f) atomicAdd(glob + i, lds[i]);
-    }
-};
-
-// Path-replay adjoint of one SURFACE vertex (detached sampling):
-//   BSDF-sample term  sum_c delta_c * Lrem_c / f_c * df_c/dparam, where Lrem is
-//     the primal radiance still to come after this vertex's emission,
-//     assuming the NEE rays are occluded;
-//   per NEE ray (applied by the shadow stage if unoccluded)
-//     sum_c delta_c * thr_c * w_c * df_c(light dir)/dparam
-//     - sum_c delta_c * C_c / f_c * df_c/dparam   (C: the ray's radiance),
-// so that together the terms use the exact remaining radiance.
-__device__ __forceinline__ void adjoint_vertex(const DevScene& sc, const DevOpts& op, const AdjBuf& ad, int slot,
-                                               int npix, const SurfState& st, const Material& m, vec3 rd_in,
-                                               vec3 rad_pre, const BsdfOut& bo, const NeeRay nee[2], vec3 thr_nee,
-                                               float g[NPARAM], float gn[2][NPARAM]) {
-    int s, px, py;
-    slot_pixel(op, slot, npix, s, px, py);
-    const float* dp = ad.dimg + (static_cast<size_t>(py) * op.width + px) * 3;
-    const vec3 delta = V(dp[0], dp[1], dp[2]) * ad.inv_n;
-    const MatD md = seed_material(sc, st, m, rd_in);
-    const vec3 Vw = -rd_in;
-    const bool bsdf_ok = bo.pdf > 0.0f;
-    vec3 r = V1(0.0f);
-    dvec3 df = D3s(dc(0.0f));
-    if (bsdf_ok) {
-        r = V(bo.f.x != 0.0f ? delta.x / bo.f.x : 0.0f, bo.f.y != 0.0f ? delta.y / bo.f.y : 0.0f,
-              bo.f.z != 0.0f ? delta.z / bo.f.z : 0.0f);
-        const vec3 lrem = xyz(ad.ltot[slot]) - rad_pre;
-        df = DisneyEvalD(m, md, Vw, st.ffnormal, bo.L);
-        grad_dot(r * lrem, df, g);
-    }
-    for (int k = 0; k < 2; ++k) {
-        if (!nee[k].valid) continue;
-        const dvec3 dl = DisneyEvalD(m, md, Vw, st.ffnormal, nee[k].d);
-        grad_dot(delta * thr_nee * nee[k].w, dl, gn[k]);
-        if (bsdf_ok) {
-            float t[NPARAM];
-            grad_dot(r * (nee[k].contrib * thr_nee), df, t);
-#pragma unroll
-            for (int j = 0; j < NPARAM; ++j) gn[k][j] = gn[k][j] - t[j];
-        }
-    }
-}
-
-// Shade: one bounce of every path that hit something, except the NEE
-// visibility tests, which go to the shadow queue. Continuing paths are
+// Shade: one bounce of every path that hit something. The NEE rays of the
+// vertex go to the shadow queue as one record; continuing paths are
 // compacted into the next extend queue; ending paths write only radiance.
-// ADJ: the replay pass of the adjoint; same path decisions, plus parameter
-// gradients of every surface vertex.
-template <bool PRIMARY, bool ADJ, bool BATCH>
+// INLINE_NEE: the vertex's NEE visibility is resolved here instead, with
+// DirectLight's own traversals in the reference's order (pathtrace.glsl:158-283).
+// Scenes need it when AnyHit draws from the path's random numbers between
+// NEE sampling and BSDF sampling (stochastic BLEND alpha, anyhit.glsl:118-141)
+// or when visibility is a transmittance (volume MIS, pathtrace.glsl:118-156);
+// no shadow records are written then. The LDS holds the traversal stack.
+template <bool PRIMARY, bool BATCH, bool INLINE_NEE>
 __global__ void __launch_bounds__(256) k_shade(DevScene sc, DevOpts op0, PathBuf pb, ShadowBuf sb,
                                                const int* __restrict__ hitq, const int* hit_count, int npix,
                                                int first_sample, int* qout, int* count_out, int* shadow_count,
-                                               int* work, int compact, AdjBuf ad, BatchBuf bb) {
+                                               int* work, int compact, BatchBuf bb, int S, int* spill,
+                                               unsigned long long* counters, int* ovf_flag) {
     extern __shared__ int lds[];
-    GradAcc acc{ADJ && ad.lds ? reinterpret_cast<float*>(lds) : nullptr, ad.grad, ad.num_mats * NPARAM};
-    if (ADJ) acc.init();
+    Stack stk{lds, spill, static_cast<int>(threadIdx.x), static_cast<int>(blockDim.x), S,
+              blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x,
+              static_cast<size_t>(gridDim.x) * blockDim.x};
+    TravCount cnt{0, 0, 0, 0, 0, 0};
+    bool ovf = false;
     const int count = *hit_count;
     for (int base = first_batch(); base < count; base = next_batch(work)) {
         const int i = base + lane_id();
@@ -730,12 +272,6 @@ __global__ void __launch_bounds__(256) k_shade(DevScene sc, DevOpts op0, PathBuf
         NeeRay nee[2];
         nee[0].valid = nee[1].valid = false;
         vec3 thr_nee = V1(0.f);
-        int gmat = -1;
-        float gn[2][NPARAM];
-        if (ADJ) {
-#pragma unroll
-            for (int k = 0; k < NPARAM; ++k) gn[0][k] = gn[1][k] = 0.0f;
-        }
         if (i < count) {
             slot = hitq[i];
             PathRegs p;
@@ -767,21 +303,20 @@ __global__ void __launch_bounds__(256) k_shade(DevScene sc, DevOpts op0, PathBuf
             const int ev = bounce_pre(sc, op, h, p, st, m, eta);
             if (ev != EV_END) {
                 if (ev != EV_PASS) {
-                    nee_rays(sc, op, p, p.rng, st, m, eta, nee);
-                    thr_nee = p.thr;
+                    if (INLINE_NEE) {
+                        vec3 Ld;
+                        if ((op.flags & OPT_MEDIUM) && (op.flags & OPT_VOL_MIS))
+                            Ld = direct_light_volmis<false>(sc, op, p.rd, st, m, eta, p.med, ev == EV_SURFACE, p.rng,
+                                                            stk, cnt, ovf);
+                        else
+                            Ld = direct_light<false>(sc, op, p.rd, st, m, eta, p.rng, stk, cnt, ovf);
+                        p.rad = p.rad + Ld * p.thr;
+                    } else {
+                        nee_rays(sc, op, p, p.rng, st, m, eta, nee);
+                        thr_nee = p.thr;
+                    }
                 }
-                const vec3 rd_in = p.rd;
-                BsdfOut bo;
-                bo.pdf = 0.0f;
-                alive = bounce_post(op, p, st, m, eta, ev, ADJ ? &bo : nullptr);
-                if (ADJ && ev == EV_SURFACE) {
-                    float g[NPARAM];
-#pragma unroll
-                    for (int k = 0; k < NPARAM; ++k) g[k] = 0.0f;
-                    adjoint_vertex(sc, op, ad, slot, npix, st, m, rd_in, p.rad, bo, nee, thr_nee, g, gn);
-                    gmat = st.matID;
-                    acc.add(gmat, g);
-                }
+                alive = bounce_post(op, p, st, m, eta, ev);
                 if (alive) p.depth++;
             }
             if (alive) {
@@ -793,21 +328,21 @@ __global__ void __launch_bounds__(256) k_shade(DevScene sc, DevOpts op0, PathBuf
             }
             keep = alive;
         }
-        // NEE rays of this vertex (environment, light): one shadow record per
-        // path holding both, so one thread adds their visible contributions in
-        // DirectLight's order (no two threads update a path's radiance).
-        const int mask = (nee[0].valid ? 1 : 0) | (nee[1].valid ? 2 : 0);
-        const int pos = wave_append(mask != 0, shadow_count);
-        if (pos >= 0) {
-            sb.o[pos] = F4(nee[(mask & 1) ? 0 : 1].o, __int_as_float(slot));  // both start at the scatter point
-            sb.t[pos] = F4(thr_nee, __int_as_float(mask));
-            for (int k = 0; k < 2; ++k) {
-                if (!((mask >> k) & 1)) continue;
-                sb.d[2 * pos + k] = F4(nee[k].d, nee[k].maxd);
-                sb.c[2 * pos + k] = F4(nee[k].contrib, 0.f);
-                if (ADJ) {
-                    ad.sg[4 * pos + 2 * k] = make_float4(gn[k][0], gn[k][1], gn[k][2], gn[k][3]);
-                    ad.sg[4 * pos + 2 * k + 1] = make_float4(gn[k][4], gn[k][5], gn[k][6], __int_as_float(gmat));
+        if (!INLINE_NEE) {
+            // NEE rays of this vertex (environment, light): one shadow record
+            // per path holding both, so one thread adds their visible
+            // contributions in DirectLight's order (no two threads update a
+            // path's radiance).
+            const int mask = (nee[0].valid ? 1 : 0) | (nee[1].valid ? 2 : 0);
+            const int pos = wave_append(mask != 0, shadow_count);
+            if (pos >= 0) {
+                sb.idx[pos] = make_int2(slot, mask);
+                sb.o[pos] = F4(nee[(mask & 1) ? 0 : 1].o, 0.f);  // both rays start at the scatter point
+                sb.t[pos] = F4(thr_nee, 0.f);
+                for (int k = 0; k < 2; ++k) {
+                    if (!((mask >> k) & 1)) continue;
+                    sb.d[2 * pos + k] = F4(nee[k].d, nee[k].maxd);
+                    sb.c[2 * pos + k] = F4(nee[k].contrib, 0.f);
                 }
             }
         }
@@ -816,60 +351,128 @@ __global__ void __launch_bounds__(256) k_shade(DevScene sc, DevOpts op0, PathBuf
             if (pos >= 0) qout[pos] = slot;
         }
     }
-    if (ADJ) acc.flush();
+    if (INLINE_NEE) {
+        flush_counts<false>(cnt, counters);
+        if (ovf) atomicOr(ovf_flag, 1);
+    }
 }
 
 // Shadow: any-hit tests of the queued NEE rays of a path; the visible
 // contributions are summed as DirectLight does and added to the path's
-// radiance times the throughput at NEE time (ADJ: and their parameter
-// gradients). Compiled for 8 waves per SIMD (64 VGPRs, a few spilled):
-// measured 9% faster than the unconstrained 5 waves on the 1080p workload.
-template <bool COUNT, bool ADJ>
+// radiance times the throughput at NEE time. `vis` (the adjoint's primal
+// pass, else null): the word array of this bounce's visibility bits, where
+// bits vshift + k are set for the visible rays k of the record's path slot.
+// Compiled for 8 waves per SIMD (64 VGPRs, a few spilled): measured 9% faster
+// than the unconstrained 5 waves on the 1080p workload.
+template <bool COUNT>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) k_shadow(DevScene sc, DevOpts op, PathBuf pb, ShadowBuf sb, const int* count_in,
                                                 int* work, int S, int* spill, unsigned long long* counters,
-                                                int* ovf_flag, AdjBuf ad) {
+                                                int* ovf_flag, unsigned* vis, int vshift) {
     extern __shared__ int lds[];
     Stack stk{lds, spill, static_cast<int>(threadIdx.x), static_cast<int>(blockDim.x), S,
               blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x,
               static_cast<size_t>(gridDim.x) * blockDim.x};
-    GradAcc acc{ADJ && ad.lds ? reinterpret_cast<float*>(lds + S * blockDim.x) : nullptr, ad.grad,
-                ad.num_mats * NPARAM};
-    if (ADJ) acc.init();
     TravCount cnt{0, 0, 0, 0, 0, 0};
     bool ovf = false;
     const int count = *count_in;
     for (int base = first_batch(); base < count; base = next_batch(work)) {
         const int i = base + lane_id();
         if (i < count) {
-            const float4 o4 = sb.o[i], t4 = sb.t[i];
-            const int mask = __float_as_int(t4.w);
+            const int2 id = sb.idx[i];
+            const float4 o4 = sb.o[i];
             vec3 Ld = V1(0.0f);
-            bool lit = false;
+            int lit = 0;
             for (int k = 0; k < 2; ++k) {
-                if (!((mask >> k) & 1)) continue;
+                if (!((id.y >> k) & 1)) continue;
                 const float4 d4 = sb.d[2 * i + k];
                 if (occluded<COUNT>(sc, op, xyz(o4), xyz(d4), d4.w, stk, cnt, ovf)) continue;
                 Ld = Ld + xyz(sb.c[2 * i + k]);
-                lit = true;
-                if (ADJ) {
-                    const float4 g0 = ad.sg[4 * i + 2 * k], g1 = ad.sg[4 * i + 2 * k + 1];
-                    const int mat = __float_as_int(g1.w);
-                    if (mat >= 0) {
-                        const float g[NPARAM] = {g0.x, g0.y, g0.z, g0.w, g1.x, g1.y, g1.z};
-                        acc.add(mat, g);
-                    }
-                }
+                lit |= 1 << k;
             }
             if (lit) {
-                const int slot = __float_as_int(o4.w);
-                const float4 r = pb.rad[slot];
-                pb.rad[slot] = F4(xyz(r) + Ld * xyz(t4), r.w);
+                const float4 r = pb.rad[id.x];
+                pb.rad[id.x] = F4(xyz(r) + Ld * xyz(sb.t[i]), r.w);
+                if (vis) vis[id.x] |= static_cast<unsigned>(lit) << vshift;
             }
         }
     }
     flush_counts<COUNT>(cnt, counters);
     if (ovf) atomicOr(ovf_flag, 1);
-    if (ADJ) acc.flush();
+}
+
+// Replay stage of the path-replay adjoint: one bounce of every path that hit
+// something, with the NEE visibility of the primal pass (ReplayBuf.vis bits
+// at vshift), so the path advances exactly as in the primal (pt_replay.h);
+// each SURFACE vertex adds its material-parameter gradient. Per-block sums in
+// LDS (num_mats * NPARAM floats when ReplayBuf.lds), flushed with one global
+// atomic per non-zero entry; otherwise global atomics. Material ids outside
+// [0, num_mats) set ReplayBuf.err; vertices with a non-finite gradient term
+// are dropped and counted in ReplayBuf.dropped.
+template <bool PRIMARY>
+__global__ void __launch_bounds__(256) k_replay_shade(DevScene sc, DevOpts op, PathBuf pb,
+                                                      const int* __restrict__ hitq, const int* hit_count, int npix,
+                                                      int first_sample, int* qout, int* count_out, int* work,
+                                                      int compact, ReplayBuf rb, int vshift) {
+    extern __shared__ float gacc[];
+    const int ng = rb.num_mats * NPARAM;
+    if (rb.lds) {
+        for (int j = threadIdx.x; j < ng; j += blockDim.x) gacc[j] = 0.0f;
+        __syncthreads();
+    }
+    const int count = *hit_count;
+    for (int base = first_batch(); base < count; base = next_batch(work)) {
+        const int i = base + lane_id();
+        int slot = -1;
+        bool keep = false;
+        if (i < count) {
+            slot = hitq[i];
+            int s, px, py;
+            slot_pixel(op, slot, npix, s, px, py);
+            PathRegs p;
+            if (PRIMARY) primary_path(op, px, py, first_sample + s, p);
+            else load_path(pb, slot, p);
+            const float4 h4 = pb.hit[slot];
+            const int2 h2 = pb.hit2[slot];
+            Hit h;
+            h.t = h4.x; h.u = h4.y; h.v = h4.z; h.tri = __float_as_int(h4.w); h.inst = h2.x; h.light = h2.y;
+            const float* dp = rb.dimg + 3 * (static_cast<size_t>(py) * op.width + px);
+            const vec3 delta = V(dp[0], dp[1], dp[2]) * rb.inv_n;
+            const int vis = static_cast<int>((rb.vis[slot] >> vshift) & 3u);
+            float g[NPARAM];
+            const ReplayVertex rv = replay_vertex(sc, op, h, p, vis, delta, xyz(rb.ltot[slot]), g);
+            if (rv.mat >= 0) {
+                if (rv.mat >= rb.num_mats) {
+                    atomicOr(rb.err, 1);
+                } else if (!rv.finite) {
+                    atomicAdd(rb.dropped, 1);
+                } else {
+                    const int g0 = rv.mat * NPARAM;
+#pragma unroll
+                    for (int k = 0; k < NPARAM; ++k) {
+                        if (g[k] == 0.0f) continue;
+                        if (rb.lds) __hip_atomic_fetch_add(&gacc[g0 + k], g[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                        else atomicAdd(rb.grad + g0 + k, g[k]);
+                    }
+                }
+            }
+            if (rv.alive) {
+                store_path(pb, slot, p);
+            } else {
+                pb.rad[slot] = F4(p.rad, p.alpha);
+                if (!compact) pb.meta[slot].y = PATH_DEAD;
+            }
+            keep = rv.alive;
+        }
+        if (compact) {
+            const int pos = wave_append(keep, count_out);
+            if (pos >= 0) qout[pos] = slot;
+        }
+    }
+    if (rb.lds) {
+        __syncthreads();
+        for (int j = threadIdx.x; j < ng; j += blockDim.x)
+            if (gacc[j] != 0.0f) atomicAdd(rb.grad + j, gacc[j]);
+    }
 }
 
 // Accumulate: accum[pixel] += radiance of samples s = 0..nsamp-1 in order.
@@ -1500,11 +1103,12 @@ int persistent_blocks(int kind, int bs, int S, bool count, int num_cu, size_t ex
     switch (kind) {
         case KIND_PLAIN: per_cu = count ? max_blocks(k_plain<true>, bs, lds) : max_blocks(k_plain<false>, bs, lds); break;
         case KIND_EXTEND: per_cu = count ? max_blocks(k_extend<true, false, false>, bs, lds) : max_blocks(k_extend<false, false, false>, bs, lds); break;
-        case KIND_SHADOW: per_cu = count ? max_blocks(k_shadow<true, false>, bs, lds) : max_blocks(k_shadow<false, false>, bs, lds); break;
+        case KIND_SHADOW: per_cu = count ? max_blocks(k_shadow<true>, bs, lds) : max_blocks(k_shadow<false>, bs, lds); break;
         case KIND_SHADE: per_cu = max_blocks(k_shade<false, false, false>, bs, 0); break;
-        case KIND_SHADE_ADJ: per_cu = max_blocks(k_shade<false, true, false>, bs, extra_lds); break;
-        case KIND_SHADE_BATCH: per_cu = max_blocks(k_shade<false, false, true>, bs, 0); break;
-        case KIND_SHADOW_ADJ: per_cu = max_blocks(k_shadow<false, true>, bs, lds + extra_lds); break;
+        case KIND_SHADE_BATCH: per_cu = max_blocks(k_shade<false, true, false>, bs, 0); break;
+        case KIND_SHADE_INLINE: per_cu = max_blocks(k_shade<false, false, true>, bs, lds); break;
+        case KIND_SHADE_BATCH_INLINE: per_cu = max_blocks(k_shade<false, true, true>, bs, lds); break;
+        case KIND_REPLAY: per_cu = max_blocks(k_replay_shade<false>, bs, extra_lds); break;
         default: break;
     }
     return per_cu * num_cu;
@@ -1530,19 +1134,30 @@ void launch_extend(hipStream_t s, int blocks, int bs, const DevScene& sc, const 
 }
 
 void launch_shade(hipStream_t s, int blocks, int bs, const DevScene& sc, const DevOpts& op, const PathBuf& pb,
-                  const ShadowBuf& sb, bool primary, const int* hitq, const int* hit_count, int npix, int first_sample,
-                  int* qout, int* count_out, int* shadow_count, int* work, bool compact, const AdjBuf* ad) {
+                  const ShadowBuf& sb, const BatchBuf* bb, bool primary, bool inline_nee, const int* hitq,
+                  const int* hit_count, int npix, int first_sample, int* qout, int* count_out, int* shadow_count,
+                  int* work, bool compact, int S, int* spill, unsigned long long* counters, int* ovf) {
     const int c = compact ? 1 : 0;
-    const BatchBuf nb{};
-    if (ad) {
-        const size_t lds = ad->lds ? static_cast<size_t>(ad->num_mats) * NPARAM * sizeof(float) : 0;
-        if (primary) hipLaunchKernelGGL((k_shade<true, true, false>), dim3(blocks), dim3(bs), lds, s, sc, op, pb, sb, hitq, hit_count, npix, first_sample, qout, count_out, shadow_count, work, c, *ad, nb);
-        else hipLaunchKernelGGL((k_shade<false, true, false>), dim3(blocks), dim3(bs), lds, s, sc, op, pb, sb, hitq, hit_count, npix, first_sample, qout, count_out, shadow_count, work, c, *ad, nb);
-        return;
+    const BatchBuf b = bb ? *bb : BatchBuf{};
+    const size_t lds = inline_nee ? static_cast<size_t>(S) * bs * sizeof(int) : 0;
+#define PT_SHADE(P, B, I) hipLaunchKernelGGL((k_shade<P, B, I>), dim3(blocks), dim3(bs), lds, s, sc, op, pb, sb, hitq, hit_count, npix, first_sample, qout, count_out, shadow_count, work, c, b, S, spill, counters, ovf)
+    if (bb) {
+        if (inline_nee) { if (primary) PT_SHADE(true, true, true); else PT_SHADE(false, true, true); }
+        else { if (primary) PT_SHADE(true, true, false); else PT_SHADE(false, true, false); }
+    } else {
+        if (inline_nee) { if (primary) PT_SHADE(true, false, true); else PT_SHADE(false, false, true); }
+        else { if (primary) PT_SHADE(true, false, false); else PT_SHADE(false, false, false); }
     }
-    const AdjBuf none{};
-    if (primary) hipLaunchKernelGGL((k_shade<true, false, false>), dim3(blocks), dim3(bs), 0, s, sc, op, pb, sb, hitq, hit_count, npix, first_sample, qout, count_out, shadow_count, work, c, none, nb);
-    else hipLaunchKernelGGL((k_shade<false, false, false>), dim3(blocks), dim3(bs), 0, s, sc, op, pb, sb, hitq, hit_count, npix, first_sample, qout, count_out, shadow_count, work, c, none, nb);
+#undef PT_SHADE
+}
+
+void launch_replay_shade(hipStream_t s, int blocks, int bs, const DevScene& sc, const DevOpts& op, const PathBuf& pb,
+                         bool primary, const int* hitq, const int* hit_count, int npix, int first_sample, int* qout,
+                         int* count_out, int* work, bool compact, const ReplayBuf& rb, int vshift) {
+    const int c = compact ? 1 : 0;
+    const size_t lds = rb.lds ? static_cast<size_t>(rb.num_mats) * NPARAM * sizeof(float) : 0;
+    if (primary) hipLaunchKernelGGL((k_replay_shade<true>), dim3(blocks), dim3(bs), lds, s, sc, op, pb, hitq, hit_count, npix, first_sample, qout, count_out, work, c, rb, vshift);
+    else hipLaunchKernelGGL((k_replay_shade<false>), dim3(blocks), dim3(bs), lds, s, sc, op, pb, hitq, hit_count, npix, first_sample, qout, count_out, work, c, rb, vshift);
 }
 
 void launch_pass_begin(hipStream_t s, const BatchBuf& bb, int* counters, int ncounters) {
@@ -1559,32 +1174,16 @@ void launch_extend_batch(hipStream_t s, int blocks, int bs, const DevScene& sc, 
     hipLaunchKernelGGL((k_extend<false, true, true>), dim3(blocks), dim3(bs), lds, s, sc, op, pb, nullptr, nullptr, 0, 0, 0, hitq, hit_count, work, S, spill, nullptr, ovf, bb);
 }
 
-void launch_shade_batch(hipStream_t s, int blocks, int bs, const DevScene& sc, const DevOpts& op, const PathBuf& pb,
-                        const ShadowBuf& sb, const BatchBuf& bb, bool primary, const int* hitq, const int* hit_count,
-                        int* qout, int* count_out, int* shadow_count, int* work, bool compact) {
-    const int c = compact ? 1 : 0;
-    const AdjBuf none{};
-    if (primary) hipLaunchKernelGGL((k_shade<true, false, true>), dim3(blocks), dim3(bs), 0, s, sc, op, pb, sb, hitq, hit_count, 0, 0, qout, count_out, shadow_count, work, c, none, bb);
-    else hipLaunchKernelGGL((k_shade<false, false, true>), dim3(blocks), dim3(bs), 0, s, sc, op, pb, sb, hitq, hit_count, 0, 0, qout, count_out, shadow_count, work, c, none, bb);
-}
-
 void launch_accum_batch(hipStream_t s, int blocks, const PathBuf& pb, const BatchBuf& bb) {
     hipLaunchKernelGGL(k_accum_batch, dim3(blocks), dim3(256), 0, s, pb, bb);
 }
 
 void launch_shadow(hipStream_t s, int blocks, int bs, const DevScene& sc, const DevOpts& op, const PathBuf& pb,
                    const ShadowBuf& sb, const int* count_in, int* work, int S, int* spill,
-                   unsigned long long* counters, int* ovf, bool count, const AdjBuf* ad) {
+                   unsigned long long* counters, int* ovf, bool count, unsigned* vis, int vshift) {
     const size_t lds = static_cast<size_t>(S) * bs * sizeof(int);
-    if (ad) {
-        const size_t extra = ad->lds ? static_cast<size_t>(ad->num_mats) * NPARAM * sizeof(float) : 0;
-        if (count) hipLaunchKernelGGL((k_shadow<true, true>), dim3(blocks), dim3(bs), lds + extra, s, sc, op, pb, sb, count_in, work, S, spill, counters, ovf, *ad);
-        else hipLaunchKernelGGL((k_shadow<false, true>), dim3(blocks), dim3(bs), lds + extra, s, sc, op, pb, sb, count_in, work, S, spill, counters, ovf, *ad);
-        return;
-    }
-    const AdjBuf none{};
-    if (count) hipLaunchKernelGGL((k_shadow<true, false>), dim3(blocks), dim3(bs), lds, s, sc, op, pb, sb, count_in, work, S, spill, counters, ovf, none);
-    else hipLaunchKernelGGL((k_shadow<false, false>), dim3(blocks), dim3(bs), lds, s, sc, op, pb, sb, count_in, work, S, spill, counters, ovf, none);
+    if (count) hipLaunchKernelGGL((k_shadow<true>), dim3(blocks), dim3(bs), lds, s, sc, op, pb, sb, count_in, work, S, spill, counters, ovf, vis, vshift);
+    else hipLaunchKernelGGL((k_shadow<false>), dim3(blocks), dim3(bs), lds, s, sc, op, pb, sb, count_in, work, S, spill, counters, ovf, vis, vshift);
 }
 
 void launch_accum(hipStream_t s, const DevOpts& op, const PathBuf& pb, float4* accum, int npix, int nsamp) {

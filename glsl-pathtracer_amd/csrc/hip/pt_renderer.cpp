@@ -20,6 +20,7 @@
 
 #include "../../../include/glslpt.h"
 #include "pt_kernels.h"
+#include "pt_layout.h"
 #include "pt_tlas.h"
 
 using ptd::DevOpts;
@@ -28,6 +29,13 @@ using ptd::GInst;
 using ptd::GLight;
 using ptd::GNode;
 using ptd::GTri;
+using ptl::ArgError;
+using ptl::make_inst;
+using ptl::make_light;
+using ptl::relayout;
+using ptl::relayout_tlas;
+using ptl::Relayout;
+using ptl::TlasRelayout;
 
 namespace {
 
@@ -36,13 +44,6 @@ struct HipError : std::exception {
     explicit HipError(std::string m) : msg(std::move(m)) {}
     const char* what() const noexcept override { return msg.c_str(); }
 };
-struct ArgError : std::exception {
-    std::string msg;
-    int code;
-    ArgError(int c, std::string m) : msg(std::move(m)), code(c) {}
-    const char* what() const noexcept override { return msg.c_str(); }
-};
-
 #define HIPCHK(x)                                                                           \
     do {                                                                                    \
         hipError_t e_ = (x);                                                                \
@@ -125,9 +126,12 @@ struct pt_renderer_s {
     DevBuf<uint4> p_rng;
     DevBuf<int4> p_meta;
     DevBuf<int2> p_hit2;
+    DevBuf<int2> s_idx;
     DevBuf<float4> s_o, s_t, s_d, s_c;
-    DevBuf<float4> a_rad, a_sg;   // adjoint: replay radiance, shadow-ray gradient records
+    DevBuf<float4> a_rad;         // adjoint: replay radiance
+    DevBuf<unsigned> a_vis;       // adjoint: primal NEE visibility bits
     DevBuf<float> a_dimg, a_grad; // adjoint: dLoss/dimage, gradient sums
+    DevBuf<int> a_flags;          // adjoint: [0] material index error, [1] dropped vertices
     // batched render: job / plane / segment / pass tables, pass state, images
     DevBuf<ptd::JobDev> b_jobs;
     DevBuf<ptd::PlaneDev> b_planes;
@@ -141,11 +145,6 @@ struct pt_renderer_s {
     DevBuf<int> ovf;
     DevBuf<uchar4> ldr;
 
-    // graph cache
-    hipGraphExec_t graph_exec = nullptr;
-    hipGraph_t graph = nullptr;
-    long long graph_key = -1;
-
     // stats
     pt_stats stats{};
     std::vector<StageEvent> pending;
@@ -155,8 +154,6 @@ struct pt_renderer_s {
 
     ~pt_renderer_s() {
         if (stream) (void)hipStreamSynchronize(stream);
-        if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
-        if (graph) (void)hipGraphDestroy(graph);
         for (auto& e : pending) { (void)hipEventDestroy(e.a); (void)hipEventDestroy(e.b); }
         for (auto e : free_events) (void)hipEventDestroy(e);
         if (ev_r0) (void)hipEventDestroy(ev_r0);
@@ -166,8 +163,8 @@ struct pt_renderer_s {
         nodes.free(); tris.free(); vidx.free(); verts.free(); norms.free(); mats.free(); inst.free();
         lights.free(); tex.free(); env_img.free(); env_cdf.free(); env_total.free(); env_ok.free(); env_scratch.free(); accum.free();
         p_ro.free(); p_rd.free(); p_thr.free(); p_rad.free(); p_aux.free(); p_med.free(); p_hit.free();
-        p_rng.free(); p_meta.free(); p_hit2.free(); s_o.free(); s_t.free(); s_d.free(); s_c.free();
-        a_rad.free(); a_sg.free(); a_dimg.free(); a_grad.free();
+        p_rng.free(); p_meta.free(); p_hit2.free(); s_idx.free(); s_o.free(); s_t.free(); s_d.free(); s_c.free();
+        a_rad.free(); a_vis.free(); a_dimg.free(); a_grad.free(); a_flags.free();
         b_jobs.free(); b_planes.free(); b_segs.free(); b_passes.free(); b_cur.free(); b_accum.free();
         work_counts.free(); ldr.free();
         t_xf.free(); t_local.free(); t_box.free(); t_cen.free(); t_blas.free(); t_mat.free(); t_ord.free();
@@ -236,41 +233,7 @@ struct pt_renderer_s {
     }
 
     DevOpts dev_opts() const {
-        DevOpts o{};
-        uint32_t f = 0;
-        if (sc.num_lights > 0) f |= ptd::OPT_LIGHTS;
-        if (opts.enable_rr) f |= ptd::OPT_RR;
-        if (opts.enable_uniform_light) f |= ptd::OPT_UNIFORM_LIGHT;
-        if (opts.enable_envmap && has_env) f |= ptd::OPT_ENVMAP;
-        if (opts.hide_emitters) f |= ptd::OPT_HIDE_EMITTERS;
-        if (opts.enable_background) f |= ptd::OPT_BACKGROUND;
-        if (opts.transparent_background) f |= ptd::OPT_TRANSPARENT_BACKGROUND;
-        if (any_alpha) f |= ptd::OPT_ALPHA_TEST;
-        if (any_medium) f |= ptd::OPT_MEDIUM;
-        if (opts.enable_volume_mis) f |= ptd::OPT_VOL_MIS;
-        if (opts.enable_roughness_mollification) f |= ptd::OPT_ROUGHNESS_MOLLIFICATION;
-        if (opts.opengl_normal_map) f |= ptd::OPT_OPENGL_NORMALMAP;
-        o.flags = f;
-        o.width = opts.width;
-        o.height = opts.height;
-        o.max_depth = opts.max_depth;
-        o.rr_depth = opts.rr_depth;
-        o.ucol_x = opts.uniform_light_col[0];
-        o.ucol_y = opts.uniform_light_col[1];
-        o.ucol_z = opts.uniform_light_col[2];
-        o.env_intensity = opts.envmap_intensity;
-        o.env_rot = opts.envmap_rot / 360.0f;
-        o.moll_amt = opts.roughness_mollification_amt;
-        for (int i = 0; i < 3; ++i) {
-            o.cam_pos[i] = cam.position[i];
-            o.cam_up[i] = cam.up[i];
-            o.cam_right[i] = cam.right[i];
-            o.cam_fwd[i] = cam.forward[i];
-        }
-        o.cam_fov = cam.fov;
-        o.cam_focal = cam.focalDist;
-        o.cam_aperture = cam.aperture;
-        return o;
+        return ptl::make_dev_opts(opts, cam, sc.num_lights, has_env, any_alpha, any_medium);
     }
 
     int block_size() const { return opts.block_size > 0 ? opts.block_size : 256; }
@@ -308,227 +271,6 @@ int Guard(pt_handle h, F&& f) {
     }
 }
 
-GInst make_inst(const float* m, int blas_ref, int material) {
-    GInst g;
-    if (!ptd::inst_record(m, blas_ref, material, &g)) throw ArgError(PT_ERR_INVALID_ARG, "singular instance transform");
-    return g;
-}
-
-// Re-layout of the reference's flat node array (bvh_translator.cpp:32-137)
-// into BVH2 nodes. Only nodes reachable from the TLAS root are converted;
-// GNode order follows the flat (preorder) order, so subtrees stay contiguous.
-struct Relayout {
-    std::vector<GNode> nodes;
-    int root = 0;
-    std::vector<int> inst_blas_ref, inst_material;
-    std::vector<float> inst_local;  // box of each instance's BLAS root: min xyz, max xyz
-    std::unordered_map<int, int> root_ref;  // flat index of each BLAS root a TLAS leaf names -> its reference
-    // The TLAS as the device rebuild writes it: one instance per leaf, every
-    // instance once, internal nodes numbered in preorder from tlas_base to the
-    // end of the node array.
-    bool tlas_rebuildable = false;
-    int tlas_base = 0;
-};
-
-Relayout relayout(const pt_bvh_node* flat, int num_nodes, int top, int num_tris, int num_instances) {
-    Relayout R;
-    R.inst_blas_ref.assign(num_instances, 0);
-    R.inst_material.assign(num_instances, 0);
-    R.inst_local.assign(6 * static_cast<size_t>(num_instances), 0.f);
-    auto kind = [&](int i) {
-        float z = flat[i].LRLeaf[2];
-        return z > 0.f ? 1 : (z < 0.f ? -1 : 0);
-    };
-    auto as_int = [](float f) { return static_cast<int>(f); };
-    std::vector<int> map(num_nodes, -1);
-    std::vector<char> reach(num_nodes, 0);
-    // mark reachable internal nodes; TLAS first, then each BLAS root
-    std::vector<int> st;
-    if (top < 0 || top >= num_nodes) throw ArgError(PT_ERR_INVALID_ARG, "top_level_index out of range");
-    st.push_back(top);
-    std::vector<char> blas_seen(num_nodes, 0);
-    while (!st.empty()) {
-        int i = st.back();
-        st.pop_back();
-        if (i < 0 || i >= num_nodes) throw ArgError(PT_ERR_INVALID_ARG, "node child index out of range");
-        if (reach[i]) throw ArgError(PT_ERR_INVALID_ARG, "node reachable twice (not a tree)");
-        reach[i] = 1;
-        int k = kind(i);
-        if (k == 0) {
-            st.push_back(as_int(flat[i].LRLeaf[0]));
-            st.push_back(as_int(flat[i].LRLeaf[1]));
-        } else if (k < 0) {
-            int inst = -as_int(flat[i].LRLeaf[2]) - 1;
-            if (inst < 0 || inst >= num_instances) throw ArgError(PT_ERR_INVALID_ARG, "TLAS leaf instance out of range");
-            int root = as_int(flat[i].LRLeaf[0]);
-            if (root < 0 || root >= num_nodes) throw ArgError(PT_ERR_INVALID_ARG, "BLAS root out of range");
-            if (!blas_seen[root]) {
-                blas_seen[root] = 1;
-                if (reach[root]) throw ArgError(PT_ERR_INVALID_ARG, "BLAS root inside the TLAS");
-                st.push_back(root);
-            }
-        } else {
-            int first = as_int(flat[i].LRLeaf[0]), count = as_int(flat[i].LRLeaf[1]);
-            if (count < 1 || count > 3) throw ArgError(PT_ERR_UNSUPPORTED, "BLAS leaves must hold 1..3 triangles");
-            if (first < 0 || first + count > num_tris) throw ArgError(PT_ERR_INVALID_ARG, "BLAS leaf triangle range out of bounds");
-            if (first >= (1 << 28)) throw ArgError(PT_ERR_UNSUPPORTED, "too many triangles for the leaf encoding");
-        }
-    }
-    int n = 0;
-    for (int i = 0; i < num_nodes; ++i)
-        if (reach[i] && kind(i) == 0) map[i] = n++;
-    R.nodes.resize(std::max(n, 1));
-    auto ref_of = [&](int i) -> int {
-        int k = kind(i);
-        if (k == 0) return map[i];
-        if (k < 0) return -1 - (-as_int(flat[i].LRLeaf[2]) - 1);
-        int first = as_int(flat[i].LRLeaf[0]), count = as_int(flat[i].LRLeaf[1]);
-        return -1 - ((first << 2) | count);
-    };
-    for (int i = 0; i < num_nodes; ++i) {
-        if (!reach[i]) continue;
-        int k = kind(i);
-        if (k == 0) {
-            int l = as_int(flat[i].LRLeaf[0]), r = as_int(flat[i].LRLeaf[1]);
-            const float* lmn = flat[l].bboxmin; const float* lmx = flat[l].bboxmax;
-            const float* rmn = flat[r].bboxmin; const float* rmx = flat[r].bboxmax;
-            GNode& g = R.nodes[map[i]];
-            g.a = make_float4(lmn[0], lmn[1], lmn[2], lmx[0]);
-            g.b = make_float4(lmx[1], lmx[2], rmn[0], rmn[1]);
-            g.c = make_float4(rmn[2], rmx[0], rmx[1], rmx[2]);
-            g.d = make_int4(ref_of(l), ref_of(r), 0, 0);
-        } else if (k < 0) {
-            int inst = -as_int(flat[i].LRLeaf[2]) - 1;
-            const int root = as_int(flat[i].LRLeaf[0]);
-            R.inst_blas_ref[inst] = ref_of(root);
-            R.root_ref[root] = R.inst_blas_ref[inst];
-            R.inst_material[inst] = as_int(flat[i].LRLeaf[1]);
-            for (int j = 0; j < 3; ++j) {
-                R.inst_local[6 * inst + j] = flat[root].bboxmin[j];
-                R.inst_local[6 * inst + 3 + j] = flat[root].bboxmax[j];
-            }
-        }
-    }
-    R.root = ref_of(top);
-    {
-        bool ok = true;
-        int internal = 0, leaves = 0;
-        const int base = kind(top) == 0 ? map[top] : n;
-        std::vector<char> inst_seen(num_instances, 0);
-        st.assign(1, top);
-        while (!st.empty() && ok) {
-            const int i = st.back();
-            st.pop_back();
-            const int k = kind(i);
-            if (k == 0) {
-                ok = map[i] == base + internal;
-                ++internal;
-                st.push_back(as_int(flat[i].LRLeaf[1]));
-                st.push_back(as_int(flat[i].LRLeaf[0]));
-            } else if (k < 0) {
-                const int inst = -as_int(flat[i].LRLeaf[2]) - 1;
-                ok = !inst_seen[inst];
-                inst_seen[inst] = 1;
-                ++leaves;
-            } else {
-                ok = false;
-            }
-        }
-        R.tlas_rebuildable = ok && leaves == num_instances && internal == num_instances - 1 && base + internal == n;
-        R.tlas_base = base;
-    }
-    return R;
-}
-
-// TLAS-only re-layout for an update of the TLAS node range [top, nflat) with
-// the BLASes unchanged (the reference uploads exactly that range,
-// src/core/Renderer.cpp:659-663): the n - 1 internal nodes, numbered in
-// preorder from base, plus each instance's BLAS root reference and material.
-// Returns false when the new range is not a one-instance-per-leaf preorder
-// tree or names a BLAS root that the last full re-layout did not convert;
-// the caller then re-lays the whole array out.
-struct TlasRelayout {
-    std::vector<GNode> nodes;
-    std::vector<int> blas, mat, roots;
-};
-bool relayout_tlas(const pt_bvh_node* tl, int top, int nflat, int n, int base,
-                   const std::unordered_map<int, int>& root_ref, TlasRelayout& T) {
-    auto at = [&](int i) -> const pt_bvh_node& { return tl[i - top]; };
-    auto kind = [&](int i) { const float z = at(i).LRLeaf[2]; return z > 0.f ? 1 : (z < 0.f ? -1 : 0); };
-    auto as_int = [](float f) { return static_cast<int>(f); };
-    T.nodes.assign(std::max(n - 1, 0), GNode{});
-    T.blas.assign(n, 0);
-    T.mat.assign(n, 0);
-    T.roots.assign(n, -1);
-    std::vector<char> seen(n, 0);
-    struct Item { int i, parent, side; };
-    std::vector<Item> st{{top, -1, 0}};
-    int internal = 0, leaves = 0;
-    while (!st.empty()) {
-        const Item it = st.back();
-        st.pop_back();
-        if (it.i < top || it.i >= nflat) return false;
-        int ref;
-        const int k = kind(it.i);
-        if (k == 0) {
-            if (internal >= n - 1) return false;
-            const int idx = internal++;
-            ref = base + idx;
-            const int l = as_int(at(it.i).LRLeaf[0]), r = as_int(at(it.i).LRLeaf[1]);
-            if (l < top || l >= nflat || r < top || r >= nflat) return false;
-            const float* lmn = at(l).bboxmin; const float* lmx = at(l).bboxmax;
-            const float* rmn = at(r).bboxmin; const float* rmx = at(r).bboxmax;
-            GNode& g = T.nodes[idx];
-            g.a = make_float4(lmn[0], lmn[1], lmn[2], lmx[0]);
-            g.b = make_float4(lmx[1], lmx[2], rmn[0], rmn[1]);
-            g.c = make_float4(rmn[2], rmx[0], rmx[1], rmx[2]);
-            g.d = make_int4(0, 0, 0, 0);
-            st.push_back({r, idx, 1});
-            st.push_back({l, idx, 0});
-        } else if (k < 0) {
-            const int inst = -as_int(at(it.i).LRLeaf[2]) - 1;
-            if (inst < 0 || inst >= n || seen[inst]) return false;
-            seen[inst] = 1;
-            ++leaves;
-            const int root = as_int(at(it.i).LRLeaf[0]);
-            const auto f = root_ref.find(root);
-            if (f == root_ref.end()) return false;
-            T.blas[inst] = f->second;
-            T.mat[inst] = as_int(at(it.i).LRLeaf[1]);
-            T.roots[inst] = root;
-            ref = -1 - inst;
-        } else {
-            return false;
-        }
-        if (it.parent >= 0) {
-            if (it.side == 0) T.nodes[it.parent].d.x = ref;
-            else T.nodes[it.parent].d.y = ref;
-        }
-    }
-    return internal == n - 1 && leaves == n;
-}
-
-GLight make_light(const pt_light& l) {
-    GLight g{};
-    g.p = make_float4(l.position[0], l.position[1], l.position[2], l.radius);
-    g.e = make_float4(l.emission[0], l.emission[1], l.emission[2], l.area);
-    g.u = make_float4(l.u[0], l.u[1], l.u[2], l.type);
-    g.v = make_float4(l.v[0], l.v[1], l.v[2], 0.f);
-    // normalize(cross(u, v)), plane w, u / |u|^2, v / |v|^2 (closest_hit.glsl:50-54),
-    // in the same float operations the shader performs.
-    const float* u = l.u; const float* v = l.v;
-    float cx = u[1] * v[2] - u[2] * v[1], cy = u[2] * v[0] - u[0] * v[2], cz = u[0] * v[1] - u[1] * v[0];
-    float len = std::sqrt(cx * cx + cy * cy + cz * cz);
-    float nx = cx / len, ny = cy / len, nz = cz / len;
-    float w = nx * l.position[0] + ny * l.position[1] + nz * l.position[2];
-    g.n = make_float4(nx, ny, nz, w);
-    float iu = 1.0f / (u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
-    float iv = 1.0f / (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
-    g.uu = make_float4(u[0] * iu, u[1] * iu, u[2] * iu, 0.f);
-    g.vv = make_float4(v[0] * iv, v[1] * iv, v[2] * iv, 0.f);
-    return g;
-}
-
 ptd::PathBuf path_buf(pt_renderer_s* h) {
     ptd::PathBuf pb;
     pb.ro = h->p_ro.p; pb.rd = h->p_rd.p; pb.thr = h->p_thr.p; pb.rad = h->p_rad.p; pb.rng = h->p_rng.p;
@@ -550,22 +292,31 @@ void check_ready(pt_renderer_s* h) {
 //   [3D+3, 6D+6)        work counters (extend, shade, shadow) of bounce b
 int counter_cells(int D) { return 6 * D + 6; }
 
+// NEE visibility is resolved inside the shade stage (shadow records are not
+// used) when AnyHit draws from the path's random numbers between NEE sampling
+// and BSDF sampling (stochastic BLEND alpha without media, anyhit.glsl:118-141)
+// or when it is a transmittance (volume MIS with media, pathtrace.glsl:118-156).
+bool inline_nee(const pt_renderer_s* h, const DevOpts& op) {
+    if ((op.flags & ptd::OPT_MEDIUM) && (op.flags & ptd::OPT_VOL_MIS)) return true;
+    return (op.flags & ptd::OPT_ALPHA_TEST) && !(op.flags & ptd::OPT_MEDIUM) && h->any_blend;
+}
+
+void ensure_shadow(pt_renderer_s* h, size_t max_paths) {
+    h->s_idx.ensure(max_paths); h->s_o.ensure(max_paths); h->s_t.ensure(max_paths);
+    h->s_d.ensure(2 * max_paths); h->s_c.ensure(2 * max_paths);
+}
+
+ptd::ShadowBuf shadow_buf(pt_renderer_s* h) { return ptd::ShadowBuf{h->s_idx.p, h->s_o.p, h->s_t.p, h->s_d.p, h->s_c.p}; }
+
 // Launch shape and buffers of the wavefront form for passes of up to
 // spp_pass samples of every pixel.
 struct Wavefront {
     int npix, spp_pass, D, bs, S, b_ext, b_sha, b_sdw;
-    bool cw;
+    bool cw, inline_nee;
+    size_t max_paths;
 };
 
-Wavefront wavefront_setup(pt_renderer_s* h, const DevOpts& op, int num_samples, size_t adj_lds) {
-    if ((op.flags & ptd::OPT_VOL_MIS) && (op.flags & ptd::OPT_MEDIUM))
-        throw ArgError(PT_ERR_UNSUPPORTED, "volume MIS is implemented by the plain kernel only (mode PT_MODE_PLAIN)");
-    if ((op.flags & ptd::OPT_ALPHA_TEST) && !(op.flags & ptd::OPT_MEDIUM) && h->any_blend) {
-        // Stochastic (BLEND) alpha in AnyHit draws from the path's RNG stream
-        // between NEE and BSDF sampling, which the deferred shadow stage cannot
-        // reproduce; MASK alpha is deterministic and runs in the shadow stage.
-        throw ArgError(PT_ERR_UNSUPPORTED, "stochastic (blend) alpha in shadow rays needs the plain kernel (mode PT_MODE_PLAIN)");
-    }
+Wavefront wavefront_setup(pt_renderer_s* h, const DevOpts& op, int num_samples) {
     Wavefront w;
     const int W = op.width, H = op.height;
     const int tiles = ((W + 7) / 8) * ((H + 7) / 8);
@@ -573,11 +324,13 @@ Wavefront wavefront_setup(pt_renderer_s* h, const DevOpts& op, int num_samples, 
     w.spp_pass = std::max(1, std::min(h->opts.wave_spp > 0 ? h->opts.wave_spp : 1, num_samples));
     const size_t max_paths = static_cast<size_t>(w.npix) * w.spp_pass;
     if (max_paths > static_cast<size_t>(INT32_MAX / 2)) throw ArgError(PT_ERR_OVERFLOW, "too many paths per pass");
+    w.max_paths = max_paths;
     h->p_ro.ensure(max_paths); h->p_rd.ensure(max_paths); h->p_thr.ensure(max_paths); h->p_rad.ensure(max_paths);
     h->p_rng.ensure(max_paths); h->p_meta.ensure(max_paths); h->p_aux.ensure(max_paths);
     if (h->any_medium) h->p_med.ensure(max_paths);
     h->p_hit.ensure(max_paths); h->p_hit2.ensure(max_paths);
-    h->s_o.ensure(max_paths); h->s_t.ensure(max_paths); h->s_d.ensure(2 * max_paths); h->s_c.ensure(2 * max_paths);
+    w.inline_nee = inline_nee(h, op);
+    if (!w.inline_nee) ensure_shadow(h, max_paths);
     h->qa.ensure(max_paths); h->qb.ensure(max_paths); h->hq.ensure(max_paths);
     // Bounce iterations: maxDepth + 1 closest hits; alpha pass-through repeats a depth.
     w.D = op.max_depth + 1 + ((op.flags & ptd::OPT_ALPHA_TEST) ? 64 : 0);
@@ -587,19 +340,19 @@ Wavefront wavefront_setup(pt_renderer_s* h, const DevOpts& op, int num_samples, 
     w.S = h->stack_S();
     w.cw = h->opts.count_work != 0;
     w.b_ext = ptk::persistent_blocks(ptk::KIND_EXTEND, w.bs, w.S, w.cw, h->num_cu);
-    w.b_sha = adj_lds != SIZE_MAX ? ptk::persistent_blocks(ptk::KIND_SHADE_ADJ, w.bs, w.S, false, h->num_cu, adj_lds)
-                                  : ptk::persistent_blocks(ptk::KIND_SHADE, w.bs, w.S, w.cw, h->num_cu);
-    w.b_sdw = adj_lds != SIZE_MAX ? ptk::persistent_blocks(ptk::KIND_SHADOW_ADJ, w.bs, w.S, false, h->num_cu, adj_lds)
-                                  : ptk::persistent_blocks(ptk::KIND_SHADOW, w.bs, w.S, w.cw, h->num_cu);
-    h->ensure_spill(static_cast<size_t>(std::max(w.b_ext, w.b_sdw)) * w.bs);
+    w.b_sha = ptk::persistent_blocks(w.inline_nee ? ptk::KIND_SHADE_INLINE : ptk::KIND_SHADE, w.bs, w.S, w.cw, h->num_cu);
+    w.b_sdw = ptk::persistent_blocks(ptk::KIND_SHADOW, w.bs, w.S, w.cw, h->num_cu);
+    // every traversing grid (extend, shadow, an inline-NEE shade) indexes the
+    // stack's backing store by its own thread slot
+    h->ensure_spill(static_cast<size_t>(std::max({w.b_ext, w.b_sdw, w.inline_nee ? w.b_sha : 0})) * w.bs);
     return w;
 }
 
 // Every bounce of one pass (paths [0, npaths), samples fs...): extend, shade,
-// shadow per bounce. With `ad` the shade and shadow stages are the adjoint's
-// replay variants.
+// shadow per bounce. vis (the adjoint's primal pass, else null): visibility
+// words of the pass, ceil(D / 16) arrays of max_paths words.
 void run_bounces(pt_renderer_s* h, const DevOpts& op, const Wavefront& w, const ptd::PathBuf& pb,
-                 const ptd::ShadowBuf& sb, int npaths, int fs, const ptd::AdjBuf* ad) {
+                 const ptd::ShadowBuf& sb, int npaths, int fs, unsigned* vis) {
     int* cnt = h->counters.p;
     hipStream_t s = h->stream;
     const int D = w.D;
@@ -609,7 +362,6 @@ void run_bounces(pt_renderer_s* h, const DevOpts& op, const Wavefront& w, const 
     int* qout = h->qb.p;
     int* c_ext = cnt;  // count of the current extend queue
     const int cadence = std::max(1, h->opts.compact_every);
-    const bool cw = w.cw && !ad;
     for (int b = 0; b < D; ++b) {
         int* c_hit = cnt + (D + 1) + b;
         int* c_sh = cnt + 2 * (D + 1) + b;
@@ -622,17 +374,19 @@ void run_bounces(pt_renderer_s* h, const DevOpts& op, const Wavefront& w, const 
         int* c_next = compact ? cnt + b + 1 : c_ext;
         h->stage_begin(PT_STAGE_EXTEND, ev);
         ptk::launch_extend(s, w.b_ext, w.bs, h->sc, op, pb, primary, qin, c_ext, npaths, w.npix, fs, h->hq.p, c_hit,
-                           wk + 0, w.S, h->spill.p, h->work_counts.p, h->ovf.p, cw);
+                           wk + 0, w.S, h->spill.p, h->work_counts.p, h->ovf.p, w.cw);
         h->stage_end(PT_STAGE_EXTEND, ev);
-        const int shade_stage = ad ? PT_STAGE_ADJOINT : PT_STAGE_SHADE;
-        h->stage_begin(shade_stage, ev);
-        ptk::launch_shade(s, w.b_sha, w.bs, h->sc, op, pb, sb, primary, h->hq.p, c_hit, w.npix, fs, qout, c_next, c_sh,
-                          wk + 1, compact, ad);
-        h->stage_end(shade_stage, ev);
-        h->stage_begin(PT_STAGE_SHADOW, ev);
-        ptk::launch_shadow(s, w.b_sdw, w.bs, h->sc, op, pb, sb, c_sh, wk + 2, w.S, h->spill.p, h->work_counts.p + 8,
-                           h->ovf.p, cw, ad);
-        h->stage_end(PT_STAGE_SHADOW, ev);
+        h->stage_begin(PT_STAGE_SHADE, ev);
+        ptk::launch_shade(s, w.b_sha, w.bs, h->sc, op, pb, sb, nullptr, primary, w.inline_nee, h->hq.p, c_hit, w.npix,
+                          fs, qout, c_next, c_sh, wk + 1, compact, w.S, h->spill.p, h->work_counts.p + 8, h->ovf.p);
+        h->stage_end(PT_STAGE_SHADE, ev);
+        if (!w.inline_nee) {
+            unsigned* vb = vis ? vis + static_cast<size_t>(b / ptd::VIS_BOUNCES_PER_WORD) * w.max_paths : nullptr;
+            h->stage_begin(PT_STAGE_SHADOW, ev);
+            ptk::launch_shadow(s, w.b_sdw, w.bs, h->sc, op, pb, sb, c_sh, wk + 2, w.S, h->spill.p,
+                               h->work_counts.p + 8, h->ovf.p, w.cw, vb, 2 * (b % ptd::VIS_BOUNCES_PER_WORD));
+            h->stage_end(PT_STAGE_SHADOW, ev);
+        }
         if (compact) {
             std::swap(qin, qout);
             c_ext = c_next;
@@ -642,9 +396,9 @@ void run_bounces(pt_renderer_s* h, const DevOpts& op, const Wavefront& w, const 
 
 void render_wavefront(pt_renderer_s* h, int first_sample, int num_samples) {
     const DevOpts op = h->dev_opts();
-    const Wavefront w = wavefront_setup(h, op, num_samples, SIZE_MAX);
+    const Wavefront w = wavefront_setup(h, op, num_samples);
     const ptd::PathBuf pb = path_buf(h);
-    const ptd::ShadowBuf sb{h->s_o.p, h->s_t.p, h->s_d.p, h->s_c.p};
+    const ptd::ShadowBuf sb = shadow_buf(h);
     for (int s0 = 0; s0 < num_samples; s0 += w.spp_pass) {
         const int ns = std::min(w.spp_pass, num_samples - s0);
         run_bounces(h, op, w, pb, sb, w.npix * ns, first_sample + s0, nullptr);
@@ -656,35 +410,76 @@ void render_wavefront(pt_renderer_s* h, int first_sample, int num_samples) {
     }
 }
 
-// Path-replay adjoint: per pass, a primal pass leaves every path's total
-// radiance in the radiance buffer; the replay pass runs the same bounces on a
-// second radiance buffer and accumulates parameter gradients.
+// Path-replay adjoint (pt_replay.h). Per pass of samples: the primal pass is
+// the forward wavefront pass, which leaves every path's total radiance in the
+// radiance buffer and records which NEE rays were visible; the replay pass
+// re-traces the same paths (closest hits again, same seeds, same decisions)
+// on a second radiance buffer, applies the recorded visibility instead of
+// shadow rays, and accumulates each surface vertex's parameter gradient.
 void adjoint_wavefront(pt_renderer_s* h, int first_sample, int num_samples, float inv_n) {
     const DevOpts op = h->dev_opts();
     if (op.flags & ptd::OPT_ROUGHNESS_MOLLIFICATION)
         throw ArgError(PT_ERR_UNSUPPORTED, "the adjoint does not differentiate roughness mollification");
+    if (inline_nee(h, op))
+        throw ArgError(PT_ERR_UNSUPPORTED, "the adjoint replays recorded NEE visibility: stochastic (blend) alpha "
+                                           "in shadow rays and volume MIS are not supported");
+    const Wavefront w = wavefront_setup(h, op, num_samples);
     const size_t grad_bytes = static_cast<size_t>(h->num_materials) * ptd::NPARAM * sizeof(float);
     const bool lds = grad_bytes <= 32768;
-    const Wavefront w = wavefront_setup(h, op, num_samples, lds ? grad_bytes : 0);
-    const size_t max_paths = static_cast<size_t>(w.npix) * w.spp_pass;
-    h->a_rad.ensure(max_paths);
-    h->a_sg.ensure(4 * max_paths);
+    const int b_rep = ptk::persistent_blocks(ptk::KIND_REPLAY, w.bs, w.S, false, h->num_cu, lds ? grad_bytes : 0);
+    const int nwords = (w.D + ptd::VIS_BOUNCES_PER_WORD - 1) / ptd::VIS_BOUNCES_PER_WORD;
+    const size_t vis_words = static_cast<size_t>(nwords) * w.max_paths;
+    if (vis_words > static_cast<size_t>(INT32_MAX)) throw ArgError(PT_ERR_OVERFLOW, "too many paths per pass");
+    h->a_rad.ensure(w.max_paths);
+    h->a_vis.ensure(vis_words);
     const ptd::PathBuf pb = path_buf(h);
     ptd::PathBuf pb2 = pb;
     pb2.rad = h->a_rad.p;
-    const ptd::ShadowBuf sb{h->s_o.p, h->s_t.p, h->s_d.p, h->s_c.p};
-    ptd::AdjBuf ad{};
-    ad.ltot = pb.rad;
-    ad.dimg = h->a_dimg.p;
-    ad.inv_n = inv_n;
-    ad.grad = h->a_grad.p;
-    ad.sg = h->a_sg.p;
-    ad.num_mats = h->num_materials;
-    ad.lds = lds ? 1 : 0;
+    const ptd::ShadowBuf sb = shadow_buf(h);
+    ptd::ReplayBuf rb{};
+    rb.ltot = pb.rad;
+    rb.dimg = h->a_dimg.p;
+    rb.inv_n = inv_n;
+    rb.grad = h->a_grad.p;
+    rb.err = h->a_flags.p;
+    rb.dropped = h->a_flags.p + 1;
+    rb.num_mats = h->num_materials;
+    rb.lds = lds ? 1 : 0;
+    hipStream_t s = h->stream;
+    int* cnt = h->counters.p;
+    const int D = w.D;
+    const int cadence = std::max(1, h->opts.compact_every);
     for (int s0 = 0; s0 < num_samples; s0 += w.spp_pass) {
         const int ns = std::min(w.spp_pass, num_samples - s0);
-        run_bounces(h, op, w, pb, sb, w.npix * ns, first_sample + s0, nullptr);
-        run_bounces(h, op, w, pb2, sb, w.npix * ns, first_sample + s0, &ad);
+        const int npaths = w.npix * ns, fs = first_sample + s0;
+        ptk::launch_fill(s, reinterpret_cast<int*>(h->a_vis.p), static_cast<int>(vis_words), 0);
+        run_bounces(h, op, w, pb, sb, npaths, fs, h->a_vis.p);
+        // replay: the extend stage of the forward renderer, then the replay stage
+        ptk::launch_fill(s, cnt, counter_cells(D), 0);
+        int* qin = h->qa.p;
+        int* qout = h->qb.p;
+        int* c_ext = cnt;
+        hipEvent_t ev;
+        for (int b = 0; b < D; ++b) {
+            int* c_hit = cnt + (D + 1) + b;
+            int* wk = cnt + 3 * (D + 1) + 3 * b;
+            const bool primary = b == 0;
+            const bool compact = primary || (b % cadence) == 0;
+            int* c_next = compact ? cnt + b + 1 : c_ext;
+            h->stage_begin(PT_STAGE_EXTEND, ev);
+            ptk::launch_extend(s, w.b_ext, w.bs, h->sc, op, pb2, primary, qin, c_ext, npaths, w.npix, fs, h->hq.p,
+                               c_hit, wk + 0, w.S, h->spill.p, h->work_counts.p, h->ovf.p, false);
+            h->stage_end(PT_STAGE_EXTEND, ev);
+            rb.vis = h->a_vis.p + static_cast<size_t>(b / ptd::VIS_BOUNCES_PER_WORD) * w.max_paths;
+            h->stage_begin(PT_STAGE_ADJOINT, ev);
+            ptk::launch_replay_shade(s, b_rep, w.bs, h->sc, op, pb2, primary, h->hq.p, c_hit, w.npix, fs, qout, c_next,
+                                     wk + 1, compact, rb, 2 * (b % ptd::VIS_BOUNCES_PER_WORD));
+            h->stage_end(PT_STAGE_ADJOINT, ev);
+            if (compact) {
+                std::swap(qin, qout);
+                c_ext = c_next;
+            }
+        }
         HIPCHK(hipGetLastError());
     }
 }
@@ -775,10 +570,7 @@ BatchPlan plan_batch(const pt_job* jobs, int n, long long budget) {
 
 void batch_render(pt_renderer_s* h, const pt_job* jobs, int n) {
     DevOpts op = h->dev_opts();
-    if ((op.flags & ptd::OPT_VOL_MIS) && (op.flags & ptd::OPT_MEDIUM))
-        throw ArgError(PT_ERR_UNSUPPORTED, "volume MIS is implemented by the plain kernel only");
-    if ((op.flags & ptd::OPT_ALPHA_TEST) && !(op.flags & ptd::OPT_MEDIUM) && h->any_blend)
-        throw ArgError(PT_ERR_UNSUPPORTED, "stochastic (blend) alpha in shadow rays needs the plain kernel");
+    const bool inl = inline_nee(h, op);
     // pass budget: wave_spp samples of the largest job
     long long max_npix = 0;
     for (int j = 0; j < n; ++j)
@@ -789,7 +581,7 @@ void batch_render(pt_renderer_s* h, const pt_job* jobs, int n) {
     h->p_rng.ensure(max_paths); h->p_meta.ensure(max_paths); h->p_aux.ensure(max_paths);
     if (h->any_medium) h->p_med.ensure(max_paths);
     h->p_hit.ensure(max_paths); h->p_hit2.ensure(max_paths);
-    h->s_o.ensure(max_paths); h->s_t.ensure(max_paths); h->s_d.ensure(2 * max_paths); h->s_c.ensure(2 * max_paths);
+    if (!inl) ensure_shadow(h, max_paths);
     h->qa.ensure(max_paths); h->qb.ensure(max_paths); h->hq.ensure(max_paths);
     const int D = P.max_depth + 1 + ((op.flags & ptd::OPT_ALPHA_TEST) ? 64 : 0);
     const int ncells = counter_cells(D);
@@ -797,10 +589,11 @@ void batch_render(pt_renderer_s* h, const pt_job* jobs, int n) {
     h->ensure_misc();
     const int bs = h->block_size(), S = h->stack_S();
     const int b_ext = ptk::persistent_blocks(ptk::KIND_EXTEND, bs, S, false, h->num_cu);
-    const int b_sha = ptk::persistent_blocks(ptk::KIND_SHADE_BATCH, bs, S, false, h->num_cu);
+    const int b_sha = ptk::persistent_blocks(inl ? ptk::KIND_SHADE_BATCH_INLINE : ptk::KIND_SHADE_BATCH, bs, S, false,
+                                             h->num_cu);
     const int b_sdw = ptk::persistent_blocks(ptk::KIND_SHADOW, bs, S, false, h->num_cu);
     const int b_acc = 4 * h->num_cu;
-    h->ensure_spill(static_cast<size_t>(std::max(b_ext, b_sdw)) * bs);
+    h->ensure_spill(static_cast<size_t>(std::max({b_ext, b_sdw, inl ? b_sha : 0})) * bs);
     hipStream_t s = h->stream;
     h->b_jobs.upload(P.jobs.data(), P.jobs.size(), s);
     h->b_planes.upload(P.planes.data(), P.planes.size(), s);
@@ -812,7 +605,7 @@ void batch_render(pt_renderer_s* h, const pt_job* jobs, int n) {
     HIPCHK(hipMemsetAsync(h->b_accum.p, 0, P.pixels * sizeof(float4), s));
     const ptd::BatchBuf bb{h->b_jobs.p, h->b_planes.p, h->b_segs.p, h->b_passes.p, h->b_cur.p, h->b_accum.p};
     const ptd::PathBuf pb = path_buf(h);
-    const ptd::ShadowBuf sb{h->s_o.p, h->s_t.p, h->s_d.p, h->s_c.p};
+    const ptd::ShadowBuf sb = shadow_buf(h);
     int* cnt = h->counters.p;
     const int cadence = std::max(1, h->opts.compact_every);
     const bool graph = h->opts.use_graph != 0;
@@ -843,13 +636,14 @@ void batch_render(pt_renderer_s* h, const pt_job* jobs, int n) {
                                         h->spill.p, h->work_counts.p, h->ovf.p, false);
             });
             stage(PT_STAGE_SHADE, [&] {
-                ptk::launch_shade_batch(s, b_sha, bs, h->sc, op, pb, sb, bb, primary, h->hq.p, c_hit, qout, c_next, c_sh,
-                                        wk + 1, compact);
+                ptk::launch_shade(s, b_sha, bs, h->sc, op, pb, sb, &bb, primary, inl, h->hq.p, c_hit, 0, 0, qout,
+                                  c_next, c_sh, wk + 1, compact, S, h->spill.p, h->work_counts.p + 8, h->ovf.p);
             });
-            stage(PT_STAGE_SHADOW, [&] {
-                ptk::launch_shadow(s, b_sdw, bs, h->sc, op, pb, sb, c_sh, wk + 2, S, h->spill.p, h->work_counts.p + 8,
-                                   h->ovf.p, false);
-            });
+            if (!inl)
+                stage(PT_STAGE_SHADOW, [&] {
+                    ptk::launch_shadow(s, b_sdw, bs, h->sc, op, pb, sb, c_sh, wk + 2, S, h->spill.p,
+                                       h->work_counts.p + 8, h->ovf.p, false);
+                });
             if (compact) {
                 std::swap(qin, qout);
                 c_ext = c_next;
@@ -1030,20 +824,9 @@ int pt_upload_scene(pt_handle h, const pt_scene_view* v) {
         for (int i = 0; i < v->num_instances; ++i)
             if (R.inst_material[i] < 0 || R.inst_material[i] >= v->num_materials)
                 return Fail(h, PT_ERR_INVALID_ARG, "TLAS leaf material out of range");
-        // triangles in leaf order
-        std::vector<GTri> tris(v->num_tris);
-        std::vector<int4> vidx(v->num_tris);
-        const float* P = v->vertices_uvx;
-        for (int r = 0; r < v->num_tris; ++r) {
-            const int* vi = v->vert_indices + 3 * r;
-            const float* a = P + 4 * static_cast<size_t>(vi[0]);
-            const float* b = P + 4 * static_cast<size_t>(vi[1]);
-            const float* c = P + 4 * static_cast<size_t>(vi[2]);
-            tris[r].v0 = make_float4(a[0], a[1], a[2], 0.f);
-            tris[r].e0 = make_float4(b[0] - a[0], b[1] - a[1], b[2] - a[2], 0.f);
-            tris[r].e1 = make_float4(c[0] - a[0], c[1] - a[1], c[2] - a[2], 0.f);
-            vidx[r] = make_int4(vi[0], vi[1], vi[2], 0);
-        }
+        std::vector<GTri> tris;
+        std::vector<int4> vidx;
+        ptl::make_tris(*v, tris, vidx);
         std::vector<GInst> insts(v->num_instances);
         for (int i = 0; i < v->num_instances; ++i)
             insts[i] = make_inst(v->transforms + 16 * static_cast<size_t>(i), R.inst_blas_ref[i], R.inst_material[i]);
@@ -1515,15 +1298,6 @@ int pt_batch_render(pt_handle h, const pt_job* jobs, int32_t num_jobs) {
 
 int pt_adjoint(pt_handle h, int32_t first_sample, int32_t num_samples, const float* dL_dimage, float* d_params) {
     return Guard(h, [&] {
-#ifndef PT_ENABLE_ADJOINT
-        // The replay kernels are built but not launched: after the per-path
-        // shadow-record change they produced NaN gradients and then an illegal
-        // memory access on gfx950, with no cause found yet. Build with
-        // -DPT_ENABLE_ADJOINT to run them.
-        (void)first_sample; (void)num_samples; (void)dL_dimage; (void)d_params;
-        return Fail(h, PT_ERR_UNSUPPORTED,
-                    "pt_adjoint is disabled in this build (its kernels faulted on gfx950; cause under investigation)");
-#endif
         check_ready(h);
         if (first_sample < 0 || num_samples <= 0) return Fail(h, PT_ERR_INVALID_ARG, "invalid sample range");
         if (!dL_dimage || !d_params) return Fail(h, PT_ERR_INVALID_ARG, "null gradient buffers");
@@ -1534,7 +1308,9 @@ int pt_adjoint(pt_handle h, int32_t first_sample, int32_t num_samples, const flo
         const size_t ng = static_cast<size_t>(h->num_materials) * ptd::NPARAM;
         h->a_dimg.upload(dL_dimage, 3 * npx, h->stream);
         h->a_grad.ensure(ng);
-        HIPCHK(hipMemsetAsync(h->a_grad.p, 0, ng * sizeof(float), h->stream));
+        h->a_flags.ensure(2);
+        ptk::launch_fill(h->stream, reinterpret_cast<int*>(h->a_grad.p), static_cast<int>(ng), 0);
+        ptk::launch_fill(h->stream, h->a_flags.p, 2, 0);
         h->ensure_misc();
         HIPCHK(hipMemsetAsync(h->ovf.p, 0, sizeof(int), h->stream));
         HIPCHK(hipEventRecord(h->ev_r0, h->stream));
@@ -1542,8 +1318,12 @@ int pt_adjoint(pt_handle h, int32_t first_sample, int32_t num_samples, const flo
         HIPCHK(hipEventRecord(h->ev_r1, h->stream));
         h->render_pending = true;
         std::vector<float> g(ng);
+        int flags[2] = {0, 0};
         HIPCHK(hipMemcpyAsync(g.data(), h->a_grad.p, ng * sizeof(float), hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipMemcpyAsync(flags, h->a_flags.p, sizeof(flags), hipMemcpyDeviceToHost, h->stream));
         h->resolve_events();
+        h->stats.adjoint_dropped = flags[1];
+        if (flags[0]) return Fail(h, PT_ERR_STATE, "adjoint: a path vertex named a material outside the scene's table");
         for (size_t i = 0; i < ng; ++i) d_params[i] += g[i];
         return h->stats.stack_overflow ? Fail(h, PT_ERR_OVERFLOW, "BVH traversal exceeded the maximum stack depth") : PT_OK;
     });

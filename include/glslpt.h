@@ -175,6 +175,8 @@ typedef struct pt_stats {
     int64_t rays_shadow, shadow_nodes, shadow_tris, shadow_instances, shadow_lights;
     int64_t stack_spills;            /* traversal-stack entries that went to the global backing store */
     int32_t stack_overflow;          /* 1 if a traversal exceeded the maximum stack depth */
+    int32_t adjoint_dropped;         /* last pt_adjoint: path vertices whose gradient had a non-finite
+                                        term and was left out */
 } pt_stats;
 
 /* stage indices of pt_stats.ms_stage */
@@ -281,11 +283,13 @@ int pt_batch_render(pt_handle h, const pt_job* jobs, int32_t num_jobs);
  * d_params receives num_materials*7 floats, added to (not overwritten).
  * Sampling, MIS weights and Russian roulette are detached: the estimate is
  * unbiased for the gradient of the expected image. Runs the wavefront form
- * (primal pass + replay pass) whatever pt_options.mode says; returns
- * PT_ERR_UNSUPPORTED with roughness mollification enabled.
- * The library is built with this entry point switched off: it returns
- * PT_ERR_UNSUPPORTED unless compiled with -DPT_ENABLE_ADJOINT (its replay
- * kernels faulted on gfx950 and the cause is not yet found). */
+ * whatever pt_options.mode says: a primal pass (the forward render, which
+ * also records which NEE rays were visible) and a replay pass that re-traces
+ * the same paths and accumulates the gradients. Returns PT_ERR_UNSUPPORTED
+ * with roughness mollification, stochastic (blend) alpha in shadow rays or
+ * volume MIS enabled; PT_ERR_STATE if a path vertex names a material outside
+ * the table. Vertices whose gradient has a non-finite term are left out and
+ * counted in pt_stats.adjoint_dropped. */
 #define PT_ADJOINT_PARAMS 7
 int pt_adjoint(pt_handle h, int32_t first_sample, int32_t num_samples, const float* dL_dimage,
                float* d_params);

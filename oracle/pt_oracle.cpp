@@ -710,10 +710,11 @@ template <class T> V3<T> DisneySample(Ctx<T>& c, const State<T>& state, V3<T> V,
 }
 
 // ------------------------------------------------------------ closest_hit.glsl:25-266
-// Instance inverse (closest_hit.glsl:163-164 calls GLSL inverse() per TLAS
-// leaf; its precision is left to the GL driver). 1 (default) = the inverse
-// computed in double and rounded to the scalar type, i.e. the correctly rounded
-// inverse; 0 = cofactor inverse evaluated in the scalar type on every visit.
+// Instance inverses (closest_hit.glsl:163-164 calls GLSL inverse() per TLAS
+// leaf, and :246 transpose(inverse(mat3(transform))) for the normal; their
+// precision is left to the GL driver). 1 (default) = both inverses computed in
+// double and rounded to the scalar type, i.e. correctly rounded; 0 = cofactor
+// inverses evaluated in the scalar type on every visit.
 // The two differ only on rays that tie between coplanar faces or pass through
 // a shared edge (tests/test_oracle_geometry.py bounds this). Set before use.
 int g_inverse_mode = 1;
@@ -724,6 +725,15 @@ template <class T> M4<T> instance_inverse(const M4<T>& m) {
     M4<double> id = inverse4(md);
     M4<T> r;
     for (int i = 0; i < 4; ++i) for (int j = 0; j < 4; ++j) r.m[i][j] = static_cast<T>(id.m[i][j]);
+    return r;
+}
+template <class T> M3<T> normal_inverse(const M3<T>& m) {
+    if (g_inverse_mode == 0) return inverse3(m);
+    M3<double> md;
+    for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) md.m[i][j] = static_cast<double>(m.m[i][j]);
+    M3<double> id = inverse3(md);
+    M3<T> r;
+    for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) r.m[i][j] = static_cast<T>(id.m[i][j]);
     return r;
 }
 
@@ -898,7 +908,7 @@ template <class T> bool ClosestHit(Ctx<T>& c, const Ray<T>& r, State<T>& state, 
                                  V3<T>(n2.x, n2.y, n2.z) * bary.z);
         M3<T> m3;
         for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) m3.m[i][j] = transform.m[i][j];
-        state.normal = normalize(mul3(transpose3(inverse3(m3)), normal));
+        state.normal = normalize(mul3(transpose3(normal_inverse(m3)), normal));
         state.ffnormal = dot(state.normal, r.direction) <= T(0) ? state.normal : -state.normal;
         V3<T> deltaPos1 = V3<T>(vert1.x, vert1.y, vert1.z) - V3<T>(vert0.x, vert0.y, vert0.z);
         V3<T> deltaPos2 = V3<T>(vert2.x, vert2.y, vert2.z) - V3<T>(vert0.x, vert0.y, vert0.z);

@@ -1,21 +1,22 @@
 """GPU parity of the option paths beyond the basic scenes (HDR environment
 map with importance sampling, textures incl. normal/emission maps,
 homogeneous media, volume MIS, alpha mask and stochastic alpha) against the
-CPU oracle on the same seeds, in both kernel forms where the form supports
-the path; unsupported combinations must fail loudly."""
+CPU oracle on the same seeds, in both kernel forms and in the batched form.
+Stochastic alpha and volume MIS resolve NEE visibility inside the wavefront
+shade stage (the reference's draw order), the other paths through the
+shadow stage."""
 import numpy as np
 import pytest
 
 import oracle
 from common import oracle_setup, rel_err
 from glslpt import scenes
-from glslpt._capi import PT_MODE_PLAIN, PT_MODE_WAVEFRONT, PtError
+from glslpt._capi import PT_MODE_PLAIN, PT_MODE_WAVEFRONT
 from glslpt.renderer import Renderer, options_for
 
 pytestmark = pytest.mark.gpu
 
 KINDS = ["envmap", "textures", "media", "volmis", "alpha_mask", "alpha_blend"]
-WAVEFRONT_UNSUPPORTED = {"volmis", "alpha_blend"}
 
 
 @pytest.fixture(scope="module")
@@ -33,10 +34,6 @@ def test_feature_parity(renderer, kind, mode):
     renderer.set_options(options_for(rs, mode=mode, wave_spp=2))
     renderer.set_camera(sc.camera())
     renderer.clear()
-    if mode == PT_MODE_WAVEFRONT and kind in WAVEFRONT_UNSUPPORTED:
-        with pytest.raises(PtError):
-            renderer.render(0, rs.spp)
-        return
     renderer.render(0, rs.spp)
     renderer.synchronize()
     g = renderer.read_accum()
@@ -83,11 +80,11 @@ def test_scene_file_render_parity(renderer, name):
     assert within >= 0.995
 
 
-@pytest.mark.parametrize("kind", [k for k in KINDS if k not in WAVEFRONT_UNSUPPORTED])
+@pytest.mark.parametrize("kind", KINDS)
 def test_plain_equals_wavefront_features(renderer, kind):
-    """Both kernel forms give bit-identical images on every option path they
-    share (two NEE rays per vertex with an environment map, media, alpha
-    masks, textures)."""
+    """Both kernel forms give bit-identical images on every option path (two
+    NEE rays per vertex with an environment map, media, alpha masks,
+    textures; stochastic alpha and volume MIS with NEE in the shade stage)."""
     sc, rs = scenes.feature_scene(kind)
     renderer.upload_scene(sc)
     out = []
@@ -101,3 +98,28 @@ def test_plain_equals_wavefront_features(renderer, kind):
     d = np.abs(out[0] - out[1]).max()
     print(f"{kind}: plain vs wavefront max abs diff {d}")
     assert np.array_equal(out[0], out[1])
+
+
+@pytest.mark.parametrize("kind", ["alpha_blend", "volmis"])
+def test_batch_equals_single_render_inline_nee(renderer, kind):
+    """The batched form with NEE resolved in the shade stage: three jobs of
+    the scene from their own cameras and depths, each bit-identical to a
+    single render of that job."""
+    sc, rs = scenes.feature_scene(kind)
+    renderer.upload_scene(sc)
+    cam = sc.camera()
+    jobs = [dict(camera=cam, width=48, height=32, spp=3, max_depth=4),
+            dict(camera=cam, width=40, height=40, spp=2, max_depth=6),
+            dict(camera=cam, width=64, height=24, spp=4, max_depth=2)]
+    renderer.set_options(options_for(rs, wave_spp=2, use_graph=1))
+    outs = renderer.batch_render(jobs)
+    for j, (job, out) in enumerate(zip(jobs, outs)):
+        renderer.set_options(options_for(rs, width=job["width"], height=job["height"], max_depth=job["max_depth"],
+                                         wave_spp=2))
+        renderer.set_camera(job["camera"])
+        renderer.clear()
+        renderer.render(0, job["spp"])
+        renderer.synchronize()
+        ref = renderer.read_accum()
+        print(f"{kind} job {j}: max abs diff {np.abs(out - ref).max()}")
+        assert np.array_equal(out, ref), f"{kind} job {j}"

@@ -160,16 +160,3 @@ def test_ldr_matches_oracle_tonemap(renderer, tm, aces, simple, bg):
     print(f"tm={tm} aces={aces} simple={simple} bg={bg}: max diff {d.max()}, off-by-one {(d > 0).mean():.2e}, "
           f"alpha<1 px {(acc[..., 3] < 3).mean():.3f}")
     assert d.max() <= 1 and (d > 0).mean() < 1e-3
-
-
-def test_adjoint_entry_point_refuses(renderer):
-    """The library is built with pt_adjoint switched off: the call returns
-    PT_ERR_UNSUPPORTED before any device work and leaves the output alone."""
-    from glslpt._capi import PtError
-    sc, rs = scenes.cornell(width=32, height=32, spp=1)
-    renderer.upload_scene(sc)
-    renderer.set_options(options_for(rs))
-    renderer.set_camera(sc.camera())
-    with pytest.raises(PtError) as e:
-        renderer.adjoint(0, 1, np.zeros((rs.height, rs.width, 3), np.float32))
-    assert e.value.code == -6
