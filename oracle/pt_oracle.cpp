@@ -18,10 +18,19 @@
 //   Disney lobes, sampling        closed forms (Lambert, GGX conductor, Walter dielectric),
 //                                 sample-vs-pdf histogram agreement, pdf integral,
 //                                 reciprocity of reflection lobes
-//   Light sampling                closed-form pdfs, Monte Carlo integral of the pdf
+//   Light sampling                rect/sphere points on the light from the independent RNG,
+//                                 E[1/pdf] = solid angle (rect) or hemisphere quadrature
+//                                 (sphere); distant light draws nothing
 //   PathTrace                     analytic single-bounce irradiance (diffuse plane under a
 //                                 rect light), white furnace, FP64 instantiation agreement
-//   EnvMap CDF / search           brute force on small maps
+//   EnvMap CDF / search / eval    texel = linear scan of the CDF for the same draw, corner
+//                                 colour = mean of four texels, pdf integrates to 1,
+//                                 rotation = azimuth shift, sample pdf = eval pdf
+//   Raygen (CameraRay)            tent jitter (moments, KS), aperture disc, focal-plane meet
+//   Media                         absorb / emissive closed forms across a slab, free-flight
+//                                 fraction exp(-density * chord)
+//   Alpha-tested AnyHit           float64 brute force with a block mask; blend comparison
+//   (test_oracle_sampling.py pins the last six rows)
 // Parity unpinned: texture filtering weights, GLSL inverse(), transcendental ULPs.
 #include "pt_oracle.h"
 
@@ -1339,7 +1348,8 @@ template <class T> V4<T> PathTrace(Ctx<T>& c, Ray<T> r) {
 }
 
 // ------------------------------------------------------------ tile.glsl:41-75
-template <class T> V4<T> TileSample(Ctx<T>& c, const oracle_camera* cam, int px, int py, int sample) {
+// tile.glsl:41-71: seed the pixel's RNG and generate its camera ray.
+template <class T> Ray<T> CameraRay(Ctx<T>& c, const oracle_camera* cam, int px, int py, int sample) {
     const oracle_options* o = c.o;
     // InitRNG(gl_FragCoord.xy, frameNum): seed = (x, y, frame, x + y); with one tile the
     // first accumulated frame is frameNum 2.
@@ -1367,7 +1377,11 @@ template <class T> V4<T> TileSample(Ctx<T>& c, const oracle_camera* cam, int px,
     T cam_r2 = c.rand() * T(cam->aperture);
     V3<T> randomAperturePos = (std::cos(cam_r1) * right + std::sin(cam_r1) * up) * std::sqrt(cam_r2);
     V3<T> finalRayDir = normalize(focalPoint - randomAperturePos);
-    Ray<T> ray{fetch3<T>(cam->position) + randomAperturePos, finalRayDir};
+    return Ray<T>{fetch3<T>(cam->position) + randomAperturePos, finalRayDir};
+}
+
+template <class T> V4<T> TileSample(Ctx<T>& c, const oracle_camera* cam, int px, int py, int sample) {
+    Ray<T> ray = CameraRay(c, cam, px, py, sample);
     return PathTrace(c, ray);
 }
 
@@ -1506,6 +1520,19 @@ int oracle_trace_samples(const oracle_scene* s, const oracle_camera* cam, const 
         Ctx<float> c{s, o, {0, 0, 0, 0}, nullptr};
         V4<float> col = TileSample(c, cam, px[i], py[i], sample[i]);
         rgba[4 * i] = col.x; rgba[4 * i + 1] = col.y; rgba[4 * i + 2] = col.z; rgba[4 * i + 3] = col.w;
+    }
+    return 0;
+}
+
+int oracle_camera_rays(const oracle_camera* cam, const oracle_options* o, int32_t n, const int32_t* px,
+                       const int32_t* py, const int32_t* sample, float* orig, float* dir) {
+    if (!cam || !o || n < 0 || !px || !py || !sample || !orig || !dir) return -1;
+    oracle_scene s{};
+    for (int i = 0; i < n; ++i) {
+        Ctx<float> c{&s, o, {0, 0, 0, 0}, nullptr};
+        Ray<float> r = CameraRay(c, cam, px[i], py[i], sample[i]);
+        orig[3 * i] = r.origin.x; orig[3 * i + 1] = r.origin.y; orig[3 * i + 2] = r.origin.z;
+        dir[3 * i] = r.direction.x; dir[3 * i + 1] = r.direction.y; dir[3 * i + 2] = r.direction.z;
     }
     return 0;
 }

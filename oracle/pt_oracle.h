@@ -80,6 +80,11 @@ int oracle_tonemap(const float* accum, int32_t w, int32_t h, float inv_spp, int3
 int oracle_trace_samples(const oracle_scene* s, const oracle_camera* cam, const oracle_options* o,
                          int32_t n, const int32_t* px, const int32_t* py, const int32_t* sample, float* rgba);
 
+/* Camera rays of tile.glsl:41-71 (float path): origin and direction of
+ * sample[i] of pixel (px[i], py[i]), 3 floats each. */
+int oracle_camera_rays(const oracle_camera* cam, const oracle_options* o, int32_t n, const int32_t* px,
+                       const int32_t* py, const int32_t* sample, float* orig, float* dir);
+
 /* ClosestHit / AnyHit queries (float path). Outputs as the GPU query. */
 int oracle_closest_hit(const oracle_scene* s, const oracle_options* o, const float* orig, const float* dir,
                        int32_t n, float* t, int32_t* instance, int32_t* triangle, int32_t* light, float* uv,

@@ -63,6 +63,8 @@ def lib():
                                     P(f32), P(C.c_double), i32, P(OracleCounters)]
         l.oracle_trace_samples.argtypes = [P(OracleScene), P(OracleCamera), P(OracleOptions), i32, P(i32), P(i32),
                                            P(i32), P(f32)]
+        l.oracle_camera_rays.argtypes = [P(OracleCamera), P(OracleOptions), i32, P(i32), P(i32), P(i32), P(f32),
+                                         P(f32)]
         l.oracle_closest_hit.argtypes = [P(OracleScene), P(OracleOptions), P(f32), P(f32), i32, P(f32), P(i32),
                                          P(i32), P(i32), P(f32), P(OracleCounters)]
         l.oracle_any_hit.argtypes = [P(OracleScene), P(OracleOptions), P(f32), P(f32), P(f32), i32, P(i32)]
@@ -219,6 +221,36 @@ def tonemap(accum, num_samples, tonemap=True, aces=False, simple_aces=False, bg_
     if rc != 0:
         raise RuntimeError("oracle_tonemap failed")
     return out
+
+
+def camera_rays(cam, opts, px, py, sample):
+    """Origins and directions (n, 3) of the camera rays of tile.glsl."""
+    px, py, sample = [np.ascontiguousarray(np.broadcast_to(np.asarray(a, np.int32), np.shape(px)), np.int32)
+                      for a in (px, py, sample)]
+    n = px.shape[0]
+    o = np.zeros((n, 3), np.float32)
+    d = np.zeros((n, 3), np.float32)
+    rc = lib().oracle_camera_rays(C.byref(cam), C.byref(opts), n, ip(px), ip(py), ip(sample), fp(o), fp(d))
+    assert rc == 0
+    return o, d
+
+
+def eval_envmap(sa, opts, d):
+    """EvalEnvMap for directions d (n, 3): (n, 4) of rgb and pdf."""
+    d = np.ascontiguousarray(d, np.float32)
+    out = np.zeros((d.shape[0], 4), np.float32)
+    for i in range(d.shape[0]):
+        lib().oracle_eval_envmap(C.byref(sa.s), C.byref(opts), fp(d[i]), fp(out[i]))
+    return out
+
+
+def sample_envmap(sa, opts, seed):
+    """SampleEnvMap from RNG state seed: direction and pdf (4), rgb (3), next state."""
+    s = np.asarray(seed, np.uint32).copy()
+    dp = np.zeros(4, np.float32)
+    rgb = np.zeros(3, np.float32)
+    lib().oracle_sample_envmap(C.byref(sa.s), C.byref(opts), up(s), fp(dp), fp(rgb))
+    return dp, rgb, s
 
 
 def closest_hit(sa, opts, o, d, counters=False):
