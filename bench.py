@@ -30,7 +30,16 @@ pass is the whole pt_adjoint call: a primal pass (the forward render, recording
 NEE visibility) and the replay pass that accumulates the gradients.
 
 batch: the batched form on 32 uneven Cornell-box jobs (pt_batch_render,
-graph-captured passes), reported beside the headline.
+graph-captured passes), reported beside the headline, and the same jobs run
+one after another (batch.sequential: one pt_render per job, plain and
+wavefront).
+
+cornell: the Cornell box (256x256, 4 spp, depth 5) in both kernel forms and
+on the CPU oracle.
+
+config.paths: from the instrumented frame, the fraction of camera rays that
+hit the scene and the mean number of path vertices (closest hits shaded) per
+sample.
 
 heavy: the heavy mixed scene (1000 instances of 8 seeded meshes, ~24.5M
 instanced triangles of which ~32% alpha-tested leaf quads, two scattering
@@ -127,7 +136,7 @@ def cpu_baseline(sc, rs, target_s):
     oracle.render(sa, cam, o, 1, spp, rows=(0, rs.height, step), threads=threads)
     dt = time.perf_counter() - t0
     samples = rows * rs.width * spp
-    return {"value": samples / dt / 1e6, "unit": "Msamples/s", "cores": threads, "kind": "port",
+    return {"value": samples / dt / 1e6, "unit": "Msamples/s", "cores": threads, "kind": "port", "cpu": cpu_model(),
             "sample": f"{rows} rows (every {step}th) x {rs.width} px x {spp} spp of the same frame, {dt:.1f} s"}
 
 
@@ -199,6 +208,91 @@ def batch_throughput(reps):
     best = min(ms)
     return {"workload": f"{len(jobs)} Cornell-box jobs, resolutions 128^2..1920x1080, depth 2-12, 4-32 spp",
             "samples": total, "ms": best, "msamples_per_s": total / best / 1e3, "launch": cfg}
+
+
+def batch_sequential(reps):
+    """The same 32 jobs rendered one after another, one pt_render per job
+    (plain: one kernel launch per sample; wavefront: the C3 launch
+    configuration without batching). Each job's image is bit-identical to its
+    batched image. Device time summed over the jobs, best of `reps`."""
+    from glslpt import scenes
+    from glslpt._capi import PT_MODE_PLAIN, PT_MODE_WAVEFRONT
+    from glslpt.autotune import tuned
+    from glslpt.renderer import Renderer, options_for
+    sc, rs, jobs = scenes.batch_jobs()
+    cfg = dict(block_size=64, stack_depth=16, wave_spp=8, compact_every=1)
+    cfg.update({k: v for k, v in (tuned("c3") or {}).items() if k != "use_graph"})
+    r = Renderer(0)
+    r.upload_scene(sc)
+    total = sum(j["width"] * j["height"] * j["spp"] for j in jobs)
+    out = {}
+    for name, mode in (("plain", PT_MODE_PLAIN), ("wavefront", PT_MODE_WAVEFRONT)):
+        best = None
+        for _ in range(reps + 1):  # the first round is warm-up
+            ms = 0.0
+            for j in jobs:
+                o = options_for(rs, mode=mode, **cfg)
+                o.width, o.height, o.max_depth = j["width"], j["height"], j["max_depth"]
+                r.set_options(o)
+                r.set_camera(j["camera"])
+                r.clear()
+                r.render(0, j["spp"])
+                r.synchronize()
+                ms += r.stats()["ms_last_render"]
+            best = ms if best is None else min(best, ms)
+        out[name] = {"ms": best, "msamples_per_s": total / best / 1e3}
+    r.close()
+    out["samples"] = total
+    return out
+
+
+def cornell_throughput(steps):
+    """The Cornell box (256x256, 4 spp, depth 5): both kernel forms, device
+    time per frame (median of `steps`), and the CPU oracle on the host cores."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle
+    from common import oracle_setup
+    from glslpt import scenes
+    from glslpt._capi import PT_MODE_PLAIN, PT_MODE_WAVEFRONT
+    from glslpt.renderer import Renderer, options_for
+    sc, rs = scenes.cornell()
+    samples = rs.width * rs.height * rs.spp
+    r = Renderer(0)
+    r.upload_scene(sc)
+    r.set_camera(sc.camera())
+    out = {"workload": f"Cornell box, 34 triangles, {rs.width}x{rs.height}, {rs.spp} spp, max depth {rs.max_depth}"}
+    for name, mode in (("plain", PT_MODE_PLAIN), ("wavefront", PT_MODE_WAVEFRONT)):
+        r.set_options(options_for(rs, mode=mode, wave_spp=rs.spp))
+        ms = []
+        for k in range(steps + 1):
+            r.clear()
+            r.render(k * rs.spp, rs.spp)
+            r.synchronize()
+            ms.append(r.stats()["ms_last_render"])
+        med = float(np.median(ms[1:]))
+        out[name] = {"ms": med, "msamples_per_s": samples / med / 1e3}
+    r.close()
+    sa, cam, o = oracle_setup(sc, rs)
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+    t = []
+    for k in range(3):
+        t0 = time.perf_counter()
+        oracle.render(sa, cam, o, 0, rs.spp, threads=threads)
+        t.append(time.perf_counter() - t0)
+    out["cpu_oracle"] = {"msamples_per_s": samples / float(np.median(t)) / 1e6, "cores": threads}
+    return out
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 def heavy_throughput(steps, warmup, count=True):
@@ -355,6 +449,7 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-heavy", action="store_true")
+    ap.add_argument("--no-cornell", action="store_true")
     ap.add_argument("--heavy-steps", type=int, default=2)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -391,6 +486,7 @@ def main():
                  "kernel_ms_per_step": st_p["ms_stage"]["plain"] / ps}
 
     roofline = None
+    path_stats = None
     extend_ms = st_w["ms_stage"]["extend"]
     extend_launches = max(1, st_w["launches_stage"]["extend"])
     if not args.no_count:
@@ -416,10 +512,19 @@ def main():
                                        "instances": cw["shadow_instances"] / max(1, cw["rays_shadow"])},
                     "shadow_rays_per_sample": cw["rays_shadow"] / (rs.width * rs.height * cspp),
                     "stack_spills": cw["stack_spills"]}
+        nsmp = rs.width * rs.height * cspp
+        path_stats = {"primary_hit_fraction": cw["bounce_hits"][0] / nsmp,
+                      "vertices_per_path": sum(cw["bounce_hits"]) / nsmp,
+                      "hits_per_bounce": [h / nsmp for h in cw["bounce_hits"] if h > 0]}
     r.close()
 
     adjoint = None if args.no_adjoint else adjoint_ratio(args.adjoint_reps, 16)
-    batch = None if args.no_batch else batch_throughput(3)
+    batch = None
+    if not args.no_batch:
+        batch = batch_throughput(3)
+        batch["sequential"] = batch_sequential(1)
+        batch["batched_over_sequential_plain"] = batch["msamples_per_s"] / batch["sequential"]["plain"]["msamples_per_s"]
+    cornell = None if args.no_cornell else cornell_throughput(5)
     heavy = None if args.no_heavy else heavy_throughput(args.heavy_steps, 1, count=not args.no_count)
 
     cpu = None
@@ -434,13 +539,15 @@ def main():
                                "triangles), 16 Disney materials, 2 rect + 1 sphere light + constant environment, "
                                f"{rs.width}x{rs.height}, {args.spp} spp, max depth 8, RR from depth 3",
                    "renderer": "wavefront", "launch": cfg, "samples_per_step": samples_per_step,
-                   "unique_triangles": int(a["vert_indices"].shape[0]), "instances": int(a["transforms"].shape[0])},
+                   "unique_triangles": int(a["vert_indices"].shape[0]), "instances": int(a["transforms"].shape[0]),
+                   "paths": path_stats},
         "stage_ms_per_step": {k: v / args.steps for k, v in st_w["ms_stage"].items() if v > 0},
         "plain": plain,
         "wavefront_over_plain": (value / plain["value"]) if plain else None,
         "adjoint": adjoint,
         "batch": batch,
         "heavy": heavy,
+        "cornell": cornell,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

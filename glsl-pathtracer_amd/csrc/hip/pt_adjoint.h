@@ -165,7 +165,7 @@ __device__ __forceinline__ MatD seed_material(const DevScene& sc, const SurfStat
     if (sc.num_tex > 0) {
         if (static_cast<int>(p7.x) >= 0) {
             float4 col = tex_layer(sc, st.tu, st.tv, p7.x);
-            bscale = vpow(V(col.x, col.y, col.z), 2.2f);
+            bscale = vpow_cr(V(col.x, col.y, col.z), 2.2f);
         }
         tex_mr = static_cast<int>(p7.y) >= 0;
     }

@@ -125,6 +125,14 @@ __device__ __forceinline__ vec3 refract(vec3 I, vec3 N, float eta) {
 }
 __device__ __forceinline__ vec3 vexp(vec3 a) { return V(expf(a.x), expf(a.y), expf(a.z)); }
 __device__ __forceinline__ vec3 vpow(vec3 a, float p) { return V(powf(a.x, p), powf(a.y, p), powf(a.z, p)); }
+// pow of the texture decode (texture value ^ 2.2 in GetMaterial): evaluated in
+// double and rounded once, i.e. the correctly rounded float in all but
+// vanishingly rare double-rounding cases, so host and device agree bit for bit
+// (GLSL leaves pow's precision to the driver).
+__device__ __forceinline__ float pow_cr(float x, float p) {
+    return static_cast<float>(pow(static_cast<double>(x), static_cast<double>(p)));
+}
+__device__ __forceinline__ vec3 vpow_cr(vec3 a, float p) { return V(pow_cr(a.x, p), pow_cr(a.y, p), pow_cr(a.z, p)); }
 __device__ __forceinline__ vec3 xyz(float4 a) { return V(a.x, a.y, a.z); }
 __device__ __forceinline__ float4 F4(vec3 a, float w) { return make_float4(a.x, a.y, a.z, w); }
 
@@ -631,7 +639,7 @@ __device__ Material GetMaterial(const DevScene& sc, const DevOpts& op, SurfState
     if (sc.num_tex > 0) {
         if (texX >= 0) {
             float4 col = tex_layer(sc, st.tu, st.tv, static_cast<float>(texX));
-            m.baseColor = m.baseColor * vpow(V(col.x, col.y, col.z), 2.2f);
+            m.baseColor = m.baseColor * vpow_cr(V(col.x, col.y, col.z), 2.2f);
             m.opacity = m.opacity * col.w;
         }
         if (texY >= 0) {
@@ -653,7 +661,7 @@ __device__ Material GetMaterial(const DevScene& sc, const DevOpts& op, SurfState
         m.roughness = gmax(gmix(0.0f, prev_roughness, op.moll_amt), m.roughness);
     if (sc.num_tex > 0 && texW >= 0) {
         float4 t4 = tex_layer(sc, st.tu, st.tv, static_cast<float>(texW));
-        m.emission = vpow(V(t4.x, t4.y, t4.z), 2.2f);
+        m.emission = vpow_cr(V(t4.x, t4.y, t4.z), 2.2f);
     }
     float aspect = sqrtf(1.0f - m.anisotropic * 0.9f);
     m.ax = gmax(0.001f, m.roughness / aspect);

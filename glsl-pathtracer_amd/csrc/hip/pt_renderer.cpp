@@ -407,6 +407,13 @@ void render_wavefront(pt_renderer_s* h, int first_sample, int num_samples) {
         ptk::launch_accum(h->stream, op, pb, h->accum.p, w.npix, ns);
         h->stage_end(PT_STAGE_ACCUM, ev);
         HIPCHK(hipGetLastError());
+        if (w.cw) {  // instrumented runs: hits per bounce of this pass (the counters are reused per pass)
+            std::vector<int> hits(w.D);
+            HIPCHK(hipMemcpyAsync(hits.data(), h->counters.p + (w.D + 1), w.D * sizeof(int), hipMemcpyDeviceToHost,
+                                  h->stream));
+            HIPCHK(hipStreamSynchronize(h->stream));
+            for (int b = 0; b < w.D; ++b) h->stats.bounce_hits[std::min(b, 15)] += hits[b];
+        }
     }
 }
 

@@ -84,7 +84,8 @@ class PtStats(C.Structure):
                 ("instances_entered", C.c_int64), ("lights_tested", C.c_int64),
                 ("rays_shadow", C.c_int64), ("shadow_nodes", C.c_int64), ("shadow_tris", C.c_int64),
                 ("shadow_instances", C.c_int64), ("shadow_lights", C.c_int64),
-                ("stack_spills", C.c_int64), ("stack_overflow", i32), ("adjoint_dropped", i32)]
+                ("stack_spills", C.c_int64), ("stack_overflow", i32), ("adjoint_dropped", i32),
+                ("bounce_hits", C.c_int64 * 16)]
 
 
 class PtJob(C.Structure):

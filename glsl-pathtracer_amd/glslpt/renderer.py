@@ -145,7 +145,8 @@ class Renderer:
     def stats(self):
         s = api.PtStats()
         self._check("pt_get_stats", self._lib.pt_get_stats(self._h, C.byref(s)))
-        d = {f: getattr(s, f) for f, _ in s._fields_ if f not in ("ms_stage", "launches_stage")}
+        d = {f: getattr(s, f) for f, _ in s._fields_ if f not in ("ms_stage", "launches_stage", "bounce_hits")}
+        d["bounce_hits"] = list(s.bounce_hits)
         d["ms_stage"] = {api.STAGES[i]: s.ms_stage[i] for i in range(8)}
         d["launches_stage"] = {api.STAGES[i]: s.launches_stage[i] for i in range(8)}
         return d

@@ -177,6 +177,11 @@ typedef struct pt_stats {
     int32_t stack_overflow;          /* 1 if a traversal exceeded the maximum stack depth */
     int32_t adjoint_dropped;         /* last pt_adjoint: path vertices whose gradient had a non-finite
                                         term and was left out */
+    /* count_work, wavefront form only: closest hits that entered the shade
+     * stage at bounce b (b < 15; entry 15 sums bounces 15 and later), over
+     * every pass since pt_reset_stats. hits[0] / samples is the fraction of
+     * camera rays that hit; the sum / samples the mean path length in vertices. */
+    int64_t bounce_hits[16];
 } pt_stats;
 
 /* stage indices of pt_stats.ms_stage */

@@ -88,6 +88,14 @@ template <class T> inline V3<T> refract(V3<T> I, V3<T> N, T eta) {
 }
 template <class T> inline V3<T> vexp(V3<T> a) { return V3<T>(std::exp(a.x), std::exp(a.y), std::exp(a.z)); }
 template <class T> inline V3<T> vpow(V3<T> a, T p) { return V3<T>(std::pow(a.x, p), std::pow(a.y, p), std::pow(a.z, p)); }
+// pow of the texture decode (GetMaterial, texture ^ 2.2): GLSL leaves pow's
+// precision to the driver; the float path takes the value rounded once from
+// double (correctly rounded but for vanishingly rare double roundings).
+template <class T> inline V3<T> vpow_tex(V3<T> a, T p) { return vpow(a, p); }
+template <> inline V3<float> vpow_tex<float>(V3<float> a, float p) {
+    auto f = [p](float x) { return static_cast<float>(std::pow(static_cast<double>(x), static_cast<double>(p))); };
+    return V3<float>(f(a.x), f(a.y), f(a.z));
+}
 
 // 4x4 and 3x3 matrices stored as columns (GLSL / reference Mat4 convention).
 template <class T> struct M4 { T m[4][4]; };
@@ -1080,7 +1088,7 @@ template <class T> void GetMaterialFrom(Ctx<T>& c, State<T>& state, const Ray<T>
 
     if (texIDs[0] >= 0 && s->num_textures > 0) {
         V4<T> col = tex_array<T>(s, state.texCoord, T(texIDs[0]));
-        mat.baseColor = mat.baseColor * vpow(V3<T>(col.x, col.y, col.z), T(2.2));
+        mat.baseColor = mat.baseColor * vpow_tex(V3<T>(col.x, col.y, col.z), T(2.2f));
         mat.opacity = mat.opacity * col.w;
     }
     if (texIDs[1] >= 0 && s->num_textures > 0) {
@@ -1101,7 +1109,7 @@ template <class T> void GetMaterialFrom(Ctx<T>& c, State<T>& state, const Ray<T>
         mat.roughness = gmax(gmix(T(0), state.mat.roughness, T(o->roughness_mollification_amt)), mat.roughness);
     if (texIDs[3] >= 0 && s->num_textures > 0) {
         V4<T> t4 = tex_array<T>(s, state.texCoord, T(texIDs[3]));
-        mat.emission = vpow(V3<T>(t4.x, t4.y, t4.z), T(2.2));
+        mat.emission = vpow_tex(V3<T>(t4.x, t4.y, t4.z), T(2.2f));
     }
     T aspect = std::sqrt(T(1) - mat.anisotropic * T(0.9));
     mat.ax = gmax(T(0.001), mat.roughness / aspect);
