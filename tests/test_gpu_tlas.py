@@ -68,6 +68,15 @@ def test_rebuild_sizes(renderer, n):
     check_case(renderer, "random", n, 11)
 
 
+@pytest.mark.parametrize("n", [1500, 4097])
+def test_rebuild_signed_zero_sizes(renderer, n):
+    """-0.0 box coordinates select the 64-bit rank keys; at 1500 instances the
+    arrays sit in LDS and the keys in global memory, at 4097 both in global
+    memory, so every key type and placement the launcher picks is compared."""
+    print(f"signed_zero n={n}", flush=True)
+    check_case(renderer, "signed_zero", n, 13)
+
+
 def test_repeated_rebuilds(renderer):
     """Several edits in a row, each equal to the host path from the same state."""
     n = 200
