@@ -119,6 +119,12 @@ void launch_extend_batch(hipStream_t s, int blocks, int bs, const ptd::DevScene&
 void launch_accum_batch(hipStream_t s, int blocks, const ptd::PathBuf& pb, const ptd::BatchBuf& bb);
 // Environment-map CDF and total luminance of an RGB32F image of n texels
 // (ok: one int of scratch; scratch: env_cdf_scratch_bytes(n) bytes, 8-byte aligned).
+// The parallel scan works in chunks of ENV_CDF_CHUNK texels (texels 1..n-1)
+// and keeps 36 bytes per chunk (sum, grid, function, carried state) plus 64.
+constexpr int ENV_CDF_CHUNK = 16384;
+constexpr size_t env_cdf_scratch_size(long long n) {
+    return (n > 1 ? static_cast<size_t>((n - 2) / ENV_CDF_CHUNK + 1) : 0) * 36 + 64;
+}
 size_t env_cdf_scratch_bytes(int n);
 void launch_env_cdf(hipStream_t s, const float* img, int n, float* cdf, float* total, int* ok, void* scratch);
 void launch_fill(hipStream_t s, int* p, int n, int v);

@@ -1190,10 +1190,9 @@ void launch_accum(hipStream_t s, const DevOpts& op, const PathBuf& pb, float4* a
     hipLaunchKernelGGL(k_accum, dim3((npix + 255) / 256), dim3(256), 0, s, op, pb, accum, npix, nsamp);
 }
 
-size_t env_cdf_scratch_bytes(int n) {
-    const size_t nchunks = n > 1 ? (static_cast<size_t>(n) - 2) / ZS_CHUNK + 1 : 0;
-    return nchunks * (sizeof(double) + sizeof(int) + sizeof(int4) + sizeof(int2)) + 64;
-}
+static_assert(ZS_CHUNK == ENV_CDF_CHUNK, "chunk size of the CDF scan");
+static_assert(sizeof(double) + sizeof(int) + sizeof(int4) + sizeof(int2) == 36, "CDF scan scratch per chunk");
+size_t env_cdf_scratch_bytes(int n) { return env_cdf_scratch_size(n); }
 
 void launch_env_cdf(hipStream_t s, const float* img, int n, float* cdf, float* total, int* ok, void* scratch) {
     launch_fill(s, ok, 1, 1);

@@ -744,27 +744,33 @@ int pt_device_count(int32_t* n) {
     return PT_OK;
 }
 
-// Message of the last failure that had no handle to hold it (pt_create).
-static thread_local std::string g_create_err = "no error";
+// Message of the last failure that had no handle to hold it (pt_create,
+// pt_env_cdf_noalloc), in a fixed per-thread buffer.
+static thread_local char g_handleless_err[512] = "no error";
+
+__attribute__((visibility("hidden"))) void pt_set_handleless_error(const char* msg) {
+    std::snprintf(g_handleless_err, sizeof(g_handleless_err), "%s", msg);
+}
 
 int pt_create(int32_t device, pt_handle* out) {
     if (!out) {
-        g_create_err = "pt_create: null output pointer";
+        pt_set_handleless_error("pt_create: null output pointer");
         return PT_ERR_INVALID_ARG;
     }
     *out = nullptr;
     int n = 0;
     if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
-        g_create_err = "pt_create: no HIP device visible";
+        pt_set_handleless_error("pt_create: no HIP device visible");
         return PT_ERR_NO_DEVICE;
     }
     if (device < 0 || device >= n) {
-        g_create_err = "pt_create: device " + std::to_string(device) + " out of range (" + std::to_string(n) + " visible)";
+        pt_set_handleless_error(("pt_create: device " + std::to_string(device) + " out of range (" + std::to_string(n) +
+                                 " visible)").c_str());
         return PT_ERR_INVALID_ARG;
     }
     pt_renderer_s* h = new (std::nothrow) pt_renderer_s;
     if (!h) {
-        g_create_err = "pt_create: out of host memory";
+        pt_set_handleless_error("pt_create: out of host memory");
         return PT_ERR_OVERFLOW;
     }
     try {
@@ -779,7 +785,7 @@ int pt_create(int32_t device, pt_handle* out) {
         HIPCHK(hipEventCreate(&h->ev_r1));
         pt_default_options(&h->opts);
     } catch (const std::exception& e) {
-        g_create_err = std::string("pt_create: ") + e.what();
+        pt_set_handleless_error((std::string("pt_create: ") + e.what()).c_str());
         delete h;
         return PT_ERR_HIP;
     }
@@ -794,8 +800,9 @@ int pt_destroy(pt_handle h) {
     return PT_OK;
 }
 
-// A null handle reports the last pt_create failure of this thread.
-const char* pt_last_error(pt_handle h) { return h ? h->err.c_str() : g_create_err.c_str(); }
+// A null handle reports the last failure of this thread that had no handle
+// (pt_create, pt_env_cdf_noalloc).
+const char* pt_last_error(pt_handle h) { return h ? h->err.c_str() : g_handleless_err; }
 
 // Uploads an RGB32F environment image and builds its CDF on the device;
 // returns the total luminance (the CDF's last entry).

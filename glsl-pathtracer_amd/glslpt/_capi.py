@@ -11,6 +11,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_DIR = os.path.join(_HERE, "lib")
 
 PT_OK = 0
+PT_ERR_OVERFLOW = -4
 PT_MODE_WAVEFRONT = 0
 PT_MODE_PLAIN = 1
 
@@ -152,6 +153,7 @@ _HIP_SIGS = {
     "pt_read_env_cdf": [C.c_void_p, P(f32), C.c_size_t, P(f32)],
     "pt_rebuild_instances": [C.c_void_p, P(f32), i32, P(i32)],
     "pt_read_tlas": [C.c_void_p, P(f32), i32, P(f32), i32, P(i32)],
+    "pt_env_cdf_noalloc": [P(f32), i32, i32, P(f32), P(f32), C.c_void_p],
 }
 
 # Every symbol include/glslpt.h declares, per library.

@@ -235,6 +235,19 @@ int pt_set_env_map(pt_handle h, const float* rgb, int32_t width, int32_t height)
 /* The device CDF of the current environment map (count = width*height floats)
  * and its total luminance. */
 int pt_read_env_cdf(pt_handle h, float* cdf, size_t count, float* total);
+/* pt_set_env_map's CDF build for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc); no counterpart in the reference. rgb: width*height*3 floats in
+ * device memory (rows top-down); cdf: width*height floats and total: one float,
+ * both in device memory, receive the CDF and its total luminance, bit-identical
+ * to pt_set_env_map's. The work is enqueued on `stream` (a hipStream_t; NULL =
+ * the null stream) on the current device and nothing is allocated: the scan's
+ * scratch is a fixed device array of the library (~74 KB, one per GPU, held
+ * until the process ends), shared by every call on that GPU, so calls that
+ * overlap on one GPU race. At most PT_ENV_NOALLOC_MAX_TEXELS texels
+ * (PT_ERR_OVERFLOW beyond). Errors are reported by pt_last_error(NULL). */
+#define PT_ENV_NOALLOC_MAX_TEXELS (1 << 25)
+int pt_env_cdf_noalloc(const float* rgb, int32_t width, int32_t height, float* cdf, float* total, void* stream);
 int pt_set_camera(pt_handle h, const pt_camera* c);            /* camera uniforms (:764-771) */
 int pt_clear_accum(pt_handle h);                               /* accumFBO clear (:733-744) */
 /* Adds samples [first_sample, first_sample + num_samples) of every pixel to the
