@@ -32,6 +32,18 @@ struct alignas(16) GTri {
     float4 e1;
 };
 
+// Four-wide BLAS node (the reference's BVH2 collapsed: each node holds up to
+// four of its descendants): the children's boxes as six rows of four (min x,
+// max x, min y, max y, min z, max z of children 0..3) and their references
+// (internal: index of a GNode4; leaf: -1 - (row << 2 | count) as in GNode;
+// EMPTY4: no child). Two 64-byte lines.
+struct alignas(16) GNode4 {
+    float4 lox, hix, loy, hiy, loz, hiz;
+    int4 ref;
+    int4 pad;
+};
+constexpr int EMPTY4 = static_cast<int>(0x80000000u);
+
 // Instance: inverse affine as 4 columns (world -> object), forward linear
 // part as 3 columns, BLAS root reference and material id (from the TLAS leaf).
 struct alignas(16) GInst {
@@ -40,7 +52,7 @@ struct alignas(16) GInst {
     float4 i2;  // ic2.z ic3.x ic3.y ic3.z
     float4 f0;  // fc0.x fc0.y fc0.z fc1.x
     float4 f1;  // fc1.y fc1.z fc2.x fc2.y
-    float4 f2;  // fc2.z, blas_root (int bits), material (int bits), unused
+    float4 f2;  // fc2.z, blas_root (int bits), material (int bits), four-wide blas_root (int bits)
 };
 
 // Light record (src/core/Scene.h:50-59) plus the per-light constants the
@@ -58,6 +70,7 @@ struct alignas(16) GLight {
 
 struct DevScene {
     const GNode* __restrict__ nodes;
+    const GNode4* __restrict__ nodes4;  // four-wide BLAS nodes; null: BLAS traversed in BVH2 form
     const GTri* __restrict__ tris;
     const int4* __restrict__ vidx;      // vertIndices rows (x, y, z, unused)
     const float4* __restrict__ verts;   // verticesUVX
@@ -765,7 +778,53 @@ __device__ bool traverse(const DevScene& sc, vec3 O, vec3 D, float tmax, Stack& 
     int ref = sc.root;
     for (;;) {
         bool do_pop = false;
-        if (ref >= 0) {
+        if (ref >= 0 && blas_base >= 0 && sc.nodes4) {
+            // Four-wide BLAS node: the hit children in order of the reference's
+            // key (nearest first; equal keys in child order), the nearest
+            // visited next, the others pushed farthest first.
+            if (COUNT) cnt.nodes += 2;
+            const GNode4& n = sc.nodes4[ref];
+            const float4 lx = n.lox, hx = n.hix, ly = n.loy, hy = n.hiy, lz = n.loz, hz = n.hiz;
+            const int4 rf = n.ref;
+            float t0a, t0b, t0c, t0d;
+            float ka = slab(lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, o, inv, t0a);
+            float kb = slab(lx.y, ly.y, lz.y, hx.y, hy.y, hz.y, o, inv, t0b);
+            float kc = slab(lx.z, ly.z, lz.z, hx.z, hy.z, hz.z, o, inv, t0c);
+            float kd = slab(lx.w, ly.w, lz.w, hx.w, hy.w, hz.w, o, inv, t0d);
+            const bool ha = rf.x != EMPTY4 && ka > 0.0f && !(t0a > best);
+            const bool hb = rf.y != EMPTY4 && kb > 0.0f && !(t0b > best);
+            const bool hc = rf.z != EMPTY4 && kc > 0.0f && !(t0c > best);
+            const bool hd = rf.w != EMPTY4 && kd > 0.0f && !(t0d > best);
+            const int c = int(ha) + int(hb) + int(hc) + int(hd);
+            if (c == 0) {
+                do_pop = true;
+            } else {
+                const float big = __builtin_inff();
+                float q0 = ha ? ka : big, q1 = hb ? kb : big, q2 = hc ? kc : big, q3 = hd ? kd : big;
+                int r0 = rf.x, r1 = rf.y, r2 = rf.z, r3 = rf.w;
+                // sorting network on (key, ref); a pair swaps only on a strictly smaller key
+#define PT_CSWAP(qa, ra, qb, rb)                                      \
+    if (qb < qa) {                                                    \
+        const float tq = qa; qa = qb; qb = tq;                        \
+        const int tr = ra; ra = rb; rb = tr;                          \
+    }
+                PT_CSWAP(q0, r0, q1, r1)
+                PT_CSWAP(q2, r2, q3, r3)
+                PT_CSWAP(q0, r0, q2, r2)
+                PT_CSWAP(q1, r1, q3, r3)
+                PT_CSWAP(q1, r1, q2, r2)
+#undef PT_CSWAP
+                if (sp + c - 1 > MAXD) { overflow = true; break; }
+                if (COUNT) {
+                    const int lo = sp > stk.S ? sp : stk.S, hi = sp + c - 1;
+                    if (hi > lo) cnt.spills += hi - lo;
+                }
+                if (c > 3) stk.push(sp++, r3);
+                if (c > 2) stk.push(sp++, r2);
+                if (c > 1) stk.push(sp++, r1);
+                ref = r0;
+            }
+        } else if (ref >= 0) {
             if (COUNT) cnt.nodes++;
             const GNode n = sc.nodes[ref];
             float tl0, tr0;
@@ -796,7 +855,7 @@ __device__ bool traverse(const DevScene& sc, vec3 O, vec3 D, float tmax, Stack& 
             inst_to_object(I, O, D, o, d);
             inv = V(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
             blas_base = sp;
-            ref = __float_as_int(I.f2.y);
+            ref = __float_as_int(sc.nodes4 ? I.f2.w : I.f2.y);
         } else {
             const int code = -1 - ref;
             const int first = code >> 2, count = code & 3;

@@ -126,6 +126,17 @@ constexpr size_t env_cdf_scratch_size(long long n) {
     return (n > 1 ? static_cast<size_t>((n - 2) / ENV_CDF_CHUNK + 1) : 0) * 36 + 64;
 }
 size_t env_cdf_scratch_bytes(int n);
+// Sizes of the records the launchers take from the host code, as this
+// translation unit compiled them; the host side compares them with its own
+// at pt_create, so a library linked from objects built against different
+// headers refuses to run instead of handing the kernels misread pointers.
+struct LayoutSizes { size_t scene, opts, path, shadow, batch, replay, inst, node, job; };
+LayoutSizes layout_sizes();
+inline LayoutSizes layout_sizes_here() {
+    return LayoutSizes{sizeof(ptd::DevScene), sizeof(ptd::DevOpts), sizeof(ptd::PathBuf), sizeof(ptd::ShadowBuf),
+                       sizeof(ptd::BatchBuf), sizeof(ptd::ReplayBuf), sizeof(ptd::GInst), sizeof(ptd::GNode),
+                       sizeof(ptd::JobDev)};
+}
 void launch_env_cdf(hipStream_t s, const float* img, int n, float* cdf, float* total, int* ok, void* scratch);
 void launch_fill(hipStream_t s, int* p, int n, int v);
 void launch_query_closest(hipStream_t s, const ptd::DevScene& sc, const ptd::DevOpts& op, const float* o,

@@ -35,10 +35,87 @@ struct ArgError : std::exception {
     const char* what() const noexcept override { return msg.c_str(); }
 };
 
-inline GInst make_inst(const float* m, int blas_ref, int material) {
+inline GInst make_inst(const float* m, int blas_ref, int material, int blas4_ref) {
     GInst g;
-    if (!ptd::inst_record(m, blas_ref, material, &g)) throw ArgError(PT_ERR_INVALID_ARG, "singular instance transform");
+    if (!ptd::inst_record(m, blas_ref, material, &g, blas4_ref))
+        throw ArgError(PT_ERR_INVALID_ARG, "singular instance transform");
     return g;
+}
+
+// Four-wide BLAS nodes from the BVH2 re-layout. Each four-wide node of a BVH2
+// node gathers its descendants: starting from the two children, while fewer
+// than four are held, the internal one of largest surface area (the first on
+// ties) is replaced in place by its two children, so the children stay in the
+// BVH2's left-to-right order. Leaves keep their references; a node's internal
+// children get consecutive indices when the node is converted (depth first).
+// root4 maps each BLAS root's BVH2 reference to its four-wide one.
+struct Wide4 {
+    std::vector<ptd::GNode4> nodes;
+    std::unordered_map<int, int> root4;
+};
+inline void collapse4_root(const std::vector<GNode>& n2, int root, Wide4& W) {
+    if (W.root4.count(root)) return;
+    if (root < 0) {  // a BLAS that is one leaf
+        W.root4[root] = root;
+        return;
+    }
+    struct Ch { float lo[3], hi[3]; int ref; };
+    auto kids = [&](int r, Ch& l, Ch& rr) {
+        const GNode& g = n2[r];
+        l = Ch{{g.a.x, g.a.y, g.a.z}, {g.a.w, g.b.x, g.b.y}, g.d.x};
+        rr = Ch{{g.b.z, g.b.w, g.c.x}, {g.c.y, g.c.z, g.c.w}, g.d.y};
+    };
+    auto area = [](const Ch& c) {
+        const double dx = double(c.hi[0]) - c.lo[0], dy = double(c.hi[1]) - c.lo[1], dz = double(c.hi[2]) - c.lo[2];
+        return dx * dy + dy * dz + dz * dx;
+    };
+    struct Job { int ref2, slot; };  // BVH2 node to convert, its GNode4 index
+    std::vector<Job> todo;
+    auto alloc = [&](int ref2) {
+        const int idx = static_cast<int>(W.nodes.size());
+        W.nodes.emplace_back();
+        todo.push_back({ref2, idx});
+        return idx;
+    };
+    W.root4[root] = alloc(root);
+    while (!todo.empty()) {
+        const Job jb = todo.back();
+        todo.pop_back();
+        std::vector<Ch> ch(2);
+        kids(jb.ref2, ch[0], ch[1]);
+        while (ch.size() < 4) {
+            int best = -1;
+            double ba = -1.0;
+            for (int k = 0; k < static_cast<int>(ch.size()); ++k)
+                if (ch[k].ref >= 0 && area(ch[k]) > ba) { ba = area(ch[k]); best = k; }
+            if (best < 0) break;
+            Ch l, r;
+            kids(ch[best].ref, l, r);
+            ch[best] = l;
+            ch.insert(ch.begin() + best + 1, r);
+        }
+        float lo[3][4], hi[3][4];
+        int ref[4];
+        for (int k = 0; k < 4; ++k) {
+            for (int a = 0; a < 3; ++a) {
+                lo[a][k] = k < static_cast<int>(ch.size()) ? ch[k].lo[a] : 0.f;
+                hi[a][k] = k < static_cast<int>(ch.size()) ? ch[k].hi[a] : 0.f;
+            }
+            ref[k] = k < static_cast<int>(ch.size()) ? ch[k].ref : ptd::EMPTY4;
+        }
+        // the first child is converted next (depth first)
+        for (int k = static_cast<int>(ch.size()) - 1; k >= 0; --k)
+            if (ref[k] >= 0) ref[k] = alloc(ref[k]);
+        ptd::GNode4& g = W.nodes[jb.slot];
+        g.lox = make_float4(lo[0][0], lo[0][1], lo[0][2], lo[0][3]);
+        g.hix = make_float4(hi[0][0], hi[0][1], hi[0][2], hi[0][3]);
+        g.loy = make_float4(lo[1][0], lo[1][1], lo[1][2], lo[1][3]);
+        g.hiy = make_float4(hi[1][0], hi[1][1], hi[1][2], hi[1][3]);
+        g.loz = make_float4(lo[2][0], lo[2][1], lo[2][2], lo[2][3]);
+        g.hiz = make_float4(hi[2][0], hi[2][1], hi[2][2], hi[2][3]);
+        g.ref = make_int4(ref[0], ref[1], ref[2], ref[3]);
+        g.pad = make_int4(0, 0, 0, 0);
+    }
 }
 
 // Re-layout of the reference's flat node array (bvh_translator.cpp:32-137)
@@ -50,6 +127,8 @@ struct Relayout {
     std::vector<int> inst_blas_ref, inst_material;
     std::vector<float> inst_local;  // box of each instance's BLAS root: min xyz, max xyz
     std::unordered_map<int, int> root_ref;  // flat index of each BLAS root a TLAS leaf names -> its reference
+    Wide4 wide;                             // four-wide BLAS nodes and their roots
+    std::vector<int> inst_blas4_ref;        // each instance's four-wide BLAS root reference
     // The TLAS as the device rebuild writes it: one instance per leaf, every
     // instance once, internal nodes numbered in preorder from tlas_base to the
     // end of the node array.
@@ -164,6 +243,12 @@ inline Relayout relayout(const pt_bvh_node* flat, int num_nodes, int top, int nu
         R.tlas_rebuildable = ok && leaves == num_instances && internal == num_instances - 1 && base + internal == n;
         R.tlas_base = base;
     }
+    R.inst_blas4_ref.assign(num_instances, 0);
+    for (int i = 0; i < num_instances; ++i) {
+        collapse4_root(R.nodes, R.inst_blas_ref[i], R.wide);
+        R.inst_blas4_ref[i] = R.wide.root4[R.inst_blas_ref[i]];
+    }
+    if (R.wide.nodes.empty()) R.wide.nodes.emplace_back();  // never read: keeps the array non-empty
     return R;
 }
 

@@ -85,6 +85,8 @@ struct pt_renderer_s {
 
     // scene (GPU layout)
     DevBuf<GNode> nodes;
+    DevBuf<ptd::GNode4> nodes4;            // four-wide BLAS nodes
+    std::unordered_map<int, int> root4;    // BLAS root: BVH2 reference -> four-wide reference
     DevBuf<GTri> tris;
     DevBuf<int4> vidx;
     DevBuf<float4> verts, norms, mats;
@@ -103,14 +105,14 @@ struct pt_renderer_s {
     std::vector<pt_bvh_node> flat;
     int top_level_index = 0;
     // per-instance BLAS root reference and material id (TLAS leaves)
-    std::vector<int> inst_blas, inst_mat;
+    std::vector<int> inst_blas, inst_blas4, inst_mat;
     std::unordered_map<int, int> root_ref;  // BLAS roots of the last full re-layout
     // device TLAS rebuild: inputs, scratch, and the TLAS placement
     bool tlas_rebuildable = false;
     bool tlas_on_device = false;  // the device TLAS was rebuilt after the last host node upload
     int tlas_base = 0;
     DevBuf<float> t_xf, t_local, t_box, t_cen;
-    DevBuf<int> t_blas, t_mat, t_ord, t_pre, t_pos, t_pseg;
+    DevBuf<int> t_blas, t_blas4, t_mat, t_ord, t_pre, t_pos, t_pseg;
     DevBuf<int> t_seg;
     DevBuf<unsigned long long> t_keys;
 
@@ -160,14 +162,14 @@ struct pt_renderer_s {
         if (ev_r1) (void)hipEventDestroy(ev_r1);
         DevBuf<int>* ib[] = {&qa, &qb, &hq, &counters, &spill, &ovf};
         for (auto* b : ib) b->free();
-        nodes.free(); tris.free(); vidx.free(); verts.free(); norms.free(); mats.free(); inst.free();
+        nodes.free(); nodes4.free(); tris.free(); vidx.free(); verts.free(); norms.free(); mats.free(); inst.free();
         lights.free(); tex.free(); env_img.free(); env_cdf.free(); env_total.free(); env_ok.free(); env_scratch.free(); accum.free();
         p_ro.free(); p_rd.free(); p_thr.free(); p_rad.free(); p_aux.free(); p_med.free(); p_hit.free();
         p_rng.free(); p_meta.free(); p_hit2.free(); s_idx.free(); s_o.free(); s_t.free(); s_d.free(); s_c.free();
         a_rad.free(); a_vis.free(); a_dimg.free(); a_grad.free(); a_flags.free();
         b_jobs.free(); b_planes.free(); b_segs.free(); b_passes.free(); b_cur.free(); b_accum.free();
         work_counts.free(); ldr.free();
-        t_xf.free(); t_local.free(); t_box.free(); t_cen.free(); t_blas.free(); t_mat.free(); t_ord.free();
+        t_xf.free(); t_local.free(); t_box.free(); t_cen.free(); t_blas.free(); t_blas4.free(); t_mat.free(); t_ord.free();
         t_pre.free(); t_pos.free(); t_pseg.free(); t_seg.free(); t_keys.free();
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -230,6 +232,19 @@ struct pt_renderer_s {
             HIPCHK(hipMemcpy(&f, ovf.p, sizeof(int), hipMemcpyDeviceToHost));
             stats.stack_overflow = f;
         }
+    }
+
+    // The BLAS form the kernels traverse follows pt_options.bvh_width.
+    void select_bvh() { sc.nodes4 = (opts.bvh_width == 4 && nodes4.p) ? nodes4.p : nullptr; }
+    // Four-wide BLAS roots of BVH2 BLAS root references.
+    std::vector<int> blas4_of(const std::vector<int>& blas2) const {
+        std::vector<int> out(blas2.size());
+        for (size_t i = 0; i < blas2.size(); ++i) {
+            auto it = root4.find(blas2[i]);
+            if (it == root4.end()) throw ArgError(PT_ERR_STATE, "instance names a BLAS root without four-wide nodes");
+            out[i] = it->second;
+        }
+        return out;
     }
 
     DevOpts dev_opts() const {
@@ -734,6 +749,7 @@ void pt_default_options(pt_options* o) {
     o->stack_depth = 24;
     o->wave_spp = 1;
     o->compact_every = 1;
+    o->bvh_width = 2;
 }
 
 int pt_device_count(int32_t* n) {
@@ -758,6 +774,15 @@ int pt_create(int32_t device, pt_handle* out) {
         return PT_ERR_INVALID_ARG;
     }
     *out = nullptr;
+    {
+        const ptk::LayoutSizes a = ptk::layout_sizes(), b = ptk::layout_sizes_here();
+        const ptk::TlasLayout c = ptk::tlas_layout(), d = ptk::tlas_layout_here();
+        if (std::memcmp(&a, &b, sizeof(a)) != 0 || std::memcmp(&c, &d, sizeof(c)) != 0) {
+            pt_set_handleless_error("pt_create: the library's kernels and host code were built from different "
+                                    "headers (record layouts differ); rebuild it");
+            return PT_ERR_STATE;
+        }
+    }
     int n = 0;
     if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
         pt_set_handleless_error("pt_create: no HIP device visible");
@@ -843,11 +868,13 @@ int pt_upload_scene(pt_handle h, const pt_scene_view* v) {
         ptl::make_tris(*v, tris, vidx);
         std::vector<GInst> insts(v->num_instances);
         for (int i = 0; i < v->num_instances; ++i)
-            insts[i] = make_inst(v->transforms + 16 * static_cast<size_t>(i), R.inst_blas_ref[i], R.inst_material[i]);
+            insts[i] = make_inst(v->transforms + 16 * static_cast<size_t>(i), R.inst_blas_ref[i], R.inst_material[i],
+                                 R.inst_blas4_ref[i]);
         std::vector<GLight> lights(std::max(1, v->num_lights));
         for (int i = 0; i < v->num_lights; ++i) lights[i] = make_light(v->lights[i]);
         hipStream_t s = h->stream;
         h->nodes.upload(R.nodes.data(), R.nodes.size(), s);
+        h->nodes4.upload(R.wide.nodes.data(), R.wide.nodes.size(), s);
         h->tris.upload(tris.data(), tris.size(), s);
         h->vidx.upload(vidx.data(), vidx.size(), s);
         h->verts.upload(reinterpret_cast<const float4*>(v->vertices_uvx), v->num_vertices, s);
@@ -882,6 +909,7 @@ int pt_upload_scene(pt_handle h, const pt_scene_view* v) {
         sc.env_img = h->has_env ? h->env_img.p : nullptr;
         sc.env_cdf = h->has_env ? h->env_cdf.p : nullptr;
         sc.root = R.root;
+        h->select_bvh();
         sc.num_lights = v->num_lights;
         sc.tex_w = v->tex_width; sc.tex_h = v->tex_height;
         sc.num_tex = sc.tex ? v->num_textures : 0;
@@ -891,13 +919,16 @@ int pt_upload_scene(pt_handle h, const pt_scene_view* v) {
         h->flat.assign(v->nodes, v->nodes + v->num_nodes);
         h->top_level_index = v->top_level_index;
         h->inst_blas = R.inst_blas_ref;
+        h->inst_blas4 = R.inst_blas4_ref;
         h->inst_mat = R.inst_material;
         h->root_ref = std::move(R.root_ref);
+        h->root4 = std::move(R.wide.root4);
         h->tlas_rebuildable = R.tlas_rebuildable;
         h->tlas_on_device = false;
         h->tlas_base = R.tlas_base;
         h->t_local.upload(R.inst_local.data(), R.inst_local.size(), s);
         h->t_blas.upload(R.inst_blas_ref.data(), R.inst_blas_ref.size(), s);
+        h->t_blas4.upload(R.inst_blas4_ref.data(), R.inst_blas4_ref.size(), s);
         h->t_mat.upload(R.inst_material.data(), R.inst_material.size(), s);
         h->num_tris = v->num_tris;
         h->num_instances = v->num_instances;
@@ -931,8 +962,9 @@ int pt_update_instances(pt_handle h, const float* transforms, int32_t num_instan
             for (int i = 0; i < num_instances; ++i)
                 if (T.mat[i] < 0 || T.mat[i] >= h->num_materials)
                     return Fail(h, PT_ERR_INVALID_ARG, "TLAS leaf material out of range");
+            const std::vector<int> blas4 = h->blas4_of(T.blas);
             for (int i = 0; i < num_instances; ++i)
-                insts[i] = make_inst(transforms + 16 * static_cast<size_t>(i), T.blas[i], T.mat[i]);
+                insts[i] = make_inst(transforms + 16 * static_cast<size_t>(i), T.blas[i], T.mat[i], blas4[i]);
             std::vector<float> local(6 * static_cast<size_t>(num_instances));
             for (int i = 0; i < num_instances; ++i)
                 for (int j = 0; j < 3; ++j) {
@@ -944,9 +976,11 @@ int pt_update_instances(pt_handle h, const float* transforms, int32_t num_instan
                                       hipMemcpyHostToDevice, h->stream));
             h->t_local.upload(local.data(), local.size(), h->stream);
             h->t_blas.upload(T.blas.data(), T.blas.size(), h->stream);
+            h->t_blas4.upload(blas4.data(), blas4.size(), h->stream);
             h->t_mat.upload(T.mat.data(), T.mat.size(), h->stream);
             std::copy(nodes, nodes + num_nodes, h->flat.begin() + first_node);
             h->inst_blas = T.blas;
+            h->inst_blas4 = blas4;
             h->inst_mat = T.mat;
             h->tlas_on_device = false;
         } else if (nodes) {
@@ -958,24 +992,31 @@ int pt_update_instances(pt_handle h, const float* transforms, int32_t num_instan
                 if (R.inst_material[i] < 0 || R.inst_material[i] >= h->num_materials)
                     return Fail(h, PT_ERR_INVALID_ARG, "TLAS leaf material out of range");
             for (int i = 0; i < num_instances; ++i)
-                insts[i] = make_inst(transforms + 16 * static_cast<size_t>(i), R.inst_blas_ref[i], R.inst_material[i]);
+                insts[i] = make_inst(transforms + 16 * static_cast<size_t>(i), R.inst_blas_ref[i], R.inst_material[i],
+                                     R.inst_blas4_ref[i]);
             h->nodes.upload(R.nodes.data(), R.nodes.size(), h->stream);
+            h->nodes4.upload(R.wide.nodes.data(), R.wide.nodes.size(), h->stream);
             h->t_local.upload(R.inst_local.data(), R.inst_local.size(), h->stream);
             h->t_blas.upload(R.inst_blas_ref.data(), R.inst_blas_ref.size(), h->stream);
+            h->t_blas4.upload(R.inst_blas4_ref.data(), R.inst_blas4_ref.size(), h->stream);
             h->t_mat.upload(R.inst_material.data(), R.inst_material.size(), h->stream);
             h->flat.swap(flat);
             h->inst_blas = R.inst_blas_ref;
+            h->inst_blas4 = R.inst_blas4_ref;
             h->inst_mat = R.inst_material;
             h->root_ref = std::move(R.root_ref);
+            h->root4 = std::move(R.wide.root4);
             h->tlas_rebuildable = R.tlas_rebuildable;
             h->tlas_base = R.tlas_base;
             h->tlas_on_device = false;
             h->sc.nodes = h->nodes.p;
             h->sc.root = R.root;
+            h->select_bvh();
         } else {
             // TLAS unchanged: only the instance records.
             for (int i = 0; i < num_instances; ++i)
-                insts[i] = make_inst(transforms + 16 * static_cast<size_t>(i), h->inst_blas[i], h->inst_mat[i]);
+                insts[i] = make_inst(transforms + 16 * static_cast<size_t>(i), h->inst_blas[i], h->inst_mat[i],
+                                     h->inst_blas4[i]);
         }
         h->inst.upload(insts.data(), insts.size(), h->stream);
         if (materials) h->mats.upload(reinterpret_cast<const float4*>(materials), 8 * static_cast<size_t>(num_materials), h->stream);
@@ -1024,6 +1065,7 @@ int pt_rebuild_instances(pt_handle h, const float* transforms, int32_t num_insta
         a.xf = h->t_xf.p;
         a.local = h->t_local.p;
         a.blas_ref = h->t_blas.p;
+        a.blas4_ref = h->t_blas4.p;
         a.mat = h->t_mat.p;
         a.inst = h->inst.p;
         a.nodes = h->nodes.p;
@@ -1108,9 +1150,12 @@ int pt_set_options(pt_handle h, const pt_options* o) {
         if (o->mode != PT_MODE_PLAIN && o->mode != PT_MODE_WAVEFRONT) return Fail(h, PT_ERR_INVALID_ARG, "unknown mode");
         if (o->compact_every < 0 || o->compact_every > 64) return Fail(h, PT_ERR_INVALID_ARG, "compact_every out of range");
         if (static_cast<long long>(o->width) * o->height > (1LL << 28)) return Fail(h, PT_ERR_INVALID_ARG, "resolution too large");
+        if (o->bvh_width != 0 && o->bvh_width != 2 && o->bvh_width != 4)
+            return Fail(h, PT_ERR_INVALID_ARG, "bvh_width must be 2 or 4");
         HIPCHK(hipSetDevice(h->device));
         h->opts = *o;
         h->has_opts = true;
+        h->select_bvh();
         if (o->width != h->accum_w || o->height != h->accum_h) {
             HIPCHK(hipStreamSynchronize(h->stream));
             h->accum.ensure(static_cast<size_t>(o->width) * o->height);

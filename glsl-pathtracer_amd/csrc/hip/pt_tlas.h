@@ -12,7 +12,8 @@ namespace ptd {
 // BLAS root reference and the material id. The same code runs on the host
 // (scene upload) and on the device (instance rebuild), so both give the same
 // bits. Returns false for a singular or non-finite linear part.
-__host__ __device__ __forceinline__ bool inst_record(const float* m, int blas_ref, int material, GInst* g) {
+__host__ __device__ __forceinline__ bool inst_record(const float* m, int blas_ref, int material, GInst* g,
+                                                     int blas4_ref = 0) {
     double a[3][3];  // a[col][row]
     for (int c = 0; c < 3; ++c)
         for (int r = 0; r < 3; ++r) a[c][r] = m[4 * c + r];
@@ -41,7 +42,8 @@ __host__ __device__ __forceinline__ bool inst_record(const float* m, int blas_re
     g->i2 = make_float4(ic[2][2], ic[3][0], ic[3][1], ic[3][2]);
     g->f0 = make_float4(m[0], m[1], m[2], m[4]);
     g->f1 = make_float4(m[5], m[6], m[8], m[9]);
-    g->f2 = make_float4(m[10], __builtin_bit_cast(float, blas_ref), __builtin_bit_cast(float, material), 0.f);
+    g->f2 = make_float4(m[10], __builtin_bit_cast(float, blas_ref), __builtin_bit_cast(float, material),
+                        __builtin_bit_cast(float, blas4_ref));
     return true;
 }
 
@@ -67,6 +69,7 @@ struct TlasBuild {
     const float* xf;        // [n * 16] instance transforms, column-major
     const float* local;     // [n * 6] object-space box of each instance's BLAS: min xyz, max xyz
     const int* blas_ref;    // [n] BLAS root reference
+    const int* blas4_ref;   // [n] four-wide BLAS root reference
     const int* mat;         // [n] material id
     GInst* inst;            // [n] out: instance records
     GNode* nodes;           // out: nodes [base, base + n - 1)
@@ -91,4 +94,9 @@ struct TlasBuild {
 namespace ptk {
 // Instance records, world boxes and the TLAS nodes of a rebuild (a.n >= 1).
 void launch_tlas_rebuild(hipStream_t s, const ptd::TlasBuild& a);
+// Record sizes as the rebuild's unit compiled them (compared with the host
+// code's at pt_create).
+struct TlasLayout { size_t build, inst, node; };
+TlasLayout tlas_layout();
+inline TlasLayout tlas_layout_here() { return TlasLayout{sizeof(ptd::TlasBuild), sizeof(ptd::GInst), sizeof(ptd::GNode)}; }
 }  // namespace ptk

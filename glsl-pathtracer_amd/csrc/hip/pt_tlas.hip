@@ -117,7 +117,7 @@ __global__ void k_tlas_prep(TlasBuild a) {
     if (negz) a.negz[0] = 1;
     GInst g;
     // The host validated every transform; a singular one cannot reach here.
-    (void)ptd::inst_record(m, a.blas_ref[i], a.mat[i], &g);
+    (void)ptd::inst_record(m, a.blas_ref[i], a.mat[i], &g, a.blas4_ref[i]);
     a.inst[i] = g;
 }
 
@@ -522,5 +522,7 @@ void launch_tlas_rebuild(hipStream_t s, const TlasBuild& a) {
         hipLaunchKernelGGL(k_tlas_build, dim3(1), dim3(kBuildThreads), words * sizeof(int), s, b);
     }
 }
+
+TlasLayout tlas_layout() { return tlas_layout_here(); }
 
 }  // namespace ptk

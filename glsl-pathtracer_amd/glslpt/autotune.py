@@ -1,8 +1,8 @@
 """Launch-configuration autotuner for the wavefront renderer.
 
 Sweeps workgroup size {64, 128, 256}, LDS traversal-stack depth {16, 24, 32},
-samples per wavefront pass and queue-compaction cadence on the actual
-workload, timing it on the GPU, and stores the fastest configuration as JSON
+samples per wavefront pass, queue-compaction cadence and BLAS node width
+{2, 4} on the actual workload, timing it on the GPU, and stores the fastest configuration as JSON
 in glslpt/tuning/<key>.json, where bench.py and callers look it up.
 
     python -m glslpt.autotune c2 [spp] [--quick]   # 1080p instanced spheres, pt_render
@@ -22,9 +22,11 @@ TUNING_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning")
 
 BLOCK_SIZES = (64, 128, 256)
 STACK_DEPTHS = (16, 24, 32)
-QUICK = dict(block_size=(64, 128), stack_depth=(16, 24), wave_spp=(16, 32, 64), compact_every=(1, 2))
+QUICK = dict(block_size=(64, 128), stack_depth=(16, 24), wave_spp=(16, 32, 64), compact_every=(1, 2),
+             bvh_width=(2, 4))
 WAVE_SPP = (8, 16, 32, 64)
 CADENCE = (1, 2)
+BVH_WIDTHS = (2, 4)
 
 
 def tuned(key):
@@ -50,7 +52,8 @@ def _time(r, sc, rs, spp, reps, **kw):
 
 
 def sweep(r, sc, rs, spp=16, reps=2, grid=None, log=print):
-    grid = grid or dict(block_size=BLOCK_SIZES, stack_depth=STACK_DEPTHS, wave_spp=WAVE_SPP, compact_every=CADENCE)
+    grid = grid or dict(block_size=BLOCK_SIZES, stack_depth=STACK_DEPTHS, wave_spp=WAVE_SPP, compact_every=CADENCE,
+                        bvh_width=BVH_WIDTHS)
     keys = list(grid)
     results = []
     for vals in itertools.product(*[grid[k] for k in keys]):
@@ -77,7 +80,7 @@ def _time_batch(r, rs, jobs, reps, **kw):
 
 def sweep_batch(r, rs, jobs, reps=1, grid=None, log=print):
     grid = grid or dict(block_size=BLOCK_SIZES, stack_depth=STACK_DEPTHS, wave_spp=(2, 4, 8, 16),
-                        compact_every=(1, 2, 3))
+                        compact_every=(1, 2, 3), bvh_width=BVH_WIDTHS)
     keys = list(grid)
     total = sum(j["width"] * j["height"] * j["spp"] for j in jobs)
     results = []
@@ -98,8 +101,8 @@ def main(argv):
         sc, rs, jobs = batch_jobs()
         r = Renderer(0)
         r.upload_scene(sc)
-        grid = dict(block_size=(64, 256), stack_depth=(16, 24), wave_spp=(4, 8, 16), compact_every=(1, 2)) \
-            if "--quick" in argv else None
+        grid = dict(block_size=(64, 256), stack_depth=(16, 24), wave_spp=(4, 8, 16), compact_every=(1, 2),
+                    bvh_width=BVH_WIDTHS) if "--quick" in argv else None
         res = sweep_batch(r, rs, jobs, reps=2, grid=grid)
         os.makedirs(TUNING_DIR, exist_ok=True)
         with open(os.path.join(TUNING_DIR, "c3.json"), "w") as f:

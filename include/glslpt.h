@@ -163,6 +163,8 @@ typedef struct pt_options {
     int32_t use_graph;       /* capture the bounce stages of a pass in a HIP graph */
     int32_t count_work;      /* instrumented kernels: count nodes / triangles / instances */
     int32_t compact_every;   /* compact the path queue every k-th bounce (1 = every bounce) */
+    int32_t bvh_width;       /* 2: BLAS traversed as the reference's BVH2; 4: as four-wide nodes
+                                collapsed from it (same hits but for exact ties) */
 } pt_options;
 
 typedef struct pt_stats {

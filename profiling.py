@@ -3,6 +3,8 @@
     python profiling.py copy NBYTES
     python profiling.py heavy [SPP]
     python profiling.py edits
+    python profiling.py l2read [MB] [ROWS] [REPS]
+    python profiling.py c2 [SPP]
 
 copy: copies NBYTES device-to-device with hipMemcpy (one blit-kernel dispatch
 that reads and writes NBYTES). Profiled with `rocprofv3 --pmc FETCH_SIZE`
@@ -21,6 +23,18 @@ k_env_chunk_fun, k_env_carry, k_env_chunk_write, and the k_env_scan fallback)
 four times, then twenty pt_rebuild_instances of its 1003 instances with
 alternating transforms (k_tlas_prep, k_tlas_build). Prints the wall time of
 each call.
+
+l2read: the L2 read rate of this GPU, measured. A float32 tensor of MB MiB
+(default 2: inside one XCD's 4 MiB L2) is summed over a stride-0 expansion of
+ROWS rows (default 512), so each launch reads the same MB MiB ROWS times and,
+after the first touch, every read is served by the L2 (a CU's L1 holds far less
+than its share). REPS launches (default 50) timed with HIP events; prints the
+read bytes per launch and GB/s. Profiled with `rocprofv3 --pmc TCC_REQ_sum`, the
+same run gives the bytes one L2 request carries (read bytes / requests).
+
+c2: one wavefront frame of SPP samples (default 16) of the 1080p headline
+workload at the stored launch configuration; a kernel filter on "k_extend"
+selects its closest-hit launches.
 """
 import ctypes as C
 import os
@@ -111,6 +125,52 @@ def edits():
                       "tlas": {"instances": int(xf0.shape[0]), "rebuild_instances_ms": tlas_ms}}))
 
 
+def l2read(mb=2, rows=512, reps=50):
+    import json
+
+    import torch
+    n = mb * (1 << 20) // 4
+    x = torch.ones(n, device="cuda", dtype=torch.float32)
+    y = x.unsqueeze(0).expand(rows, n)
+    for _ in range(3):
+        y.sum()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        y.sum()
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    nbytes = rows * n * 4
+    print(json.dumps({"bytes_per_launch": nbytes, "ms_per_launch": ms, "gb_per_s": nbytes / ms / 1e6,
+                      "mb": mb, "rows": rows, "reps": reps}))
+
+
+def c2(spp=16):
+    import json
+    from glslpt._capi import PT_MODE_WAVEFRONT
+    from glslpt.autotune import tuned
+    from glslpt.renderer import Renderer, options_for
+    from glslpt.scenes import instanced_spheres
+    sc, rs = instanced_spheres()
+    cfg = dict(block_size=256, stack_depth=24, wave_spp=4, compact_every=1)
+    cfg.update(tuned("c2") or {})
+    r = Renderer(0)
+    r.upload_scene(sc)
+    r.set_options(options_for(rs, mode=PT_MODE_WAVEFRONT, **cfg))
+    r.set_camera(sc.camera())
+    r.clear()
+    r.reset_stats()
+    r.render(0, spp)
+    r.synchronize()
+    st = r.stats()
+    r.close()
+    print(json.dumps({"spp": spp, "launch": cfg, "ms_last_render": st["ms_last_render"],
+                      "stage_ms": {k: v for k, v in st["ms_stage"].items() if v > 0},
+                      "launches": {k: v for k, v in st["launches_stage"].items() if v > 0}}))
+
+
 if __name__ == "__main__":
     if len(sys.argv) == 3 and sys.argv[1] == "copy":
         copy(int(sys.argv[2]))
@@ -118,5 +178,9 @@ if __name__ == "__main__":
         heavy(int(sys.argv[2]) if len(sys.argv) == 3 else 4)
     elif len(sys.argv) == 2 and sys.argv[1] == "edits":
         edits()
+    elif len(sys.argv) >= 2 and sys.argv[1] == "l2read":
+        l2read(*[int(a) for a in sys.argv[2:5]])
+    elif len(sys.argv) in (2, 3) and sys.argv[1] == "c2":
+        c2(int(sys.argv[2]) if len(sys.argv) == 3 else 16)
     else:
-        sys.exit("usage: python -m glslpt.profiling copy NBYTES | heavy [SPP] | edits")
+        sys.exit("usage: python profiling.py copy NBYTES | heavy [SPP] | edits | l2read [MB] [ROWS] [REPS] | c2 [SPP]")

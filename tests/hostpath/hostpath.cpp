@@ -29,7 +29,8 @@ struct HostScene {
         ptl::make_tris(*v, tris, vidx);
         insts.resize(v->num_instances);
         for (int i = 0; i < v->num_instances; ++i)
-            insts[i] = ptl::make_inst(v->transforms + 16 * (size_t)i, R.inst_blas_ref[i], R.inst_material[i]);
+            insts[i] = ptl::make_inst(v->transforms + 16 * (size_t)i, R.inst_blas_ref[i], R.inst_material[i],
+                                      R.inst_blas4_ref[i]);
         lights.resize(std::max(1, v->num_lights));
         for (int i = 0; i < v->num_lights; ++i) lights[i] = ptl::make_light(v->lights[i]);
         bool any_alpha = false, any_medium = false;
@@ -39,6 +40,7 @@ struct HostScene {
         }
         bool has_env = v->env_width > 0 && v->env_height > 0 && v->env_img && v->env_cdf;
         sc.nodes = R.nodes.data(); sc.tris = tris.data(); sc.vidx = vidx.data();
+        sc.nodes4 = o->bvh_width == 4 ? R.wide.nodes.data() : nullptr;
         sc.verts = reinterpret_cast<const float4*>(v->vertices_uvx);
         sc.norms = reinterpret_cast<const float4*>(v->normals_uvy);
         sc.mats = reinterpret_cast<const float4*>(v->materials);

@@ -65,12 +65,16 @@ def test_scene_error_codes():
 
 
 def test_renderer_without_gpu_fails_loudly():
+    """Without a GPU pt_create reports PT_ERR_NO_DEVICE; it first compares the
+    record layouts the kernels' units and the host code were compiled with,
+    which would return PT_ERR_STATE for a library linked from mismatched
+    builds, so reaching PT_ERR_NO_DEVICE also shows that the layouts agree."""
     lib = api.hip_lib()
     n = C.c_int32(-1)
     assert lib.pt_device_count(C.byref(n)) == 0
     if n.value == 0:
         h = C.c_void_p()
-        assert lib.pt_create(0, C.byref(h)) == -7  # PT_ERR_NO_DEVICE
+        assert lib.pt_create(0, C.byref(h)) == -7, lib.pt_last_error(None)  # PT_ERR_NO_DEVICE
         from glslpt.renderer import Renderer
         with pytest.raises(api.PtError):
             Renderer(0)

@@ -1,5 +1,5 @@
 """The renderer's per-path device code compiled for the CPU (tests/hostpath)
-against the oracle, on the CPU: the kernels' arithmetic, expression by
+against the oracle, on the CPU (BLAS as BVH2 and as four-wide nodes): the kernels' arithmetic, expression by
 expression, is the oracle's, so with the same math library the images are
 bit-identical on every option path (the GPU differs from the oracle only
 where the device math library rounds a transcendental function differently).
@@ -35,14 +35,18 @@ KINDS = ["cornell", "instanced", "boxes", "heavy", "envmap", "textures", "media"
          "alpha_blend"]
 
 
+@pytest.mark.parametrize("width", [2, 4])
 @pytest.mark.parametrize("kind", KINDS)
-def test_device_path_on_host_equals_oracle(kind):
+def test_device_path_on_host_equals_oracle(kind, width):
+    """width 4: the BLAS traversed as four-wide nodes collapsed from the
+    reference's BVH2 (nearest child first), which reaches the same closest
+    hits; on these scenes no ray ties, so the images stay bit-identical."""
     sc, rs = _scene(kind)
-    h = hostpath.render(sc, rs, 0, rs.spp)
+    h = hostpath.render(sc, rs, 0, rs.spp, bvh_width=width)
     sa, cam, o = oracle_setup(sc, rs)
     r = oracle.render(sa, cam, o, 0, rs.spp)
     same = (h == r).all(axis=2).mean()
-    print(f"{kind}: {rs.width}x{rs.height} x{rs.spp} spp, bit-identical pixels {same:.6f}")
+    print(f"{kind} bvh{width}: {rs.width}x{rs.height} x{rs.spp} spp, bit-identical pixels {same:.6f}")
     assert np.array_equal(h, r)
 
 
