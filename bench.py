@@ -29,10 +29,10 @@ samples for an L2 loss against a target rendered from a second parameter seed
 pass is the whole pt_adjoint call: a primal pass (the forward render, recording
 NEE visibility) and the replay pass that accumulates the gradients.
 
-batch: the batched form on 32 uneven Cornell-box jobs (pt_batch_render,
-graph-captured passes), reported beside the headline, and the same jobs run
-one after another (batch.sequential: one pt_render per job, plain and
-wavefront).
+batch: 32 uneven Cornell-box jobs in one pt_batch_render, in the wavefront
+form (graph-captured passes) and the one-launch plain form, reported beside
+the headline, and the same jobs run one after another (batch.sequential: one
+pt_render per job, plain and wavefront).
 
 cornell: the Cornell box (256x256, 4 spp, depth 5) in both kernel forms and
 on the CPU oracle.
@@ -187,9 +187,11 @@ def adjoint_ratio(reps, wave_spp):
 
 def batch_throughput(reps):
     """The 32 uneven Cornell jobs (resolutions 128^2..1920x1080, depth 2-12,
-    4-32 spp) in one pt_batch_render with graph-captured passes; device time,
-    best of `reps` after a warm-up."""
+    4-32 spp) in one pt_batch_render: the wavefront form (graph-captured
+    passes, the c3 launch configuration) and the one-launch plain form; device
+    time, best of `reps` after a warm-up."""
     from glslpt import scenes
+    from glslpt._capi import PT_MODE_PLAIN, PT_MODE_WAVEFRONT
     from glslpt.autotune import tuned
     from glslpt.renderer import Renderer, options_for
     sc, rs, jobs = scenes.batch_jobs()
@@ -197,17 +199,21 @@ def batch_throughput(reps):
     cfg.update(tuned("c3") or {})
     r = Renderer(0)
     r.upload_scene(sc)
-    r.set_options(options_for(rs, **cfg))
-    r.batch_render(jobs)
-    ms = []
-    for _ in range(reps):
-        r.batch_render(jobs)
-        ms.append(r.stats()["ms_last_render"])
-    r.close()
     total = sum(j["width"] * j["height"] * j["spp"] for j in jobs)
-    best = min(ms)
-    return {"workload": f"{len(jobs)} Cornell-box jobs, resolutions 128^2..1920x1080, depth 2-12, 4-32 spp",
-            "samples": total, "ms": best, "msamples_per_s": total / best / 1e3, "launch": cfg}
+    out = {"workload": f"{len(jobs)} Cornell-box jobs, resolutions 128^2..1920x1080, depth 2-12, 4-32 spp",
+           "samples": total}
+    for name, mode in (("wavefront", PT_MODE_WAVEFRONT), ("plain", PT_MODE_PLAIN)):
+        r.set_options(options_for(rs, mode=mode, **cfg))
+        r.batch_render(jobs)
+        ms = []
+        for _ in range(reps):
+            r.batch_render(jobs)
+            ms.append(r.stats()["ms_last_render"])
+        best = min(ms)
+        out[name] = {"ms": best, "msamples_per_s": total / best / 1e3}
+    out["launch"] = cfg
+    r.close()
+    return out
 
 
 def batch_sequential(reps):
@@ -312,8 +318,11 @@ def heavy_throughput(steps, warmup, count=True):
     inst_tris = sum(c * 20 * f * f for c, f in zip(scenes.HEAVY_COUNTS, scenes.HEAVY_SPHERE_FREQS))
     leaf_tris = sum(c * 2 * q for c, q in zip(scenes.HEAVY_COUNTS[len(scenes.HEAVY_SPHERE_FREQS):],
                                               scenes.HEAVY_LEAF_QUADS))
-    cfg = dict(block_size=256, stack_depth=24, wave_spp=4, compact_every=1)
+    cfg = dict(block_size=256, stack_depth=24, wave_spp=4, compact_every=1, bvh_width=2)
     cfg.update(tuned("c2") or {})
+    # four-wide BLAS nodes: same hits but for exact ties, and 6% faster on
+    # this scene's deep BLASes (closest-hit 670 -> 633 ms per 4-spp frame)
+    cfg["bvh_width"] = 4
     r = Renderer(0)
     r.upload_scene(sc)
     samples = rs.width * rs.height * rs.spp
@@ -441,6 +450,7 @@ def main():
     ap.add_argument("--block-size", type=int, default=None)
     ap.add_argument("--stack-depth", type=int, default=None)
     ap.add_argument("--compact-every", type=int, default=None)
+    ap.add_argument("--bvh-width", type=int, default=None, choices=(2, 4))
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-count", action="store_true")
@@ -467,7 +477,7 @@ def main():
     r.upload_scene(sc)
     # launch configuration: explicit flags, else the autotuner's stored choice
     from glslpt.autotune import tuned
-    cfg = dict(block_size=256, stack_depth=24, wave_spp=4, compact_every=1)
+    cfg = dict(block_size=256, stack_depth=24, wave_spp=4, compact_every=1, bvh_width=2)
     cfg.update(tuned("c2") or {})
     for k in cfg:
         if getattr(args, k) is not None:
@@ -523,7 +533,9 @@ def main():
     if not args.no_batch:
         batch = batch_throughput(3)
         batch["sequential"] = batch_sequential(1)
-        batch["batched_over_sequential_plain"] = batch["msamples_per_s"] / batch["sequential"]["plain"]["msamples_per_s"]
+        seq = batch["sequential"]["plain"]["msamples_per_s"]
+        batch["batched_plain_over_sequential_plain"] = batch["plain"]["msamples_per_s"] / seq
+        batch["batched_wavefront_over_sequential_plain"] = batch["wavefront"]["msamples_per_s"] / seq
     cornell = None if args.no_cornell else cornell_throughput(5)
     heavy = None if args.no_heavy else heavy_throughput(args.heavy_steps, 1, count=not args.no_count)
 

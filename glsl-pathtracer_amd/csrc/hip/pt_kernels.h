@@ -55,6 +55,7 @@ struct JobDev {
     float cam_pos[3], cam_up[3], cam_right[3], cam_fwd[3];
     float cam_fov, cam_focal, cam_aperture;
     int width, height, max_depth, npix;  // npix: pixel items (8x8 tiles * 64)
+    int spp, item0;                      // samples; first pixel item in the one-launch plain form
     long long accum_off;                 // float4 offset of the job's image in the batch accumulation
 };
 // One sample of one job: npix consecutive path slots of a pass starting at slot0.
@@ -78,7 +79,7 @@ struct BatchBuf {
 
 namespace ptk {
 enum { KIND_PLAIN = 0, KIND_EXTEND = 1, KIND_SHADE = 2, KIND_SHADOW = 3, KIND_SHADE_BATCH = 4,
-       KIND_SHADE_INLINE = 5, KIND_SHADE_BATCH_INLINE = 6, KIND_REPLAY = 7 };
+       KIND_SHADE_INLINE = 5, KIND_SHADE_BATCH_INLINE = 6, KIND_REPLAY = 7, KIND_PLAIN_BATCH = 8 };
 
 // Resident blocks for a persistent launch (occupancy x CUs); extra_lds is the
 // replay stage's per-block gradient table.
@@ -117,6 +118,11 @@ void launch_extend_batch(hipStream_t s, int blocks, int bs, const ptd::DevScene&
                          const ptd::PathBuf& pb, const ptd::BatchBuf& bb, int* hitq, int* hit_count, int* work, int S,
                          int* spill, int* ovf);
 void launch_accum_batch(hipStream_t s, int blocks, const ptd::PathBuf& pb, const ptd::BatchBuf& bb);
+// Batched render in one persistent launch of the plain form (jobs sorted by
+// item0, `items` = the sum of their npix).
+void launch_plain_batch(hipStream_t s, int blocks, int bs, const ptd::DevScene& sc, const ptd::DevOpts& op,
+                        const ptd::JobDev* jobs, int njobs, int items, float4* accum, int* work, int S, int* spill,
+                        int* ovf);
 // Environment-map CDF and total luminance of an RGB32F image of n texels
 // (ok: one int of scratch; scratch: env_cdf_scratch_bytes(n) bytes, 8-byte aligned).
 // The parallel scan works in chunks of ENV_CDF_CHUNK texels (texels 1..n-1)

@@ -20,14 +20,13 @@ __device__ __forceinline__ int lane_id() { return __lane_id(); }
 // static range. All waves of a grid share that one counter, and a wave waits
 // for the contended atomic's return before its next batch: claimed one batch
 // at a time by the camera-ray stage at 1080p it took half of the stage's time.
-// A launch whose input is the implicit slot range [0, count) therefore takes
-// every batch statically (wave w of W: batches w, w + W, ...) and touches no
-// counter. Stages whose items cost about the same (shade, replay) claim long
-// runs; those whose items vary with the traversal (extend, shadow, plain)
-// claim one batch at a time, since a long run left at the end of the queue
-// holds the launch's tail (measured at 1080p: shade 8.2 -> 5.6 ms from runs
-// of 1 to 16, shadow 6.5 -> 9.2 ms).
-constexpr int RUN_TRAVERSE = 1, RUN_SHADE = 16;
+// Runs trade that against the tail a long run left at the end of the queue
+// holds. Measured at 1080p (ms per frame): camera rays in runs of 4, 20.9 ->
+// 19.4 in the closest-hit stage against a static assignment (which gives a
+// wave tiles far apart); shade, whose items cost about the same, 8.2 -> 5.6
+// from runs of 1 to 16; shadow and later closest-hit bounces, whose items vary
+// with the traversal, best one batch at a time (shadow 6.5 -> 9.2 at 16).
+constexpr int RUN_CAMERA = 4, RUN_TRAVERSE = 1, RUN_SHADE = 16;
 __device__ __forceinline__ int static_total() { return static_cast<int>(gridDim.x * (blockDim.x >> 6)) * 64; }
 __device__ __forceinline__ int first_batch() {
     return static_cast<int>(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * 64;
@@ -44,7 +43,6 @@ struct WorkCursor {
         base = __builtin_amdgcn_readfirstlane(b) + static_total();
         end = base + 64 * RUN;
     }
-    __device__ __forceinline__ void next_static() { base += static_total(); }
 };
 
 // Wave-level stream compaction: one atomic per wave, lanes write at their
@@ -209,7 +207,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))
     TravCount cnt{0, 0, 0, 0, 0, 0};
     bool ovf = false;
     const int count = PRIMARY ? (BATCH ? bb.cur[3] : npaths) : *count_in;
-    for (WorkCursor<RUN_TRAVERSE> wc; wc.base < count; PRIMARY ? wc.next_static() : wc.next(work)) {
+    for (WorkCursor<PRIMARY ? RUN_CAMERA : RUN_TRAVERSE> wc; wc.base < count; wc.next(work)) {
         const int i = wc.base + lane_id();
         bool keep = false;
         int slot = -1;
@@ -492,6 +490,54 @@ __global__ void __launch_bounds__(256) k_replay_shade(DevScene sc, DevOpts op, P
         for (int j = threadIdx.x; j < ng; j += blockDim.x)
             if (gacc[j] != 0.0f) atomicAdd(rb.grad + j, gacc[j]);
     }
+}
+
+// Batched render in one persistent launch of the plain form. A work item is
+// one pixel of one job (8x8 tiles, so a wave's batch of 64 lies in one job);
+// its thread traces the pixel's samples 0..spp-1 in order and stores their
+// sum, the float sequence pt_render adds into its accumulation buffer, so
+// every job's image equals the job rendered alone. Paths stay in registers
+// from camera ray to termination: no path state is written between bounces.
+// Two waves per SIMD (the occupancy of k_plain): left to itself the allocator
+// takes 4 registers more than that allows.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
+k_plain_batch(DevScene sc, DevOpts op0, const JobDev* jobs, int njobs, int items, float4* __restrict__ accum,
+              int* work, int S, int* spill, int* ovf_flag) {
+    extern __shared__ int lds[];
+    Stack stk{lds, spill, static_cast<int>(threadIdx.x), static_cast<int>(blockDim.x), S,
+              blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x,
+              static_cast<size_t>(gridDim.x) * blockDim.x};
+    TravCount cnt{0, 0, 0, 0, 0, 0};
+    bool ovf = false;
+    for (WorkCursor<RUN_TRAVERSE> wc; wc.base < items; wc.next(work)) {
+        int lo = 0, hi = njobs - 1;  // the last job whose first item is <= the batch's
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (jobs[mid].item0 <= wc.base) lo = mid;
+            else hi = mid - 1;
+        }
+        const JobDev* Jp = jobs + lo;
+        const int width = Jp->width, height = Jp->height, spp = Jp->spp;
+        const int item = wc.base - Jp->item0 + lane_id();
+        int px, py;
+        item_pixel(item, width, px, py);
+        if (item < Jp->npix && px < width && py < height) {
+            float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+            for (int smp = 0; smp < spp; ++smp) {
+                // The camera is read again for each sample, after the previous
+                // trace (jobs is not restrict, so the trace's stores keep the
+                // loads here): only max_depth stays live through the trace.
+                PathRegs p;
+                primary_path(job_opts(op0, *Jp), px, py, smp, p);
+                DevOpts op = op0;
+                op.max_depth = Jp->max_depth;
+                const float4 c = path_trace<false>(sc, op, p, stk, cnt, ovf);
+                acc.x += c.x; acc.y += c.y; acc.z += c.z; acc.w += c.w;
+            }
+            accum[Jp->accum_off + static_cast<long long>(py) * width + px] = acc;
+        }
+    }
+    if (ovf) atomicOr(ovf_flag, 1);
 }
 
 // Accumulate: accum[pixel] += radiance of samples s = 0..nsamp-1 in order.
@@ -1121,6 +1167,7 @@ int persistent_blocks(int kind, int bs, int S, bool count, int num_cu, size_t ex
     int per_cu = 1;
     switch (kind) {
         case KIND_PLAIN: per_cu = count ? max_blocks(k_plain<true>, bs, lds) : max_blocks(k_plain<false>, bs, lds); break;
+        case KIND_PLAIN_BATCH: per_cu = max_blocks(k_plain_batch, bs, lds); break;
         case KIND_EXTEND: per_cu = count ? max_blocks(k_extend<true, false, false>, bs, lds) : max_blocks(k_extend<false, false, false>, bs, lds); break;
         case KIND_SHADOW: per_cu = count ? max_blocks(k_shadow<true>, bs, lds) : max_blocks(k_shadow<false>, bs, lds); break;
         case KIND_SHADE: per_cu = max_blocks(k_shade<false, false, false>, bs, 0); break;
@@ -1195,6 +1242,13 @@ void launch_extend_batch(hipStream_t s, int blocks, int bs, const DevScene& sc, 
 
 void launch_accum_batch(hipStream_t s, int blocks, const PathBuf& pb, const BatchBuf& bb) {
     hipLaunchKernelGGL(k_accum_batch, dim3(blocks), dim3(256), 0, s, pb, bb);
+}
+
+void launch_plain_batch(hipStream_t s, int blocks, int bs, const DevScene& sc, const DevOpts& op, const JobDev* jobs,
+                        int njobs, int items, float4* accum, int* work, int S, int* spill, int* ovf) {
+    const size_t lds = static_cast<size_t>(S) * bs * sizeof(int);
+    hipLaunchKernelGGL(k_plain_batch, dim3(blocks), dim3(bs), lds, s, sc, op, jobs, njobs, items, accum, work, S, spill,
+                       ovf);
 }
 
 void launch_shadow(hipStream_t s, int blocks, int bs, const DevScene& sc, const DevOpts& op, const PathBuf& pb,

@@ -35,7 +35,7 @@ __device__ void camera_ray(const DevOpts& op, int px, int py, Rng& rng, vec3& ro
     vec3 focalPoint = op.cam_focal * rayDir;
     float cam_r1 = rnd(rng) * TWO_PI;
     float cam_r2 = rnd(rng) * op.cam_aperture;
-    vec3 ap = (cosf(cam_r1) * right + sinf(cam_r1) * up) * sqrtf(cam_r2);
+    vec3 ap = (cos_pt(cam_r1) * right + sin_pt(cam_r1) * up) * sqrtf(cam_r2);
     rd = normalize(focalPoint - ap);
     ro = V(op.cam_pos[0], op.cam_pos[1], op.cam_pos[2]) + ap;
 }
@@ -379,9 +379,10 @@ __device__ __forceinline__ void nee_rays(const DevScene& sc, const DevOpts& op, 
     if (op.flags & OPT_LIGHTS) nee[1] = nee_setup(sc, op, 1, p0.rd, st, m, eta, rng);
 }
 
-// Full PathTrace for one sample (megakernel form).
+// Full PathTrace for one sample (megakernel form). Inlined into each caller:
+// called out of line it spills the path's registers to scratch.
 template <bool COUNT>
-__device__ float4 path_trace(const DevScene& sc, const DevOpts& op, PathRegs& p, Stack& stk, TravCount& cnt,
+__device__ __forceinline__ float4 path_trace(const DevScene& sc, const DevOpts& op, PathRegs& p, Stack& stk, TravCount& cnt,
                              bool& ovf) {
     SurfState st;
     init_surf(st);

@@ -517,6 +517,7 @@ struct BatchPlan {
     std::vector<ptd::SegDev> segs;
     std::vector<ptd::PassDev> passes;
     size_t pixels = 0;    // sum of W*H over jobs
+    long long items = 0;  // sum of npix over jobs (pixel items of the one-launch plain form)
     int max_paths = 0;    // largest pass
     int max_depth = 0;
 };
@@ -544,12 +545,16 @@ BatchPlan plan_batch(const pt_job* jobs, int n, long long budget) {
         d.height = J.height;
         d.max_depth = J.max_depth;
         d.npix = ((J.width + 7) / 8) * ((J.height + 7) / 8) * 64;
+        d.spp = J.spp;
+        d.item0 = static_cast<int>(P.items);
         d.accum_off = static_cast<long long>(P.pixels);
+        P.items += d.npix;
         P.pixels += static_cast<size_t>(J.width) * J.height;
         P.jobs.push_back(d);
         max_npix = std::max<long long>(max_npix, d.npix);
         P.max_depth = std::max(P.max_depth, J.max_depth);
     }
+    if (P.items > INT32_MAX / 2) throw ArgError(PT_ERR_OVERFLOW, "too many pixels in the batch");
     budget = std::max(budget, max_npix);
     if (budget > INT32_MAX / 2) throw ArgError(PT_ERR_OVERFLOW, "too many paths per pass");
     ptd::PassDev cur{};
@@ -590,7 +595,38 @@ BatchPlan plan_batch(const pt_job* jobs, int n, long long budget) {
     return P;
 }
 
+// Plain form of pt_batch_render: every job in one persistent launch of the
+// plain kernel (k_plain_batch), each thread one pixel, its samples in order.
+void batch_render_plain(pt_renderer_s* h, const pt_job* jobs, int n) {
+    const DevOpts op = h->dev_opts();
+    const BatchPlan P = plan_batch(jobs, n, 0);
+    hipStream_t s = h->stream;
+    h->b_jobs.upload(P.jobs.data(), P.jobs.size(), s);
+    h->b_accum.ensure(P.pixels);
+    h->counters.ensure(std::max<size_t>(h->counters.n, 8));
+    h->ensure_misc();
+    const int bs = h->block_size(), S = h->stack_S();
+    const int blocks = ptk::persistent_blocks(ptk::KIND_PLAIN_BATCH, bs, S, false, h->num_cu);
+    h->ensure_spill(static_cast<size_t>(blocks) * bs);
+    ptk::launch_fill(s, h->counters.p, 1, 0);
+    hipEvent_t ev;
+    h->stage_begin(PT_STAGE_BATCH, ev);
+    ptk::launch_plain_batch(s, blocks, bs, h->sc, op, h->b_jobs.p, n, static_cast<int>(P.items), h->b_accum.p,
+                            h->counters.p, S, h->spill.p, h->ovf.p);
+    h->stage_end(PT_STAGE_BATCH, ev);
+    HIPCHK(hipGetLastError());
+    for (int j = 0; j < n; ++j) {
+        const ptd::JobDev& J = P.jobs[j];
+        HIPCHK(hipMemcpyAsync(jobs[j].accum_out, h->b_accum.p + J.accum_off,
+                              static_cast<size_t>(J.width) * J.height * sizeof(float4), hipMemcpyDeviceToHost, s));
+    }
+}
+
 void batch_render(pt_renderer_s* h, const pt_job* jobs, int n) {
+    if (h->opts.mode == PT_MODE_PLAIN) {
+        batch_render_plain(h, jobs, n);
+        return;
+    }
     DevOpts op = h->dev_opts();
     const bool inl = inline_nee(h, op);
     // pass budget: wave_spp samples of the largest job

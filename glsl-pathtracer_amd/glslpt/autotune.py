@@ -5,9 +5,10 @@ samples per wavefront pass, queue-compaction cadence and BLAS node width
 {2, 4} on the actual workload, timing it on the GPU, and stores the fastest configuration as JSON
 in glslpt/tuning/<key>.json, where bench.py and callers look it up.
 
-    python -m glslpt.autotune c2 [spp] [--quick]   # 1080p instanced spheres, pt_render
-    python -m glslpt.autotune c3 [--quick]         # 32 uneven Cornell jobs, pt_batch_render
-                                                   # with graph-captured passes
+    python -m glslpt.autotune c2 [spp] [--quick] [--out DIR]   # 1080p instanced spheres, pt_render
+    python -m glslpt.autotune c3 [--quick] [--out DIR]         # 32 uneven Cornell jobs, pt_batch_render
+                                                               # with graph-captured passes
+--out writes the JSON to DIR instead of glslpt/tuning/.
 """
 import itertools
 import json
@@ -95,7 +96,11 @@ def sweep_batch(r, rs, jobs, reps=1, grid=None, log=print):
 
 def main(argv):
     if len(argv) < 1 or argv[0] not in ("c2", "c3"):
-        sys.exit("usage: python -m glslpt.autotune c2 [spp] [--quick] | c3 [--quick]")
+        sys.exit("usage: python -m glslpt.autotune c2 [spp] [--quick] [--out DIR] | c3 [--quick] [--out DIR]")
+    out_dir = TUNING_DIR
+    if "--out" in argv:
+        out_dir = argv[argv.index("--out") + 1]
+        argv = [a for i, a in enumerate(argv) if a != "--out" and (i == 0 or argv[i - 1] != "--out")]
     if argv[0] == "c3":
         from .scenes import batch_jobs
         sc, rs, jobs = batch_jobs()
@@ -104,8 +109,8 @@ def main(argv):
         grid = dict(block_size=(64, 256), stack_depth=(16, 24), wave_spp=(4, 8, 16), compact_every=(1, 2),
                     bvh_width=BVH_WIDTHS) if "--quick" in argv else None
         res = sweep_batch(r, rs, jobs, reps=2, grid=grid)
-        os.makedirs(TUNING_DIR, exist_ok=True)
-        with open(os.path.join(TUNING_DIR, "c3.json"), "w") as f:
+        os.makedirs(out_dir, exist_ok=True)
+        with open(os.path.join(out_dir, "c3.json"), "w") as f:
             json.dump({"key": "c3", "best": dict(res[0]["config"], use_graph=1), "results": res}, f, indent=1)
         print("best", json.dumps(res[0]))
         return
@@ -116,9 +121,9 @@ def main(argv):
     r = Renderer(0)
     r.upload_scene(sc)
     res = sweep(r, sc, rs, spp=spp, reps=1, grid=grid)
-    os.makedirs(TUNING_DIR, exist_ok=True)
+    os.makedirs(out_dir, exist_ok=True)
     out = {"key": "c2", "spp_timed": spp, "best": res[0]["config"], "results": res}
-    with open(os.path.join(TUNING_DIR, "c2.json"), "w") as f:
+    with open(os.path.join(out_dir, "c2.json"), "w") as f:
         json.dump(out, f, indent=1)
     print("best", json.dumps(res[0]))
 

@@ -275,14 +275,16 @@ int pt_any_hit(pt_handle h, const float* origins, const float* dirs, const float
 
 /* ---- batched uneven jobs: one launch sequence over many render jobs ---- */
 /* Renders every job (own camera, resolution, samples [0, spp) and max depth;
- * scene and the other options shared) in passes that mix jobs: the path slots
- * of a pass are (job, sample, pixel) planes, packed up to wave_spp samples
- * worth of slots of the largest job.
- * With pt_options.use_graph the stage launches of one pass are captured once
- * in a HIP graph and the graph is launched once per pass. Each job's image is
- * identical to rendering it alone with pt_render (same seeds, same order of
- * accumulation). No reference counterpart: the reference renders one scene
- * view at a time (Renderer::Render, src/core/Renderer.cpp:546-590). */
+ * scene and the other options shared). Wavefront mode: passes that mix jobs;
+ * the path slots of a pass are (job, sample, pixel) planes, packed up to
+ * wave_spp samples worth of slots of the largest job, and with
+ * pt_options.use_graph the stage launches of one pass are captured once in a
+ * HIP graph launched once per pass. Plain mode: one persistent launch of the
+ * plain kernel over every job's pixels, each thread tracing one pixel's
+ * samples in order. Either way each job's image is identical to rendering it
+ * alone with pt_render (same seeds, same order of accumulation). No reference
+ * counterpart: the reference renders one scene view at a time
+ * (Renderer::Render, src/core/Renderer.cpp:546-590). */
 typedef struct pt_job {
     pt_camera camera;
     int32_t width, height, spp, max_depth;

@@ -1,11 +1,13 @@
 """Batched render of uneven jobs (pt_batch_render): every job's image is
 identical to rendering that job alone with pt_render (same seeds, same order
-of accumulation), whatever the pass packing, launch shape or graph capture;
-pt_render itself is pinned to the CPU oracle by test_gpu_parity.py."""
+of accumulation), whatever the pass packing, launch shape or graph capture,
+in the wavefront form and in the one-launch plain form; pt_render itself is
+pinned to the CPU oracle by test_gpu_parity.py."""
 import numpy as np
 import pytest
 
 from glslpt import scenes
+from glslpt._capi import PT_MODE_PLAIN, PT_MODE_WAVEFRONT
 from glslpt.renderer import Renderer, options_for
 
 pytestmark = pytest.mark.gpu
@@ -71,4 +73,31 @@ def test_batch_full_size_equals_single_renders(renderer):
     print(f"32 jobs, {total / 1e6:.1f} Msamples in {ms:.1f} ms = {total / ms / 1e3:.0f} Msamples/s")
     for j, (job, out) in enumerate(zip(jobs, outs)):
         ref = single(renderer, sc, rs, job, wave_spp=job["spp"], block_size=64, stack_depth=16)
+        assert np.array_equal(out, ref), f"job {j}"
+
+
+@pytest.mark.parametrize("bs,stack", [(256, 24), (64, 16), (128, 2)])
+def test_plain_batch_equals_single_renders(renderer, bs, stack):
+    """The one-launch plain form (each thread one pixel, its samples in order)
+    against single wavefront renders of each job."""
+    sc, rs, jobs = small_jobs()
+    renderer.upload_scene(sc)
+    renderer.set_options(options_for(rs, mode=PT_MODE_PLAIN, block_size=bs, stack_depth=stack))
+    outs = renderer.batch_render(jobs)
+    for j, (job, out) in enumerate(zip(jobs, outs)):
+        ref = single(renderer, sc, rs, job, mode=PT_MODE_WAVEFRONT, wave_spp=2, block_size=bs, stack_depth=stack)
+        assert np.array_equal(out, ref), f"job {j}: max diff {np.abs(out - ref).max()}"
+
+
+def test_plain_batch_full_size_equals_single_renders(renderer):
+    """The full 32-job batch in one plain launch, against single plain renders."""
+    sc, rs, jobs = scenes.batch_jobs()
+    renderer.upload_scene(sc)
+    renderer.set_options(options_for(rs, mode=PT_MODE_PLAIN, block_size=64, stack_depth=16))
+    outs = renderer.batch_render(jobs)
+    ms = renderer.stats()["ms_last_render"]
+    total = sum(j["width"] * j["height"] * j["spp"] for j in jobs)
+    print(f"plain batch: 32 jobs, {total / 1e6:.1f} Msamples in {ms:.1f} ms = {total / ms / 1e3:.0f} Msamples/s")
+    for j, (job, out) in enumerate(zip(jobs, outs)):
+        ref = single(renderer, sc, rs, job, mode=PT_MODE_PLAIN, block_size=64, stack_depth=16)
         assert np.array_equal(out, ref), f"job {j}"
