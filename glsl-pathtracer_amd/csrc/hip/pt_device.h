@@ -146,6 +146,32 @@ __device__ __forceinline__ float pow_cr(float x, float p) {
     return static_cast<float>(pow(static_cast<double>(x), static_cast<double>(p)));
 }
 __device__ __forceinline__ vec3 vpow_cr(vec3 a, float p) { return V(pow_cr(a.x, p), pow_cr(a.y, p), pow_cr(a.z, p)); }
+// sin and cos of the path code's angles (all within [-2pi, 2pi]; the
+// reduction holds for |x| < 2^12). GLSL leaves their precision to the driver,
+// and device and host math libraries round some arguments differently, which
+// near-specular lobes turn into visible differences, so both sides use this
+// one sequence of float operations: k = nearest integer to x * 2/pi;
+// r = x - k * pi/2 by a four-part Cody-Waite subtraction (the first three
+// parts hold 8, 12 and 12 significant bits, so k times each is exact); the
+// Taylor series of sin r to r^11 and of cos r to r^12 in Horner form in r^2
+// (truncation below 1e-11 for |r| <= pi/4); the quadrant k mod 4 selects
+// +-sin r or +-cos r. Within 1.6 ulp of the exact values.
+__host__ __device__ __forceinline__ void sincos_pt(float x, float& sn, float& cs) {
+    // 2/pi; pi/2 = 0x1.92p+0 + 0x1.fb6p-12 - 0x1.778p-25 + 0x1.68c234p-39;
+    // 1/3!, 1/5!, ..., 1/11! and 1/4!, 1/6!, ..., 1/12!, each rounded to float
+    const float k = floorf(x * 0x1.45f306p-1f + 0.5f);
+    const float r = (((x - k * 0x1.92p+0f) - k * 0x1.fb6p-12f) - k * -0x1.778p-25f) - k * 0x1.68c234p-39f;
+    const float t = r * r;
+    const float sp = r + r * t * (-0x1.555556p-3f + t * (0x1.111112p-7f + t * (-0x1.a01a02p-13f +
+                                 t * (0x1.71de3ap-19f - t * 0x1.ae6456p-26f))));
+    const float cp = (1.0f - 0.5f * t) + t * t * (0x1.555556p-5f + t * (-0x1.6c16c2p-10f + t * (0x1.a01a02p-16f +
+                                                  t * (-0x1.27e4fcp-22f + t * 0x1.1eed8ep-29f))));
+    const int q = static_cast<int>(k) & 3;
+    sn = q == 0 ? sp : q == 1 ? cp : q == 2 ? -sp : -cp;
+    cs = q == 0 ? cp : q == 1 ? -sp : q == 2 ? -cp : sp;
+}
+__host__ __device__ __forceinline__ float sin_pt(float x) { float s, c; sincos_pt(x, s, c); return s; }
+__host__ __device__ __forceinline__ float cos_pt(float x) { float s, c; sincos_pt(x, s, c); return c; }
 __device__ __forceinline__ vec3 xyz(float4 a) { return V(a.x, a.y, a.z); }
 __device__ __forceinline__ float4 F4(vec3 a, float w) { return make_float4(a.x, a.y, a.z, w); }
 
@@ -248,7 +274,7 @@ __device__ vec3 SampleGTR1(float rgh, float r1, float r2) {
     float phi = r1 * TWO_PI;
     float cosTheta = sqrtf((1.0f - powf(a2, 1.0f - r2)) / (1.0f - a2));
     float sinTheta = gclamp(sqrtf(1.0f - (cosTheta * cosTheta)), 0.0f, 1.0f);
-    float sinPhi = sinf(phi), cosPhi = cosf(phi);
+    float sinPhi = sin_pt(phi), cosPhi = cos_pt(phi);
     return V(sinTheta * cosPhi, sinTheta * sinPhi, cosTheta);
 }
 __device__ vec3 SampleGGXVNDF(vec3 Vv, float ax, float ay, float r1, float r2) {
@@ -258,8 +284,8 @@ __device__ vec3 SampleGGXVNDF(vec3 Vv, float ax, float ay, float r1, float r2) {
     vec3 T2 = cross(Vh, T1);
     float r = sqrtf(r1);
     float phi = 2.0f * PI * r2;
-    float t1 = r * cosf(phi);
-    float t2 = r * sinf(phi);
+    float t1 = r * cos_pt(phi);
+    float t2 = r * sin_pt(phi);
     float s = 0.5f * (1.0f + Vh.z);
     t2 = (1.0f - s) * sqrtf(1.0f - t1 * t1) + s * t2;
     vec3 Nh = t1 * T1 + t2 * T2 + sqrtf(gmax(0.0f, 1.0f - t1 * t1 - t2 * t2)) * Vh;
@@ -299,15 +325,15 @@ __device__ vec3 CosineSampleHemisphere(float r1, float r2) {
     vec3 dir;
     float r = sqrtf(r1);
     float phi = TWO_PI * r2;
-    dir.x = r * cosf(phi);
-    dir.y = r * sinf(phi);
+    dir.x = r * cos_pt(phi);
+    dir.y = r * sin_pt(phi);
     dir.z = sqrtf(gmax(0.0f, 1.0f - dir.x * dir.x - dir.y * dir.y));
     return dir;
 }
 __device__ vec3 UniformSampleHemisphere(float r1, float r2) {
     float r = sqrtf(gmax(0.0f, 1.0f - r1 * r1));
     float phi = TWO_PI * r2;
-    return V(r * cosf(phi), r * sinf(phi), r1);
+    return V(r * cos_pt(phi), r * sin_pt(phi), r1);
 }
 __device__ __forceinline__ float PowerHeuristic(float a, float b) {
     float t = a * a;
@@ -327,7 +353,7 @@ __device__ vec3 SampleHG(vec3 Vv, float g, float r1, float r2) {
     }
     float phi = r1 * TWO_PI;
     float sinTheta = gclamp(sqrtf(1.0f - (cosTheta * cosTheta)), 0.0f, 1.0f);
-    float sinPhi = sinf(phi), cosPhi = cosf(phi);
+    float sinPhi = sin_pt(phi), cosPhi = cos_pt(phi);
     vec3 v1, v2;
     Onb(Vv, v1, v2);
     return sinTheta * cosPhi * v1 + sinTheta * sinPhi * v2 + cosTheta * Vv;
@@ -405,7 +431,7 @@ __device__ float4 EvalEnvMap(const DevScene& sc, const DevOpts& op, vec3 dir) {
     float v = theta * INV_PI;
     vec3 color = env_lookup(sc, u, v);
     float pdf = Luminance(color) / sc.env_total;
-    return F4(color, (pdf * static_cast<float>(sc.env_w) * static_cast<float>(sc.env_h)) / (TWO_PI * PI * sinf(theta)));
+    return F4(color, (pdf * static_cast<float>(sc.env_w) * static_cast<float>(sc.env_h)) / (TWO_PI * PI * sin_pt(theta)));
 }
 __device__ float4 SampleEnvMap(const DevScene& sc, const DevOpts& op, Rng& rng, vec3& color) {
     float u, v;
@@ -415,9 +441,9 @@ __device__ float4 SampleEnvMap(const DevScene& sc, const DevOpts& op, Rng& rng, 
     u = u - op.env_rot;
     float phi = u * TWO_PI;
     float theta = v * PI;
-    if (sinf(theta) == 0.0f) pdf = 0.0f;
-    return make_float4(-sinf(theta) * cosf(phi), cosf(theta), -sinf(theta) * sinf(phi),
-                       (pdf * static_cast<float>(sc.env_w) * static_cast<float>(sc.env_h)) / (TWO_PI * PI * sinf(theta)));
+    if (sin_pt(theta) == 0.0f) pdf = 0.0f;
+    return make_float4(-sin_pt(theta) * cos_pt(phi), cos_pt(theta), -sin_pt(theta) * sin_pt(phi),
+                       (pdf * static_cast<float>(sc.env_w) * static_cast<float>(sc.env_h)) / (TWO_PI * PI * sin_pt(theta)));
 }
 
 // ------------------------------------------------------------------ disney.glsl:37-351

@@ -31,7 +31,10 @@
 //                                 fraction exp(-density * chord)
 //   Alpha-tested AnyHit           float64 brute force with a block mask; blend comparison
 //   (test_oracle_sampling.py pins the last six rows)
-// Parity unpinned: texture filtering weights, GLSL inverse(), transcendental ULPs.
+//   sin / cos (float)             double sin/cos within 1.6 ulp over [-64, 64], symmetry,
+//                                 exact zero
+// Parity unpinned: texture filtering weights, GLSL inverse(), the ULPs of the
+// other transcendental functions (pow, exp, log, acos, atan2, tan).
 #include "pt_oracle.h"
 
 #include <omp.h>
@@ -96,6 +99,38 @@ template <> inline V3<float> vpow_tex<float>(V3<float> a, float p) {
     auto f = [p](float x) { return static_cast<float>(std::pow(static_cast<double>(x), static_cast<double>(p))); };
     return V3<float>(f(a.x), f(a.y), f(a.z));
 }
+
+// sin and cos of the path's angles (sampling azimuths, the aperture angle, the
+// environment map's theta and phi; all within [-2pi, 2pi]). GLSL leaves their
+// precision to the driver, so the float path fixes one sequence of float
+// operations: k = floor(x * 2/pi + 0.5); r = x - k * pi/2 with pi/2 split in
+// four floats (8, 12, 12 and 24 significant bits; the first three products
+// exact for |k| < 2^12); Taylor series in r^2 to degree 11 (sin) and 12 (cos);
+// quadrant k mod 4. The double path calls std::sin / std::cos.
+// Pinned by tests/test_oracle_sampling.py against double sin/cos (<= 1.6 ulp).
+inline void sincos_f(float x, float& sn, float& cs) {
+    const float k = std::floor(x * 0x1.45f306p-1f + 0.5f);
+    const float p1 = 0x1.92p+0f, p2 = 0x1.fb6p-12f, p3 = -0x1.778p-25f, p4 = 0x1.68c234p-39f;
+    const float r = (((x - k * p1) - k * p2) - k * p3) - k * p4;
+    const float t = r * r;
+    // 1/n! rounded to float
+    const float s3 = 0x1.555556p-3f, s5 = 0x1.111112p-7f, s7 = 0x1.a01a02p-13f, s9 = 0x1.71de3ap-19f,
+                s11 = 0x1.ae6456p-26f;
+    const float c4 = 0x1.555556p-5f, c6 = 0x1.6c16c2p-10f, c8 = 0x1.a01a02p-16f, c10 = 0x1.27e4fcp-22f,
+                c12 = 0x1.1eed8ep-29f;
+    const float ps = r + r * t * (-s3 + t * (s5 + t * (-s7 + t * (s9 - t * s11))));
+    const float pc = (1.0f - 0.5f * t) + t * t * (c4 + t * (-c6 + t * (c8 + t * (-c10 + t * c12))));
+    switch (static_cast<int>(k) & 3) {
+        case 0: sn = ps; cs = pc; break;
+        case 1: sn = pc; cs = -ps; break;
+        case 2: sn = -ps; cs = -pc; break;
+        default: sn = -pc; cs = ps; break;
+    }
+}
+template <class T> inline T tsin(T x) { return std::sin(x); }
+template <class T> inline T tcos(T x) { return std::cos(x); }
+template <> inline float tsin<float>(float x) { float s, c; sincos_f(x, s, c); return s; }
+template <> inline float tcos<float>(float x) { float s, c; sincos_f(x, s, c); return c; }
 
 // 4x4 and 3x3 matrices stored as columns (GLSL / reference Mat4 convention).
 template <class T> struct M4 { T m[4][4]; };
@@ -365,7 +400,7 @@ template <class T> V3<T> SampleGTR1(T rgh, T r1, T r2) {
     T phi = r1 * K<T>::TWO_PI;
     T cosTheta = std::sqrt((T(1) - std::pow(a2, T(1) - r2)) / (T(1) - a2));
     T sinTheta = gclamp(std::sqrt(T(1) - (cosTheta * cosTheta)), T(0), T(1));
-    T sinPhi = std::sin(phi), cosPhi = std::cos(phi);
+    T sinPhi = tsin(phi), cosPhi = tcos(phi);
     return V3<T>(sinTheta * cosPhi, sinTheta * sinPhi, cosTheta);
 }
 template <class T> V3<T> SampleGGXVNDF(V3<T> V, T ax, T ay, T r1, T r2) {
@@ -375,8 +410,8 @@ template <class T> V3<T> SampleGGXVNDF(V3<T> V, T ax, T ay, T r1, T r2) {
     V3<T> T2 = cross(Vh, T1);
     T r = std::sqrt(r1);
     T phi = T(2) * K<T>::PI * r2;
-    T t1 = r * std::cos(phi);
-    T t2 = r * std::sin(phi);
+    T t1 = r * tcos(phi);
+    T t2 = r * tsin(phi);
     T s = T(0.5) * (T(1) + Vh.z);
     t2 = (T(1) - s) * std::sqrt(T(1) - t1 * t1) + s * t2;
     V3<T> Nh = t1 * T1 + t2 * T2 + std::sqrt(gmax(T(0), T(1) - t1 * t1 - t2 * t2)) * Vh;
@@ -416,15 +451,15 @@ template <class T> V3<T> CosineSampleHemisphere(T r1, T r2) {
     V3<T> dir;
     T r = std::sqrt(r1);
     T phi = K<T>::TWO_PI * r2;
-    dir.x = r * std::cos(phi);
-    dir.y = r * std::sin(phi);
+    dir.x = r * tcos(phi);
+    dir.y = r * tsin(phi);
     dir.z = std::sqrt(gmax(T(0), T(1) - dir.x * dir.x - dir.y * dir.y));
     return dir;
 }
 template <class T> V3<T> UniformSampleHemisphere(T r1, T r2) {
     T r = std::sqrt(gmax(T(0), T(1) - r1 * r1));
     T phi = K<T>::TWO_PI * r2;
-    return V3<T>(r * std::cos(phi), r * std::sin(phi), r1);
+    return V3<T>(r * tcos(phi), r * tsin(phi), r1);
 }
 template <class T> T PowerHeuristic(T a, T b) {
     T t = a * a;
@@ -493,7 +528,7 @@ template <class T> V3<T> SampleHG(V3<T> V, T g, T r1, T r2) {
     }
     T phi = r1 * K<T>::TWO_PI;
     T sinTheta = gclamp(std::sqrt(T(1) - (cosTheta * cosTheta)), T(0), T(1));
-    T sinPhi = std::sin(phi), cosPhi = std::cos(phi);
+    T sinPhi = tsin(phi), cosPhi = tcos(phi);
     V3<T> v1, v2;
     Onb(V, v1, v2);
     return sinTheta * cosPhi * v1 + sinTheta * sinPhi * v2 + cosTheta * V;
@@ -533,7 +568,7 @@ template <class T> V4<T> EvalEnvMap(const oracle_scene* s, const oracle_options*
     V3<T> color(c4.x, c4.y, c4.z);
     T pdf = Luminance(color) / T(s->env_total_sum);
     return V4<T>{color.x, color.y, color.z,
-                 (pdf * T(s->env_width) * T(s->env_height)) / (K<T>::TWO_PI * K<T>::PI * std::sin(theta))};
+                 (pdf * T(s->env_width) * T(s->env_height)) / (K<T>::TWO_PI * K<T>::PI * tsin(theta))};
 }
 template <class T> V4<T> SampleEnvMap(Ctx<T>& c, V3<T>& color) {
     const oracle_scene* s = c.s;
@@ -544,9 +579,9 @@ template <class T> V4<T> SampleEnvMap(Ctx<T>& c, V3<T>& color) {
     uv.x = uv.x - T(c.o->envmap_rot);
     T phi = uv.x * K<T>::TWO_PI;
     T theta = uv.y * K<T>::PI;
-    if (std::sin(theta) == T(0)) pdf = T(0);
-    return V4<T>{-std::sin(theta) * std::cos(phi), std::cos(theta), -std::sin(theta) * std::sin(phi),
-                 (pdf * T(s->env_width) * T(s->env_height)) / (K<T>::TWO_PI * K<T>::PI * std::sin(theta))};
+    if (tsin(theta) == T(0)) pdf = T(0);
+    return V4<T>{-tsin(theta) * tcos(phi), tcos(theta), -tsin(theta) * tsin(phi),
+                 (pdf * T(s->env_width) * T(s->env_height)) / (K<T>::TWO_PI * K<T>::PI * tsin(theta))};
 }
 
 // ------------------------------------------------------------ disney.glsl:37-351
@@ -1383,7 +1418,7 @@ template <class T> Ray<T> CameraRay(Ctx<T>& c, const oracle_camera* cam, int px,
     V3<T> focalPoint = T(cam->focalDist) * rayDir;
     T cam_r1 = c.rand() * K<T>::TWO_PI;
     T cam_r2 = c.rand() * T(cam->aperture);
-    V3<T> randomAperturePos = (std::cos(cam_r1) * right + std::sin(cam_r1) * up) * std::sqrt(cam_r2);
+    V3<T> randomAperturePos = (tcos(cam_r1) * right + tsin(cam_r1) * up) * std::sqrt(cam_r2);
     V3<T> finalRayDir = normalize(focalPoint - randomAperturePos);
     return Ray<T>{fetch3<T>(cam->position) + randomAperturePos, finalRayDir};
 }
@@ -1463,9 +1498,9 @@ extern "C" {
 void oracle_set_inverse_mode(int32_t mode) { g_inverse_mode = mode; }
 
 int oracle_eval_fn(int32_t fn, int32_t n, const float* in, float* out) {
-    static const int IN[] = {2, 5, 2, 5, 2, 1, 2, 7, 3, 2, 2, 6, 2, 10, 19, 12};
-    static const int OUT[] = {1, 1, 1, 1, 1, 1, 1, 3, 3, 3, 3, 3, 1, 1, 1, 1};
-    if (fn < 0 || fn > 15) return -1;
+    static const int IN[] = {2, 5, 2, 5, 2, 1, 2, 7, 3, 2, 2, 6, 2, 10, 19, 12, 1};
+    static const int OUT[] = {1, 1, 1, 1, 1, 1, 1, 3, 3, 3, 3, 3, 1, 1, 1, 1, 2};
+    if (fn < 0 || fn > 16) return -1;
     const int ni = IN[fn], no = OUT[fn];
     for (int i = 0; i < n; ++i) {
         const float* a = in + static_cast<size_t>(i) * ni;
@@ -1501,6 +1536,7 @@ int oracle_eval_fn(int32_t fn, int32_t n, const float* in, float* out) {
                 o[0] = AABBIntersect(fetch3<float>(a), fetch3<float>(a + 3), r);
                 break;
             }
+            case 16: o[0] = tsin(a[0]); o[1] = tcos(a[0]); break;
         }
         if (no == 3) { o[0] = v.x; o[1] = v.y; o[2] = v.z; }
     }

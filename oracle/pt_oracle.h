@@ -103,7 +103,7 @@ void oracle_set_inverse_mode(int32_t mode);
  *  8 SampleGTR1(rgh, r1, r2)->3  9 CosineSampleHemisphere(r1, r2)->3
  * 10 UniformSampleHemisphere(r1, r2)->3  11 SampleHG(V3, g, r1, r2)->3  12 PhaseHG(cos, g)
  * 13 SphereIntersect(rad, pos3, o3, d3)  14 RectIntersect(pos3, u3, v3, plane4, o3, d3)
- * 15 AABBIntersect(min3, max3, o3, d3) */
+ * 15 AABBIntersect(min3, max3, o3, d3)  16 sincos(x)->2 (the path's float sin, cos) */
 int oracle_eval_fn(int32_t fn, int32_t n, const float* in, float* out);
 
 /* RNG known-answer access: n successive pcg4d states from seed. */

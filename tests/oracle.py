@@ -87,8 +87,8 @@ def lib():
 
 FN = dict(GTR1=0, GTR2Aniso=1, SmithG=2, SmithGAniso=3, DielectricFresnel=4, SchlickWeight=5, PowerHeuristic=6,
           SampleGGXVNDF=7, SampleGTR1=8, CosineSampleHemisphere=9, UniformSampleHemisphere=10, SampleHG=11,
-          PhaseHG=12, SphereIntersect=13, RectIntersect=14, AABBIntersect=15)
-_FN_OUT = {7: 3, 8: 3, 9: 3, 10: 3, 11: 3}
+          PhaseHG=12, SphereIntersect=13, RectIntersect=14, AABBIntersect=15, sincos=16)
+_FN_OUT = {7: 3, 8: 3, 9: 3, 10: 3, 11: 3, 16: 2}
 
 
 def fn(name, *cols):

@@ -5,8 +5,8 @@
   rays, and every ray whose ids differ is a tie (both hits at the same
   distance within 1e-5 relative); hit t within 1e-5 relative; any hit
   identical on >= 99.99% of rays;
-* rendered images within 1e-4 relative per pixel on the same share of pixels
-  as the BVH2 form (floors below), image means within 1e-4 relative;
+* rendered images within 1e-4 relative on every pixel, image means within
+  1e-4 relative;
 * the plain and wavefront forms bit-identical to each other.
 """
 import numpy as np
@@ -98,16 +98,14 @@ def _render(renderer, sc, rs, mode, spp):
     return renderer.read_accum()
 
 
-# (scene, pixel floor, image-mean tolerance): the floors and the heavy scene's
-# mean tolerance are those of the BVH2 form's tests (stochastic alpha at
-# bilinear mask edges, parity unpinned at GL filtering, moves whole paths)
+# (scene, pixel floor, image-mean tolerance): every pixel, as the BVH2 form
 CASES = {
-    "cornell": (lambda: scenes.cornell(), 0.9999, 1e-4),
-    "instanced": (lambda: small_instanced(), 0.9999, 1e-4),
-    "heavy": (lambda: small_heavy(), 0.995, 1e-2),
-    "media": (lambda: scenes.feature_scene("media"), 0.9999, 1e-4),
-    "envmap": (lambda: scenes.feature_scene("envmap"), 0.9999, 1e-4),
-    "alpha_mask": (lambda: scenes.feature_scene("alpha_mask"), 0.9995, 1e-4),
+    "cornell": (lambda: scenes.cornell(), 1.0, 1e-4),
+    "instanced": (lambda: small_instanced(), 1.0, 1e-4),
+    "heavy": (lambda: small_heavy(), 1.0, 1e-4),
+    "media": (lambda: scenes.feature_scene("media"), 1.0, 1e-4),
+    "envmap": (lambda: scenes.feature_scene("envmap"), 1.0, 1e-4),
+    "alpha_mask": (lambda: scenes.feature_scene("alpha_mask"), 1.0, 1e-4),
 }
 
 

@@ -3,10 +3,11 @@ same seeded inputs.
 
 Tolerances (stated by the workload spec): closest-hit primitive and instance
 ids identical on >= 99.99% of rays (ties aside), hit t within 1e-5 relative;
-per-pixel radiance within 1e-4 relative on almost every pixel (a path whose
-Russian-roulette or lobe draw lands within an ulp of its threshold follows
-another branch; such pixels are counted and bounded), image means within
-1e-4 relative.
+per-pixel radiance within 1e-4 relative on every pixel (kernels and oracle
+run the same float operations, sin and cos included; a pixel could differ
+only where a transcendental library call rounds differently and a path's
+Russian-roulette or lobe draw sits within an ulp of its threshold: none does
+on these scenes), image means within 1e-4 relative.
 """
 import numpy as np
 import pytest
@@ -85,7 +86,7 @@ def test_any_hit_instanced(renderer):
     assert agree >= 0.9999
 
 
-def _render_parity(renderer, sc, rs, mode, spp, label, px_tol=0.995):
+def _render_parity(renderer, sc, rs, mode, spp, label, px_tol=1.0):
     renderer.upload_scene(sc)
     renderer.set_options(options_for(rs, mode=mode, wave_spp=2))
     renderer.set_camera(sc.camera())
@@ -97,12 +98,12 @@ def _render_parity(renderer, sc, rs, mode, spp, label, px_tol=0.995):
     r = oracle.render(sa, cam, o, 0, spp)
     e = rel_err(g[..., :3], r[..., :3], floor=1e-4).max(axis=2)
     within = (e <= 1e-4).mean()
-    mg, mr = g[..., :3].mean(), r[..., :3].mean()
+    mg, mr = g[..., :3].astype(np.float64).mean(), r[..., :3].astype(np.float64).mean()
     print(f"{label}: pixels within 1e-4 = {within:.5f}, worst {e.max():.3g}, mean gpu {mg:.6f} oracle {mr:.6f}, "
           f"alpha {g[..., 3].mean():.3f}/{r[..., 3].mean():.3f}")
     assert np.isfinite(g).all()
     assert within >= px_tol, f"{label}: {within}"
-    assert abs(mg - mr) / mr < 1e-2
+    assert abs(mg - mr) <= 1e-4 * abs(mr)
     return g, r
 
 

@@ -11,7 +11,9 @@
 
 Tolerances: per-pixel radiance within 1e-4 relative on every pixel but the
 few whose Russian-roulette or lobe draw sits within an ulp of its threshold
-(a floor per scene, set at or just under its measured value), image means
+where a transcendental library call (pow, exp, log, acos, atan2) rounds
+differently on the device (a floor per scene at its measured value: 5 of
+the 1080p frame's 2,073,600 pixels, none of the material boxes'), image means
 within 1e-4 relative (float64 means of the float32 images).
 """
 import numpy as np
@@ -75,7 +77,7 @@ def test_headline_full_frame_parity(renderer, headline, mode):
     sc, rs, ref = headline
     g = _render(renderer, sc, rs, mode, 2, wave_spp=2)
     assert g.shape == (1080, 1920, 4)
-    _compare(f"1080p mode={mode}", g, ref, 0.9999)
+    _compare(f"1080p mode={mode}", g, ref, 0.99999)
 
 
 def test_headline_forms_bit_identical(renderer, headline):
@@ -91,4 +93,4 @@ def test_material_boxes_forward_parity(renderer, material_boxes, mode):
     sc, rs, ref = material_boxes
     g = _render(renderer, sc, rs, mode, 4, wave_spp=4)
     assert g.shape == (512, 512, 4)
-    _compare(f"material boxes mode={mode}", g, ref, 0.9999)
+    _compare(f"material boxes mode={mode}", g, ref, 1.0)
