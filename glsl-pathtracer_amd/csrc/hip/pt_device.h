@@ -268,12 +268,21 @@ __device__ float GTR1(float NDotH, float a) {
     float t = 1.0f + (a2 - 1.0f) * NDotH * NDotH;
     return (a2 - 1.0f) / (PI * logf(a2) * t);
 }
+// clamp(sqrt(1 - c * c), 0, 1) of SampleGTR1 and SampleHG (sampling.glsl). When
+// rounding leaves |c| a hair above 1, GLSL's sqrt is undefined and a GPU's
+// max/min return the non-NaN operand, so the reference's clamp gives 0; with
+// the spec's max (x < y ? y : x) the NaN would reach the pixel. Both sides
+// take 0 there; elsewhere the value is the clamped root bit for bit.
+__device__ __forceinline__ float sin_from_cos(float c) {
+    const float s2 = 1.0f - (c * c);
+    return s2 > 0.0f ? gmin(sqrtf(s2), 1.0f) : 0.0f;
+}
 __device__ vec3 SampleGTR1(float rgh, float r1, float r2) {
     float a = gmax(0.001f, rgh);
     float a2 = a * a;
     float phi = r1 * TWO_PI;
     float cosTheta = sqrtf((1.0f - powf(a2, 1.0f - r2)) / (1.0f - a2));
-    float sinTheta = gclamp(sqrtf(1.0f - (cosTheta * cosTheta)), 0.0f, 1.0f);
+    float sinTheta = sin_from_cos(cosTheta);
     float sinPhi = sin_pt(phi), cosPhi = cos_pt(phi);
     return V(sinTheta * cosPhi, sinTheta * sinPhi, cosTheta);
 }
@@ -352,7 +361,7 @@ __device__ vec3 SampleHG(vec3 Vv, float g, float r1, float r2) {
         cosTheta = -(1.0f + g * g - sqrTerm * sqrTerm) / (2.0f * g);
     }
     float phi = r1 * TWO_PI;
-    float sinTheta = gclamp(sqrtf(1.0f - (cosTheta * cosTheta)), 0.0f, 1.0f);
+    float sinTheta = sin_from_cos(cosTheta);
     float sinPhi = sin_pt(phi), cosPhi = cos_pt(phi);
     vec3 v1, v2;
     Onb(Vv, v1, v2);

@@ -273,9 +273,11 @@ template <class T> struct State {
     int matID = 0;
     Material<T> mat;
     Medium<T> medium;
-    // frozen-sampling mode only: the material evaluated for f (see Ctx::eval_mats)
+    // frozen-sampling mode only: the material evaluated for f and the medium
+    // whose factors weight the path (see Ctx::eval_mats)
     Material<T> mat_e;
     T eta_e = 0;
+    Medium<T> medium_e;
     // instrumentation (not in the reference): which primitive was hit
     int hitInstance = -1, hitTriRow = -1, hitLight = -1;
     T baryU = 0, baryV = 0;
@@ -301,9 +303,15 @@ template <class T> struct Ctx {
     // Frozen-sampling mode (test instrument for the adjoint, not in the
     // reference): directions, lobe choices, pdfs, MIS weights and Russian
     // roulette come from the scene's materials; the BSDF values f that weight
-    // the path come from this second material table. A central difference of
-    // the image over eval_mats is then the derivative the path-replay adjoint
-    // estimates, sample by sample.
+    // the path come from this second material table, and so do the medium
+    // factors: absorption and emission with the second table's colour and
+    // density, and in a scattering medium, whose free-flight distances s are
+    // still drawn with the scene's density, the weight of a scatter event
+    // colour_e * density_e exp(-density_e s) / (density exp(-density s)) and of
+    // a segment of length d crossed without scattering
+    // exp(-density_e d) / exp(-density d) (1 and colour when both tables agree).
+    // A central difference of the image over eval_mats is then the derivative
+    // the path-replay adjoint estimates, sample by sample.
     const float* eval_mats = nullptr;
     T rand() { return T(rand_f(seed)); }
 };
@@ -394,12 +402,20 @@ template <class T> T GTR1(T NDotH, T a) {
     T t = T(1) + (a2 - T(1)) * NDotH * NDotH;
     return (a2 - T(1)) / (K<T>::PI * std::log(a2) * t);
 }
+// clamp(sqrt(1 - c * c), 0, 1) of SampleGTR1 and SampleHG (sampling.glsl):
+// where rounding leaves |c| a hair above 1 GLSL's sqrt is undefined; a GPU's
+// max/min return the non-NaN operand, so the reference's clamp gives 0, and
+// so does this. Elsewhere the clamped root itself.
+template <class T> inline T sin_from_cos(T c) {
+    const T s2 = T(1) - (c * c);
+    return s2 > T(0) ? gmin(std::sqrt(s2), T(1)) : T(0);
+}
 template <class T> V3<T> SampleGTR1(T rgh, T r1, T r2) {
     T a = gmax(T(0.001), rgh);
     T a2 = a * a;
     T phi = r1 * K<T>::TWO_PI;
     T cosTheta = std::sqrt((T(1) - std::pow(a2, T(1) - r2)) / (T(1) - a2));
-    T sinTheta = gclamp(std::sqrt(T(1) - (cosTheta * cosTheta)), T(0), T(1));
+    T sinTheta = sin_from_cos(cosTheta);
     T sinPhi = tsin(phi), cosPhi = tcos(phi);
     return V3<T>(sinTheta * cosPhi, sinTheta * sinPhi, cosTheta);
 }
@@ -527,7 +543,7 @@ template <class T> V3<T> SampleHG(V3<T> V, T g, T r1, T r2) {
         cosTheta = -(T(1) + g * g - sqrTerm * sqrTerm) / (T(2) * g);
     }
     T phi = r1 * K<T>::TWO_PI;
-    T sinTheta = gclamp(std::sqrt(T(1) - (cosTheta * cosTheta)), T(0), T(1));
+    T sinTheta = sin_from_cos(cosTheta);
     T sinPhi = tsin(phi), cosPhi = tcos(phi);
     V3<T> v1, v2;
     Onb(V, v1, v2);
@@ -1319,16 +1335,25 @@ template <class T> V4<T> PathTrace(Ctx<T>& c, Ray<T> r) {
             mediumSampled = false;
             surfaceScatter = false;
             if (inMedium) {
+                const Medium<T>& me = c.eval_mats ? state.medium_e : state.medium;
                 if (state.medium.type == MEDIUM_ABSORB) {
-                    throughput = throughput * vexp(-(V3<T>(T(1)) - state.medium.color) * state.hitDist * state.medium.density);
+                    throughput = throughput * vexp(-(V3<T>(T(1)) - me.color) * state.hitDist * me.density);
                     thr_s = thr_s * vexp(-(V3<T>(T(1)) - state.medium.color) * state.hitDist * state.medium.density);
                 } else if (state.medium.type == MEDIUM_EMISSIVE) {
-                    radiance = radiance + state.medium.color * state.hitDist * state.medium.density * throughput;
+                    radiance = radiance + me.color * state.hitDist * me.density * throughput;
                 } else {
                     T scatterDist = gmin(-std::log(c.rand()) / state.medium.density, state.hitDist);
                     mediumSampled = scatterDist < state.hitDist;
+                    if (c.eval_mats && !mediumSampled)
+                        throughput = throughput * (std::exp(-me.density * state.hitDist) /
+                                                   std::exp(-state.medium.density * state.hitDist));
                     if (mediumSampled) {
-                        throughput = throughput * state.medium.color;
+                        if (c.eval_mats)
+                            throughput = throughput * (me.color * ((me.density * std::exp(-me.density * scatterDist)) /
+                                                                   (state.medium.density *
+                                                                    std::exp(-state.medium.density * scatterDist))));
+                        else
+                            throughput = throughput * state.medium.color;
                         thr_s = thr_s * state.medium.color;
                         r.origin = r.origin + r.direction * scatterDist;
                         state.fhp = r.origin;
@@ -1374,6 +1399,7 @@ template <class T> V4<T> PathTrace(Ctx<T>& c, Ray<T> r) {
                 if (dot(r.direction, state.normal) < T(0) && state.mat.medium.type != MEDIUM_NONE) {
                     inMedium = true;
                     state.medium = state.mat.medium;
+                    if (c.eval_mats) state.medium_e = state.mat_e.medium;
                 } else if (state.mat.medium.type != MEDIUM_NONE) {
                     inMedium = false;
                 }

@@ -263,3 +263,24 @@ def test_energy_bound_metal():
         L, f, pdf, _ = oracle.disney_sample(m, V, N, rng.integers(0, 2 ** 32, 4, dtype=np.uint64).astype(np.uint32))
         w.append(f[0] / pdf if pdf > 0 else 0.0)
     assert 0.8 < np.mean(w) <= 1.0 + 3 * np.std(w) / np.sqrt(len(w))
+
+
+def test_samplers_finite_at_the_ends_of_their_range():
+    """SampleHG and SampleGTR1 at random numbers 0, 1 and their float
+    neighbours, over anisotropies and roughnesses: rounding can put the
+    sampled cosine a hair above 1, where sqrt(1 - cos^2) is undefined (GLSL)
+    and the reference's clamp gives sin 0 on a GPU. Every direction is finite
+    and of unit length (a heavy-scene path once turned NaN here)."""
+    ends = np.float32([0.0, 1e-7, 6e-8, 0.5, 1 - 6e-8, 1 - 1.2e-7, 1.0])
+    r1, r2 = [a.ravel() for a in np.meshgrid(ends, ends)]
+    V = np.tile(np.float32([0.3, -0.5, 0.81]) / np.linalg.norm([0.3, -0.5, 0.81]), (r1.size, 1))
+    for g in (-0.9, -0.3, -0.2, -0.01, 0.001, 0.01, 0.2, 0.3, 0.6, 0.9):
+        d = oracle.fn("SampleHG", V, g, r1, r2)
+        assert np.isfinite(d).all(), g
+        np.testing.assert_allclose(np.linalg.norm(d, axis=1), 1.0, atol=1e-4)
+    # SampleGTR1's roughness is the clearcoat's mix(0.1, 0.001, gloss)
+    # (disney.glsl), in [0.001, 0.1]; the formula divides by 1 - a^2
+    for rgh in (0.0, 0.001, 0.01, 0.05, 0.1):
+        h = oracle.fn("SampleGTR1", rgh, r1, r2)
+        assert np.isfinite(h).all(), rgh
+        np.testing.assert_allclose(np.linalg.norm(h, axis=1), 1.0, atol=1e-4)
