@@ -17,6 +17,9 @@ namespace ptd {
 // PT_ADJOINT_PARAMS layout): material-record fields baseColor.rgb (floats 0-2),
 // roughness (9), metallic (8), ior (17), clearcoat (14).
 constexpr int NPARAM = 7;
+// Medium parameters differentiated per material (PT_ADJOINT_MEDIUM_PARAMS):
+// density, colour r, g, b (record floats 19, 20-22).
+constexpr int NMEDIUM = 4;
 enum { P_BASE_R = 0, P_BASE_G = 1, P_BASE_B = 2, P_ROUGHNESS = 3, P_METALLIC = 4, P_IOR = 5, P_CLEARCOAT = 6 };
 
 struct Dual {

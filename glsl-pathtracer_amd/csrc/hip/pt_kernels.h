@@ -13,7 +13,7 @@ struct PathBuf {
     float4* thr;   // throughput
     float4* rad;   // radiance, alpha
     uint4* rng;    // pcg4d state
-    int4* meta;    // depth, pass-through count | in-medium << 16 | surface-scatter << 17, previous material, medium type
+    int4* meta;    // depth, pass-through count | flags | medium type << 20, previous material, medium's material
     float4* aux;   // previous texCoord (u, v), medium anisotropy
     float4* med;   // medium colour, density (only with OPT_MEDIUM, else null)
     float4* hit;   // closest hit: t, u, v, triangle row (int bits)
@@ -43,10 +43,11 @@ struct ReplayBuf {
     const float* dimg;    // dLoss / d(mean image), RGB per pixel, rows bottom-up
     float inv_n;          // 1 / number of samples in the mean image
     float* grad;          // [num_mats * NPARAM] gradient sums
+    float* mgrad;         // [num_mats * NMEDIUM] medium gradient sums (null: not computed)
     int* err;             // set to 1 when a vertex names a material outside [0, num_mats)
-    int* dropped;         // vertices whose gradient had a non-finite term
+    int* dropped;         // vertices and medium segments whose gradient had a non-finite term
     int num_mats;
-    int lds;              // 1: accumulate per block in LDS
+    int lds;              // 1: accumulate per block in LDS (grad, then mgrad)
 };
 
 // ---- batched render of many jobs (one scene, own camera / size / depth each)

@@ -116,6 +116,10 @@ __device__ __forceinline__ bool slot_pixel(const DevOpts& op, int slot, int npix
     return px < op.width && py < op.height;
 }
 
+// meta.y: pass-through count (bits 0-15), in-medium (16), surface-scatter (17),
+// ended (18, PATH_DEAD), medium type (20-21).
+constexpr int MEDIUM_TYPE_SHIFT = 20;
+
 __device__ __forceinline__ void load_path(const PathBuf& pb, int slot, PathRegs& p) {
     float4 o4 = pb.ro[slot], d4 = pb.rd[slot], t4 = pb.thr[slot], r4 = pb.rad[slot], a4 = pb.aux[slot];
     uint4 g = pb.rng[slot];
@@ -127,7 +131,8 @@ __device__ __forceinline__ void load_path(const PathBuf& pb, int slot, PathRegs&
     p.rng = Rng{g.x, g.y, g.z, g.w};
     p.depth = mt.x; p.pass = mt.y & 0xffff;
     p.in_medium = (mt.y >> 16) & 1; p.surface_scatter = (mt.y >> 17) & 1;
-    p.prev_mat = mt.z; p.med.type = mt.w;
+    p.med.type = (mt.y >> MEDIUM_TYPE_SHIFT) & 3;
+    p.prev_mat = mt.z; p.med_mat = mt.w;
     p.tu = a4.x; p.tv = a4.y; p.med.anisotropy = a4.z;
     if (pb.med) { float4 m4 = pb.med[slot]; p.med.color = xyz(m4); p.med.density = m4.w; }
     else { p.med.color = V1(0.f); p.med.density = 0.f; }
@@ -139,8 +144,10 @@ __device__ __forceinline__ void store_path(const PathBuf& pb, int slot, const Pa
     pb.thr[slot] = F4(p.thr, 0.f);
     pb.rad[slot] = F4(p.rad, p.alpha);
     pb.rng[slot] = make_uint4(p.rng.x, p.rng.y, p.rng.z, p.rng.w);
-    pb.meta[slot] = make_int4(p.depth, (p.pass & 0xffff) | (p.in_medium ? 1 << 16 : 0) | (p.surface_scatter ? 1 << 17 : 0),
-                              p.prev_mat, p.med.type);
+    pb.meta[slot] = make_int4(p.depth,
+                              (p.pass & 0xffff) | (p.in_medium ? 1 << 16 : 0) | (p.surface_scatter ? 1 << 17 : 0) |
+                                  (p.med.type << MEDIUM_TYPE_SHIFT),
+                              p.prev_mat, p.med_mat);
     pb.aux[slot] = make_float4(p.tu, p.tv, p.med.anisotropy, 0.f);
     if (pb.med) pb.med[slot] = F4(p.med.color, p.med.density);
 }
@@ -431,7 +438,8 @@ __global__ void __launch_bounds__(256) k_replay_shade(DevScene sc, DevOpts op, P
                                                       int first_sample, int* qout, int* count_out, int* work,
                                                       int compact, ReplayBuf rb, int vshift) {
     extern __shared__ float gacc[];
-    const int ng = rb.num_mats * NPARAM;
+    const int ng0 = rb.num_mats * NPARAM;
+    const int ng = ng0 + (rb.mgrad ? rb.num_mats * NMEDIUM : 0);
     if (rb.lds) {
         for (int j = threadIdx.x; j < ng; j += blockDim.x) gacc[j] = 0.0f;
         __syncthreads();
@@ -455,8 +463,26 @@ __global__ void __launch_bounds__(256) k_replay_shade(DevScene sc, DevOpts op, P
             const float* dp = rb.dimg + 3 * (static_cast<size_t>(py) * op.width + px);
             const vec3 delta = V(dp[0], dp[1], dp[2]) * rb.inv_n;
             const int vis = static_cast<int>((rb.vis[slot] >> vshift) & 3u);
-            float g[NPARAM];
-            const ReplayVertex rv = replay_vertex(sc, op, h, p, vis, delta, xyz(rb.ltot[slot]), g);
+            float g[NPARAM], mg[NMEDIUM];
+            const ReplayVertex rv = replay_vertex(sc, op, h, p, vis, delta, xyz(rb.ltot[slot]), g,
+                                                  rb.mgrad ? mg : nullptr);
+            if (rv.med_mat >= 0) {
+                if (rv.med_mat >= rb.num_mats) {
+                    atomicOr(rb.err, 1);
+                } else if (!rv.mfinite) {
+                    atomicAdd(rb.dropped, 1);
+                } else {
+                    const int g0 = rv.med_mat * NMEDIUM;
+#pragma unroll
+                    for (int k = 0; k < NMEDIUM; ++k) {
+                        if (mg[k] == 0.0f) continue;
+                        if (rb.lds)
+                            __hip_atomic_fetch_add(&gacc[ng0 + g0 + k], mg[k], __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+                        else atomicAdd(rb.mgrad + g0 + k, mg[k]);
+                    }
+                }
+            }
             if (rv.mat >= 0) {
                 if (rv.mat >= rb.num_mats) {
                     atomicOr(rb.err, 1);
@@ -488,7 +514,7 @@ __global__ void __launch_bounds__(256) k_replay_shade(DevScene sc, DevOpts op, P
     if (rb.lds) {
         __syncthreads();
         for (int j = threadIdx.x; j < ng; j += blockDim.x)
-            if (gacc[j] != 0.0f) atomicAdd(rb.grad + j, gacc[j]);
+            if (gacc[j] != 0.0f) atomicAdd(j < ng0 ? rb.grad + j : rb.mgrad + (j - ng0), gacc[j]);
     }
 }
 
@@ -1221,7 +1247,8 @@ void launch_replay_shade(hipStream_t s, int blocks, int bs, const DevScene& sc, 
                          bool primary, const int* hitq, const int* hit_count, int npix, int first_sample, int* qout,
                          int* count_out, int* work, bool compact, const ReplayBuf& rb, int vshift) {
     const int c = compact ? 1 : 0;
-    const size_t lds = rb.lds ? static_cast<size_t>(rb.num_mats) * NPARAM * sizeof(float) : 0;
+    const size_t lds = rb.lds ? static_cast<size_t>(rb.num_mats) * (NPARAM + (rb.mgrad ? NMEDIUM : 0)) * sizeof(float)
+                              : 0;
     if (primary) hipLaunchKernelGGL((k_replay_shade<true>), dim3(blocks), dim3(bs), lds, s, sc, op, pb, hitq, hit_count, npix, first_sample, qout, count_out, work, c, rb, vshift);
     else hipLaunchKernelGGL((k_replay_shade<false>), dim3(blocks), dim3(bs), lds, s, sc, op, pb, hitq, hit_count, npix, first_sample, qout, count_out, work, c, rb, vshift);
 }

@@ -100,6 +100,7 @@ struct PathRegs {
     bool in_medium, surface_scatter;
     float tu, tv;
     Medium med;
+    int med_mat;  // material whose medium the path entered last (-1: none)
     Rng rng;
 };
 
@@ -249,12 +250,25 @@ __device__ void on_miss(const DevScene& sc, const DevOpts& op, PathRegs& p) {
     }
 }
 
+// The medium segment a bounce ended (for the adjoint): the path crossed d = the
+// hit distance inside the medium of material `mat` (-1: not in a medium) and,
+// in a scattering medium, scattered at distance s when `scattered`; thr and
+// rad are the throughput and radiance before the medium's factor was applied.
+struct MedRec {
+    int mat, type;
+    float d, s, density;
+    vec3 color, thr, rad;
+    bool scattered;
+};
+
 // Bounce body from the hit up to next-event estimation (pathtrace.glsl:341-425):
 // material, emission, emitter MIS, depth limit, medium events, alpha test.
 // `st` keeps the previous bounce's attributes when a light is hit, as the
-// shader's State does. Returns the event that decides what follows.
+// shader's State does. Returns the event that decides what follows; with `mr`,
+// records the medium segment for the adjoint.
 __device__ int bounce_pre(const DevScene& sc, const DevOpts& op, const Hit& h, PathRegs& p, SurfState& st,
-                          Material& m, float& eta) {
+                          Material& m, float& eta, MedRec* mr = nullptr) {
+    if (mr) mr->mat = -1;
     const bool is_emitter = h.tri < 0;
     float light_pdf = 0.f;
     vec3 light_em = V1(0.f);
@@ -283,6 +297,11 @@ __device__ int bounce_pre(const DevScene& sc, const DevOpts& op, const Hit& h, P
     if (op.flags & OPT_MEDIUM) {
         p.surface_scatter = false;
         if (p.in_medium) {
+            if (mr) {
+                mr->mat = p.med_mat; mr->type = p.med.type; mr->d = h.t; mr->s = 0.0f;
+                mr->density = p.med.density; mr->color = p.med.color; mr->thr = p.thr; mr->rad = p.rad;
+                mr->scattered = false;
+            }
             if (p.med.type == MEDIUM_ABSORB) {
                 p.thr = p.thr * vexp(-(V1(1.0f) - p.med.color) * h.t * p.med.density);
             } else if (p.med.type == MEDIUM_EMISSIVE) {
@@ -290,6 +309,7 @@ __device__ int bounce_pre(const DevScene& sc, const DevOpts& op, const Hit& h, P
             } else {
                 float scatterDist = gmin(-logf(rnd(p.rng)) / p.med.density, h.t);
                 if (scatterDist < h.t) {
+                    if (mr) { mr->scattered = true; mr->s = scatterDist; }
                     p.thr = p.thr * p.med.color;
                     p.ro = p.ro + p.rd * scatterDist;
                     st.fhp = p.ro;
@@ -343,6 +363,7 @@ __device__ bool bounce_post(const DevOpts& op, PathRegs& p, const SurfState& st,
             if (dot(p.rd, st.normal) < 0.0f && m.medium.type != MEDIUM_NONE) {
                 p.in_medium = true;
                 p.med = m.medium;
+                p.med_mat = st.matID;
             } else if (m.medium.type != MEDIUM_NONE) {
                 p.in_medium = false;
             }
@@ -363,6 +384,7 @@ __device__ __forceinline__ void init_path(PathRegs& p) {
     p.in_medium = false; p.surface_scatter = false;
     p.tu = 0.f; p.tv = 0.f;
     p.med.type = 0; p.med.density = 0.f; p.med.color = V1(0.f); p.med.anisotropy = 0.f;
+    p.med_mat = -1;
 }
 
 __device__ __forceinline__ void init_surf(SurfState& st) {

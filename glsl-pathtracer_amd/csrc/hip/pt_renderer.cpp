@@ -311,9 +311,16 @@ int counter_cells(int D) { return 6 * D + 6; }
 // used) when AnyHit draws from the path's random numbers between NEE sampling
 // and BSDF sampling (stochastic BLEND alpha without media, anyhit.glsl:118-141)
 // or when it is a transmittance (volume MIS with media, pathtrace.glsl:118-156).
-bool inline_nee(const pt_renderer_s* h, const DevOpts& op) {
+// The scene needs NEE inside the shade stage: stochastic alpha draws from the
+// path's RNG in shadow rays, volume MIS attenuates them through media.
+bool needs_inline_nee(const pt_renderer_s* h, const DevOpts& op) {
     if ((op.flags & ptd::OPT_MEDIUM) && (op.flags & ptd::OPT_VOL_MIS)) return true;
     return (op.flags & ptd::OPT_ALPHA_TEST) && !(op.flags & ptd::OPT_MEDIUM) && h->any_blend;
+}
+
+// NEE in the shade stage: where the scene needs it, or by pt_options.nee_inline.
+bool inline_nee(const pt_renderer_s* h, const DevOpts& op) {
+    return needs_inline_nee(h, op) || h->opts.nee_inline == 1;
 }
 
 void ensure_shadow(pt_renderer_s* h, size_t max_paths) {
@@ -331,7 +338,9 @@ struct Wavefront {
     size_t max_paths;
 };
 
-Wavefront wavefront_setup(pt_renderer_s* h, const DevOpts& op, int num_samples) {
+// adjoint: NEE always in the shadow stage, which records the visibility bits
+// the replay applies.
+Wavefront wavefront_setup(pt_renderer_s* h, const DevOpts& op, int num_samples, bool adjoint = false) {
     Wavefront w;
     const int W = op.width, H = op.height;
     const int tiles = ((W + 7) / 8) * ((H + 7) / 8);
@@ -344,7 +353,7 @@ Wavefront wavefront_setup(pt_renderer_s* h, const DevOpts& op, int num_samples) 
     h->p_rng.ensure(max_paths); h->p_meta.ensure(max_paths); h->p_aux.ensure(max_paths);
     if (h->any_medium) h->p_med.ensure(max_paths);
     h->p_hit.ensure(max_paths); h->p_hit2.ensure(max_paths);
-    w.inline_nee = inline_nee(h, op);
+    w.inline_nee = adjoint ? false : inline_nee(h, op);
     if (!w.inline_nee) ensure_shadow(h, max_paths);
     h->qa.ensure(max_paths); h->qb.ensure(max_paths); h->hq.ensure(max_paths);
     // Bounce iterations: maxDepth + 1 closest hits; alpha pass-through repeats a depth.
@@ -437,16 +446,18 @@ void render_wavefront(pt_renderer_s* h, int first_sample, int num_samples) {
 // radiance buffer and records which NEE rays were visible; the replay pass
 // re-traces the same paths (closest hits again, same seeds, same decisions)
 // on a second radiance buffer, applies the recorded visibility instead of
-// shadow rays, and accumulates each surface vertex's parameter gradient.
-void adjoint_wavefront(pt_renderer_s* h, int first_sample, int num_samples, float inv_n) {
+// shadow rays, and accumulates each surface vertex's parameter gradient (and,
+// with `medium`, each medium segment's, after them in the same buffer).
+void adjoint_wavefront(pt_renderer_s* h, int first_sample, int num_samples, float inv_n, bool medium) {
     const DevOpts op = h->dev_opts();
     if (op.flags & ptd::OPT_ROUGHNESS_MOLLIFICATION)
         throw ArgError(PT_ERR_UNSUPPORTED, "the adjoint does not differentiate roughness mollification");
-    if (inline_nee(h, op))
+    if (needs_inline_nee(h, op))
         throw ArgError(PT_ERR_UNSUPPORTED, "the adjoint replays recorded NEE visibility: stochastic (blend) alpha "
                                            "in shadow rays and volume MIS are not supported");
-    const Wavefront w = wavefront_setup(h, op, num_samples);
-    const size_t grad_bytes = static_cast<size_t>(h->num_materials) * ptd::NPARAM * sizeof(float);
+    const Wavefront w = wavefront_setup(h, op, num_samples, true);
+    const size_t grad_bytes =
+        static_cast<size_t>(h->num_materials) * (ptd::NPARAM + (medium ? ptd::NMEDIUM : 0)) * sizeof(float);
     const bool lds = grad_bytes <= 32768;
     const int b_rep = ptk::persistent_blocks(ptk::KIND_REPLAY, w.bs, w.S, false, h->num_cu, lds ? grad_bytes : 0);
     const int nwords = (w.D + ptd::VIS_BOUNCES_PER_WORD - 1) / ptd::VIS_BOUNCES_PER_WORD;
@@ -463,6 +474,7 @@ void adjoint_wavefront(pt_renderer_s* h, int first_sample, int num_samples, floa
     rb.dimg = h->a_dimg.p;
     rb.inv_n = inv_n;
     rb.grad = h->a_grad.p;
+    rb.mgrad = medium ? h->a_grad.p + static_cast<size_t>(h->num_materials) * ptd::NPARAM : nullptr;
     rb.err = h->a_flags.p;
     rb.dropped = h->a_flags.p + 1;
     rb.num_mats = h->num_materials;
@@ -1188,6 +1200,7 @@ int pt_set_options(pt_handle h, const pt_options* o) {
         if (static_cast<long long>(o->width) * o->height > (1LL << 28)) return Fail(h, PT_ERR_INVALID_ARG, "resolution too large");
         if (o->bvh_width != 0 && o->bvh_width != 2 && o->bvh_width != 4)
             return Fail(h, PT_ERR_INVALID_ARG, "bvh_width must be 2 or 4");
+        if (o->nee_inline != 0 && o->nee_inline != 1) return Fail(h, PT_ERR_INVALID_ARG, "nee_inline must be 0 or 1");
         HIPCHK(hipSetDevice(h->device));
         h->opts = *o;
         h->has_opts = true;
@@ -1391,16 +1404,21 @@ int pt_batch_render(pt_handle h, const pt_job* jobs, int32_t num_jobs) {
     });
 }
 
-int pt_adjoint(pt_handle h, int32_t first_sample, int32_t num_samples, const float* dL_dimage, float* d_params) {
+// pt_adjoint (medium false) and pt_adjoint_medium (medium true).
+static int adjoint_call(pt_handle h, int32_t first_sample, int32_t num_samples, const float* dL_dimage,
+                        float* d_params, float* d_medium, bool medium) {
     return Guard(h, [&] {
         check_ready(h);
         if (first_sample < 0 || num_samples <= 0) return Fail(h, PT_ERR_INVALID_ARG, "invalid sample range");
-        if (!dL_dimage || !d_params) return Fail(h, PT_ERR_INVALID_ARG, "null gradient buffers");
+        if (!dL_dimage || !d_params || (medium && !d_medium))
+            return Fail(h, PT_ERR_INVALID_ARG, "null gradient buffers");
+        if (!medium) d_medium = nullptr;
         if (h->num_materials <= 0) return Fail(h, PT_ERR_STATE, "scene has no materials");
         HIPCHK(hipSetDevice(h->device));
         h->resolve_events();
         const size_t npx = static_cast<size_t>(h->opts.width) * h->opts.height;
-        const size_t ng = static_cast<size_t>(h->num_materials) * ptd::NPARAM;
+        const size_t np = static_cast<size_t>(h->num_materials) * ptd::NPARAM;
+        const size_t ng = np + (d_medium ? static_cast<size_t>(h->num_materials) * ptd::NMEDIUM : 0);
         h->a_dimg.upload(dL_dimage, 3 * npx, h->stream);
         h->a_grad.ensure(ng);
         h->a_flags.ensure(2);
@@ -1409,7 +1427,7 @@ int pt_adjoint(pt_handle h, int32_t first_sample, int32_t num_samples, const flo
         h->ensure_misc();
         HIPCHK(hipMemsetAsync(h->ovf.p, 0, sizeof(int), h->stream));
         HIPCHK(hipEventRecord(h->ev_r0, h->stream));
-        adjoint_wavefront(h, first_sample, num_samples, 1.0f / static_cast<float>(num_samples));
+        adjoint_wavefront(h, first_sample, num_samples, 1.0f / static_cast<float>(num_samples), d_medium != nullptr);
         HIPCHK(hipEventRecord(h->ev_r1, h->stream));
         h->render_pending = true;
         std::vector<float> g(ng);
@@ -1419,9 +1437,19 @@ int pt_adjoint(pt_handle h, int32_t first_sample, int32_t num_samples, const flo
         h->resolve_events();
         h->stats.adjoint_dropped = flags[1];
         if (flags[0]) return Fail(h, PT_ERR_STATE, "adjoint: a path vertex named a material outside the scene's table");
-        for (size_t i = 0; i < ng; ++i) d_params[i] += g[i];
+        for (size_t i = 0; i < np; ++i) d_params[i] += g[i];
+        for (size_t i = np; i < ng; ++i) d_medium[i - np] += g[i];
         return h->stats.stack_overflow ? Fail(h, PT_ERR_OVERFLOW, "BVH traversal exceeded the maximum stack depth") : PT_OK;
     });
+}
+
+int pt_adjoint(pt_handle h, int32_t first_sample, int32_t num_samples, const float* dL_dimage, float* d_params) {
+    return adjoint_call(h, first_sample, num_samples, dL_dimage, d_params, nullptr, false);
+}
+
+int pt_adjoint_medium(pt_handle h, int32_t first_sample, int32_t num_samples, const float* dL_dimage,
+                      float* d_params, float* d_medium) {
+    return adjoint_call(h, first_sample, num_samples, dL_dimage, d_params, d_medium, true);
 }
 
 }  // extern "C"

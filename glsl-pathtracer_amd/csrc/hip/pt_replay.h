@@ -54,11 +54,55 @@ __device__ __forceinline__ bool vertex_gradient(const DevScene& sc, const SurfSt
     return finite;
 }
 
+// Gradient of the medium segment a bounce ended (pathtrace.glsl:370-410),
+// with the path's later radiance lafter (everything gathered after the
+// medium's factor was applied) and delta = dLoss/dradiance:
+//   absorb     factor exp(-(1 - c) d sigma) per channel:
+//              d/dsigma = sum_c delta_c lafter_c (-(1 - c_c) d), d/dc_c = delta_c lafter_c d sigma
+//   emissive   radiance += c d sigma thr:
+//              d/dsigma = sum_c delta_c c_c d thr_c, d/dc_c = delta_c d sigma thr_c
+//   scatter    distances drawn from sigma exp(-sigma s) (detached), so the path's
+//              factor is its score: a scatter at s weighs colour * sigma
+//              exp(-sigma s) / pdf, d log/dsigma = 1/sigma - s, d log/dc_c = 1/c_c
+//              (a channel with c_c == 0 or lafter_c == 0 contributes nothing);
+//              a crossing of length d weighs exp(-sigma d) / pdf, d log/dsigma = -d.
+// Returns false when a term is not finite.
+__device__ __forceinline__ bool medium_gradient(const MedRec& mr, vec3 delta, vec3 lafter, float g[NMEDIUM]) {
+    const float c[3] = {mr.color.x, mr.color.y, mr.color.z};
+    const float w[3] = {delta.x * lafter.x, delta.y * lafter.y, delta.z * lafter.z};
+    const float t[3] = {mr.thr.x, mr.thr.y, mr.thr.z};
+    const float dl[3] = {delta.x, delta.y, delta.z};
+    const float l[3] = {lafter.x, lafter.y, lafter.z};
+    g[0] = 0.0f;
+    for (int k = 0; k < 3; ++k) {
+        float gs = 0.0f, gc = 0.0f;
+        if (mr.type == MEDIUM_ABSORB) {
+            gs = w[k] * (-(1.0f - c[k]) * mr.d);
+            gc = w[k] * (mr.d * mr.density);
+        } else if (mr.type == MEDIUM_EMISSIVE) {
+            gs = dl[k] * c[k] * mr.d * t[k];
+            gc = dl[k] * mr.d * mr.density * t[k];
+        } else if (mr.scattered) {
+            gs = w[k] * (1.0f / mr.density - mr.s);
+            gc = (c[k] != 0.0f && l[k] != 0.0f) ? w[k] / c[k] : 0.0f;
+        } else {
+            gs = w[k] * -mr.d;
+        }
+        g[0] = g[0] + gs;
+        g[1 + k] = gc;
+    }
+    bool finite = true;
+    for (int k = 0; k < NMEDIUM; ++k) finite = finite && (g[k] - g[k] == 0.0f);
+    return finite;
+}
+
 // Result of one replayed vertex.
 struct ReplayVertex {
-    bool alive;   // the path continues (the primal queued it for the next bounce)
-    int mat;      // material of a SURFACE vertex whose gradient is in g, else -1
-    bool finite;  // false: the vertex's gradient had a non-finite term and is dropped
+    bool alive;    // the path continues (the primal queued it for the next bounce)
+    int mat;       // material of a SURFACE vertex whose gradient is in g, else -1
+    bool finite;   // false: the vertex's gradient had a non-finite term and is dropped
+    int med_mat;   // material of the medium segment whose gradient is in mg, else -1
+    bool mfinite;  // false: the segment's gradient had a non-finite term and is dropped
 };
 
 // One bounce of the replay after its closest hit `h` (the shade and shadow
@@ -71,13 +115,20 @@ struct ReplayVertex {
 // at NEE time), so p.rad stays bit-identical to the primal's.
 __device__ __forceinline__ ReplayVertex replay_vertex(const DevScene& sc, const DevOpts& op, const Hit& h,
                                                       PathRegs& p, int vis, vec3 delta, vec3 ltot,
-                                                      float g[NPARAM]) {
-    ReplayVertex out{false, -1, true};
+                                                      float g[NPARAM], float mg[NMEDIUM] = nullptr) {
+    ReplayVertex out{false, -1, true, -1, true};
     SurfState st;
     init_surf(st);
     Material m;
     float eta;
-    const int ev = bounce_pre(sc, op, h, p, st, m, eta);
+    MedRec mr;
+    const int ev = bounce_pre(sc, op, h, p, st, m, eta, mg ? &mr : nullptr);
+    if (mg && mr.mat >= 0) {
+        // everything after the medium's factor: the primal's final radiance
+        // less what the path held before it (emission of this hit included)
+        out.med_mat = mr.mat;
+        out.mfinite = medium_gradient(mr, delta, ltot - mr.rad, mg);
+    }
     if (ev == EV_END) return out;
     NeeRay nee[2];
     nee[0].valid = nee[1].valid = false;

@@ -20,6 +20,10 @@ PT_MODE_PLAIN = 1
 PT_ADJOINT_PARAMS = 7
 ADJOINT_FIELDS = (0, 1, 2, 9, 8, 17, 14)
 ADJOINT_NAMES = ("baseColor.r", "baseColor.g", "baseColor.b", "roughness", "metallic", "ior", "clearcoat")
+# pt_adjoint_medium's medium parameters and their material-record floats
+PT_ADJOINT_MEDIUM_PARAMS = 4
+ADJOINT_MEDIUM_FIELDS = (19, 20, 21, 22)
+ADJOINT_MEDIUM_NAMES = ("mediumDensity", "mediumColor.r", "mediumColor.g", "mediumColor.b")
 STAGES = ("raygen", "extend", "shade", "shadow", "accum", "plain", "adjoint", "batch")
 
 f3 = C.c_float * 3
@@ -75,7 +79,8 @@ class PtOptions(C.Structure):
                 ("enable_roughness_mollification", i32), ("roughness_mollification_amt", f32),
                 ("enable_volume_mis", i32), ("opengl_normal_map", i32),
                 ("mode", i32), ("block_size", i32), ("stack_depth", i32), ("wave_spp", i32),
-                ("use_graph", i32), ("count_work", i32), ("compact_every", i32), ("bvh_width", i32)]
+                ("use_graph", i32), ("count_work", i32), ("compact_every", i32), ("bvh_width", i32),
+                ("nee_inline", i32)]
 
 
 class PtStats(C.Structure):
@@ -149,6 +154,7 @@ _HIP_SIGS = {
     "pt_any_hit": [C.c_void_p, P(f32), P(f32), P(f32), i32, P(i32)],
     "pt_batch_render": [C.c_void_p, P(PtJob), i32],
     "pt_adjoint": [C.c_void_p, i32, i32, P(f32), P(f32)],
+    "pt_adjoint_medium": [C.c_void_p, i32, i32, P(f32), P(f32), P(f32)],
     "pt_set_env_map": [C.c_void_p, P(f32), i32, i32],
     "pt_read_env_cdf": [C.c_void_p, P(f32), C.c_size_t, P(f32)],
     "pt_rebuild_instances": [C.c_void_p, P(f32), i32, P(i32)],

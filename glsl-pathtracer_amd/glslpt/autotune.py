@@ -1,8 +1,9 @@
 """Launch-configuration autotuner for the wavefront renderer.
 
 Sweeps workgroup size {64, 128, 256}, LDS traversal-stack depth {16, 24, 32},
-samples per wavefront pass, queue-compaction cadence and BLAS node width
-{2, 4} on the actual workload, timing it on the GPU, and stores the fastest configuration as JSON
+samples per wavefront pass, queue-compaction cadence, BLAS node width {2, 4}
+and where NEE rays are traced (shadow stage or inside shade) on the actual
+workload, timing it on the GPU, and stores the fastest configuration as JSON
 in glslpt/tuning/<key>.json, where bench.py and callers look it up.
 
     python -m glslpt.autotune c2 [spp] [--quick] [--out DIR]   # 1080p instanced spheres, pt_render
@@ -24,10 +25,11 @@ TUNING_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning")
 BLOCK_SIZES = (64, 128, 256)
 STACK_DEPTHS = (16, 24, 32)
 QUICK = dict(block_size=(64, 128), stack_depth=(16, 24), wave_spp=(16, 32, 64), compact_every=(1, 2),
-             bvh_width=(2, 4))
+             bvh_width=(2, 4), nee_inline=(0, 1))
 WAVE_SPP = (8, 16, 32, 64)
 CADENCE = (1, 2)
 BVH_WIDTHS = (2, 4)
+NEE_INLINE = (0, 1)
 
 
 def tuned(key):
@@ -54,7 +56,7 @@ def _time(r, sc, rs, spp, reps, **kw):
 
 def sweep(r, sc, rs, spp=16, reps=2, grid=None, log=print):
     grid = grid or dict(block_size=BLOCK_SIZES, stack_depth=STACK_DEPTHS, wave_spp=WAVE_SPP, compact_every=CADENCE,
-                        bvh_width=BVH_WIDTHS)
+                        bvh_width=BVH_WIDTHS, nee_inline=NEE_INLINE)
     keys = list(grid)
     results = []
     for vals in itertools.product(*[grid[k] for k in keys]):
@@ -81,7 +83,7 @@ def _time_batch(r, rs, jobs, reps, **kw):
 
 def sweep_batch(r, rs, jobs, reps=1, grid=None, log=print):
     grid = grid or dict(block_size=BLOCK_SIZES, stack_depth=STACK_DEPTHS, wave_spp=(2, 4, 8, 16),
-                        compact_every=(1, 2, 3), bvh_width=BVH_WIDTHS)
+                        compact_every=(1, 2, 3), bvh_width=BVH_WIDTHS, nee_inline=NEE_INLINE)
     keys = list(grid)
     total = sum(j["width"] * j["height"] * j["spp"] for j in jobs)
     results = []
@@ -107,7 +109,7 @@ def main(argv):
         r = Renderer(0)
         r.upload_scene(sc)
         grid = dict(block_size=(64, 256), stack_depth=(16, 24), wave_spp=(4, 8, 16), compact_every=(1, 2),
-                    bvh_width=BVH_WIDTHS) if "--quick" in argv else None
+                    nee_inline=NEE_INLINE) if "--quick" in argv else None
         res = sweep_batch(r, rs, jobs, reps=2, grid=grid)
         os.makedirs(out_dir, exist_ok=True)
         with open(os.path.join(out_dir, "c3.json"), "w") as f:

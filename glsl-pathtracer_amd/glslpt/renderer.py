@@ -185,6 +185,19 @@ class Renderer:
                                                        api.fptr(out)))
         return out
 
+    def adjoint_medium(self, first_sample, num_samples, dl_dimage):
+        """As adjoint, plus the gradient with respect to each material's
+        medium (density, colour r, g, b): returns (material gradient
+        (num_materials, 7), medium gradient (num_materials, 4))."""
+        g = np.ascontiguousarray(dl_dimage, np.float32)
+        if g.shape != (self.height, self.width, 3):
+            raise ValueError(f"dl_dimage must be ({self.height}, {self.width}, 3)")
+        out = np.zeros((self.num_materials, api.PT_ADJOINT_PARAMS), np.float32)
+        med = np.zeros((self.num_materials, api.PT_ADJOINT_MEDIUM_PARAMS), np.float32)
+        self._check("pt_adjoint_medium", self._lib.pt_adjoint_medium(self._h, int(first_sample), int(num_samples),
+                                                                     api.fptr(g), api.fptr(out), api.fptr(med)))
+        return out, med
+
     # --- ray queries ------------------------------------------------------
     def closest_hit(self, origins, dirs):
         o = np.ascontiguousarray(origins, np.float32)

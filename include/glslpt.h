@@ -165,6 +165,10 @@ typedef struct pt_options {
     int32_t compact_every;   /* compact the path queue every k-th bounce (1 = every bounce) */
     int32_t bvh_width;       /* 2: BLAS traversed as the reference's BVH2; 4: as four-wide nodes
                                 collapsed from it (same hits but for exact ties) */
+    int32_t nee_inline;      /* wavefront and batched forms: 1 = the shade stage traces its NEE rays
+                                itself (no shadow stage); 0 = a shadow stage does, except where the
+                                scene needs them inline (stochastic alpha, volume MIS). Same image
+                                either way; pt_adjoint always uses the shadow stage */
 } pt_options;
 
 typedef struct pt_stats {
@@ -315,6 +319,22 @@ int pt_batch_render(pt_handle h, const pt_job* jobs, int32_t num_jobs);
 #define PT_ADJOINT_PARAMS 7
 int pt_adjoint(pt_handle h, int32_t first_sample, int32_t num_samples, const float* dL_dimage,
                float* d_params);
+
+/* As pt_adjoint, plus the gradient with respect to each material's medium
+ * (no reference counterpart; the forward path it differentiates is the
+ * medium branch of PathTrace, pathtrace.glsl:370-410, and the medium a path
+ * enters at :457-465): per material, in this order, medium density (record
+ * float 19) and medium colour r, g, b (20-22), added to d_medium
+ * (num_materials*4 floats). Free-flight distances of scattering media are
+ * drawn with the current density and held fixed; the estimate includes their
+ * score (1/density - s for a scatter at distance s, -d for a crossing of
+ * length d), so it is unbiased for the gradient of the expected image. A
+ * medium entry is credited to the material whose surface the path crossed to
+ * enter it. Same errors as pt_adjoint; medium segments with a non-finite term
+ * are left out and counted in pt_stats.adjoint_dropped. */
+#define PT_ADJOINT_MEDIUM_PARAMS 4
+int pt_adjoint_medium(pt_handle h, int32_t first_sample, int32_t num_samples, const float* dL_dimage,
+                      float* d_params, float* d_medium);
 
 #ifdef __cplusplus
 }

@@ -54,18 +54,24 @@ def render(sc, rs, first, n, **opts):
     return acc
 
 
-def adjoint(sc, rs, dl_dimage, first, n, materials=None, **opts):
+def adjoint(sc, rs, dl_dimage, first, n, materials=None, medium=False, **opts):
     """(gradient (num_materials, 7) float64, dropped vertices, paths whose
     replayed radiance differs from the primal's) of the adjoint's primal +
-    replay passes on the CPU."""
+    replay passes on the CPU; with medium, (gradient, medium gradient
+    (num_materials, 4), dropped, mismatched)."""
     v, _ = _view(sc, materials)
     o = options_for(rs, **opts)
     cam = sc.camera()
     g = np.zeros((v.num_materials, 7), np.float64)
+    mg = np.zeros((v.num_materials, 4), np.float64)
     w = np.ascontiguousarray(dl_dimage, np.float32)
     dropped, mismatch = C.c_int(0), C.c_int(0)
     rc = lib().host_adjoint(C.byref(v), C.byref(o), C.byref(cam), int(first), int(n), w.ctypes.data_as(P(C.c_float)),
-                            g.ctypes.data_as(P(C.c_double)), C.byref(dropped), C.byref(mismatch))
+                            g.ctypes.data_as(P(C.c_double)),
+                            mg.ctypes.data_as(P(C.c_double)) if medium else None,
+                            C.byref(dropped), C.byref(mismatch))
     if rc != 0:
         raise RuntimeError("host_adjoint failed")
+    if medium:
+        return g, mg, dropped.value, mismatch.value
     return g, dropped.value, mismatch.value

@@ -95,8 +95,10 @@ static float4 primal_path(const DevScene& sc, const DevOpts& op, PathRegs& p, St
 // Gradient sums (double) per material parameter; dropped: vertices with a
 // non-finite gradient term; mismatch: paths whose replayed radiance differs
 // from the primal's in any bit.
+// grad: num_materials * NPARAM sums; mgrad (null: not computed): the medium
+// gradient, num_materials * NMEDIUM sums.
 extern "C" int host_adjoint(const pt_scene_view* v, const pt_options* o, const pt_camera* cam, int first, int n,
-                            const float* dimg, double* grad, int* dropped, int* mismatch_out) {
+                            const float* dimg, double* grad, double* mgrad, int* dropped, int* mismatch_out) {
     try {
         HostScene H(v, o, cam);
         const DevScene& sc = H.sc;
@@ -110,7 +112,7 @@ extern "C" int host_adjoint(const pt_scene_view* v, const pt_options* o, const p
             Stack stk{lds.data(), spill.data(), 0, 1, MAXD, 0, 1};
             TravCount cnt{0, 0, 0, 0, 0, 0};
             bool ovf = false;
-            std::vector<double> gl(nm * NPARAM, 0.0);
+            std::vector<double> gl(nm * NPARAM, 0.0), ml(nm * NMEDIUM, 0.0);
 #pragma omp for schedule(dynamic, 4)
             for (int py = 0; py < Hh; ++py)
                 for (int px = 0; px < W; ++px)
@@ -128,8 +130,13 @@ extern "C" int host_adjoint(const pt_scene_view* v, const pt_options* o, const p
                         for (int b = 0; b < D; ++b) {
                             Hit h = closest<false>(sc, op, p.ro, p.rd, p.depth, stk, cnt, ovf);
                             if (h.t == INF) { on_miss(sc, op, p); break; }
-                            float g[NPARAM];
-                            ReplayVertex rv = replay_vertex(sc, op, h, p, vis[b], delta, V(lt.x, lt.y, lt.z), g);
+                            float g[NPARAM], mg[NMEDIUM];
+                            ReplayVertex rv = replay_vertex(sc, op, h, p, vis[b], delta, V(lt.x, lt.y, lt.z), g,
+                                                            mgrad ? mg : nullptr);
+                            if (rv.med_mat >= 0) {
+                                if (!rv.mfinite) drop++;
+                                else for (int k = 0; k < NMEDIUM; ++k) ml[rv.med_mat * NMEDIUM + k] += mg[k];
+                            }
                             if (rv.mat >= 0) {
                                 if (!rv.finite) drop++;
                                 else for (int k = 0; k < NPARAM; ++k) gl[rv.mat * NPARAM + k] += g[k];
@@ -139,7 +146,11 @@ extern "C" int host_adjoint(const pt_scene_view* v, const pt_options* o, const p
                         if (p.rad.x != lt.x || p.rad.y != lt.y || p.rad.z != lt.z) mismatch++;
                     }
 #pragma omp critical
-            for (int i = 0; i < nm * NPARAM; ++i) grad[i] += gl[i];
+            {
+                for (int i = 0; i < nm * NPARAM; ++i) grad[i] += gl[i];
+                if (mgrad)
+                    for (int i = 0; i < nm * NMEDIUM; ++i) mgrad[i] += ml[i];
+            }
         }
         *dropped = drop;
         *mismatch_out = mismatch;

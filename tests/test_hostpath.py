@@ -13,7 +13,7 @@ import hostpath
 import oracle
 from common import oracle_setup, small_instanced
 from glslpt import scenes
-from glslpt._capi import ADJOINT_FIELDS, ADJOINT_NAMES
+from glslpt._capi import ADJOINT_FIELDS, ADJOINT_MEDIUM_FIELDS, ADJOINT_MEDIUM_NAMES, ADJOINT_NAMES
 
 
 def _scene(kind):
@@ -74,6 +74,42 @@ def test_host_adjoint_matches_frozen_fd_of_oracle():
         print(f"mat {m} {ADJOINT_NAMES[k]}: adjoint {g[m, k]:.6g} fd {fd[m, k]:.6g}")
     print(f"params {fd.size}, above 2% of max {int((np.abs(fd) > 0.02 * scale).sum())}, worst {rel.max():.3g}")
     assert (np.abs(fd) > 0.02 * scale).sum() >= 20
+    assert rel.max() <= 0.01
+
+
+def test_host_medium_adjoint_matches_frozen_fd_of_oracle():
+    """Medium gradients (density, colour) of an absorbing, a scattering and an
+    emissive medium against central differences of the oracle's
+    frozen-sampling render, whose scatter and crossing weights carry the
+    free-flight density (distances drawn with the scene's density)."""
+    sc, rs = _scene("media")
+    spp = 8
+    sa, cam, o = oracle_setup(sc, rs)
+    mats = sa.a["materials"]
+    w = np.random.default_rng(43).normal(size=(rs.height, rs.width, 3)).astype(np.float32)
+    g, mg, dropped, mismatch = hostpath.adjoint(sc, rs, w, 0, spp, medium=True)
+    assert dropped == 0 and mismatch == 0
+    media = [m for m in range(mats.shape[0]) if mats[m, 18] != 0]
+    assert sorted(int(mats[m, 18]) for m in media) == [1, 2, 3]
+    assert np.all(mg[[m for m in range(mats.shape[0]) if m not in media]] == 0)
+    eps = 1e-3
+    fd = np.zeros((len(media), 4))
+    for i, m in enumerate(media):
+        for k, f in enumerate(ADJOINT_MEDIUM_FIELDS):
+            mp, mm = mats.copy(), mats.copy()
+            mp[m, f] += eps
+            mm[m, f] -= eps
+            ip = oracle.render_frozen(sa, mp, cam, o, 0, spp)[..., :3].astype(np.float64)
+            im = oracle.render_frozen(sa, mm, cam, o, 0, spp)[..., :3].astype(np.float64)
+            fd[i, k] = np.sum(w * (ip - im)) / (2 * eps) / spp
+    got = mg[media]
+    scale = np.abs(fd).max()
+    rel = np.abs(got - fd) / np.maximum(np.abs(fd), 0.02 * scale)
+    for i, m in enumerate(media):
+        print(f"medium type {int(mats[m, 18])}: " + ", ".join(
+            f"{ADJOINT_MEDIUM_NAMES[k]} {got[i, k]:.6g} / fd {fd[i, k]:.6g}" for k in range(4)))
+    print(f"worst {rel.max():.3g}")
+    assert (np.abs(fd) > 0.02 * scale).sum() >= 10
     assert rel.max() <= 0.01
 
 
