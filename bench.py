@@ -70,6 +70,7 @@ sys.path.insert(0, os.path.join(ROOT, "glsl-pathtracer_amd"))
 
 METRIC = "Msamples/s at 1080p depth 8 (batched vs plain kernel); backward/forward time ratio"
 HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3  # MI355X vector fp32, packed
 B_NODE, B_TRI, B_INST, B_LIGHT, B_RAY = 64, 36, 52, 60, 56
 
 
@@ -352,10 +353,53 @@ def heavy_throughput(steps, warmup, count=True):
                                               "tris": cw["tris_tested"] / max(1, cw["rays_closest"]),
                                               "instances": cw["instances_entered"] / max(1, cw["rays_closest"])},
                                   "rays_per_sample": cw["rays_closest"] / (rs.width * rs.height)}
+    out["adjoint"] = heavy_adjoint_ratio(r, sc, rs, cfg)
     out["tlas_rebuild"] = tlas_rebuild_times(r, sc, a)
     out["env_cdf"] = env_cdf_times(r, a)
     r.close()
     return out
+
+
+def heavy_adjoint_ratio(r, sc, rs, cfg, spp=4, reps=2):
+    """Forward and adjoint device time on the heavy scene with respect to the
+    Disney parameters and the two volumes' density and colour
+    (pt_adjoint_medium): L2 loss against a target rendered from other samples
+    with the volumes 1.5x denser and greyer (colour 0.6)."""
+    import numpy as np
+    from glslpt._capi import PT_MODE_WAVEFRONT
+    from glslpt.renderer import options_for
+    mats = sc.arrays()["materials"]
+    vols = [m for m in range(mats.shape[0]) if mats[m, 18] != 0]
+    tgt = mats.copy()
+    tgt[vols, 19] *= 1.5
+    tgt[vols, 20:23] = 0.6
+    r.set_options(options_for(rs, mode=PT_MODE_WAVEFRONT, **cfg))
+    r.set_camera(sc.camera())
+    r.upload_scene(sc, materials=tgt)
+    r.clear()
+    r.render(1 << 20, spp)
+    r.synchronize()
+    target = r.read_accum()[..., :3] / spp
+    r.upload_scene(sc)
+    fwd, bwd = [], []
+    for i in range(reps + 1):
+        r.clear()
+        r.render(0, spp)
+        r.synchronize()
+        f_ms = r.stats()["ms_last_render"]
+        diff = r.read_accum()[..., :3] / spp - target
+        loss = float(np.mean(diff.astype(np.float64) ** 2))
+        g, mg = r.adjoint_medium(0, spp, ((2.0 / diff.size) * diff).astype(np.float32))
+        st = r.stats()
+        if i > 0:  # the first round is warm-up
+            fwd.append(f_ms)
+            bwd.append(st["ms_last_render"])
+    f_med, b_med = float(np.median(fwd)), float(np.median(bwd))
+    return {"workload": f"the heavy scene, {spp} spp; Disney parameters of {mats.shape[0]} materials and the "
+                        f"density and colour of {len(vols)} volumes",
+            "forward_ms": f_med, "backward_ms": b_med, "backward_over_forward": b_med / f_med, "reps": reps,
+            "loss": loss, "volume_gradient": {int(m): [float(x) for x in mg[m]] for m in vols},
+            "grad_norm": float(np.linalg.norm(g)), "dropped": st["adjoint_dropped"]}
 
 
 def env_cdf_times(r, a, reps=4):
@@ -425,16 +469,49 @@ def tlas_rebuild_times(r, sc, a, reps=5):
             "host_total_ms": hb + hu, "host_over_device": (hb + hu) / d_ms}
 
 
-def measured_traffic(cfg):
-    """HBM bytes per k_extend dispatch from the committed rocprofv3 counter
-    runs (profiles/extend_c2_1080p_traffic.*), when they were taken with the
-    same launch configuration; else None."""
-    path = os.path.join(ROOT, "profiles", "extend_c2_1080p_traffic.json")
+def counter_profile(cfg):
+    """The committed rocprofv3 counter summary of the closest-hit stage on this
+    workload (profiles/extend_c2_1080p_counters.json, written by `python
+    profiling.py summarize`), when it was taken with the same launch
+    configuration; else None."""
+    path = os.path.join(ROOT, "profiles", "extend_c2_1080p_counters.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
         t = json.load(f)
-    return t["extend_hbm_bytes_per_dispatch"] if t.get("launch") == cfg else None
+    return t if t.get("launch") == cfg else None
+
+
+def counter_rooflines(prof, avg_ms):
+    """traffic, l2 and valu entries of the roofline object: the profile's
+    per-dispatch counter figures over the closest-hit kernel's average launch
+    time measured in this run."""
+    if prof is None:
+        return None, None, None
+    per = prof["frame"]["per_dispatch"]
+    cal = prof["calibration"]
+    sec = avg_ms / 1e3
+    src = "profiles/extend_c2_1080p_counters.json"
+    l2_ach = per["l2_bytes"] / sec / 1e9
+    l2 = {"achieved": l2_ach, "peak": cal["l2_read_gb_per_s"], "unit": "GB/s",
+          "frac": l2_ach / cal["l2_read_gb_per_s"], "bytes_per_launch": per["l2_bytes"],
+          "peak_basis": "measured: L2-resident read probe (profiling.py l2read), all CUs", "source": src}
+    # lane operations = VALU wave-instructions x active lanes per instruction
+    lanes = per["SQ_THREAD_CYCLES_VALU"] / per["SQ_ACTIVE_INST_VALU"]
+    lane_ops = per["SQ_INSTS_VALU"] * lanes
+    v_ach = lane_ops / sec / 1e12
+    v_peak = cal["valu_lane_op_per_s"] / 1e12
+    issue = per["SQ_INSTS_VALU"] / sec / cal["valu_wave_instr_per_s"]
+    valu = {"achieved": v_ach, "peak": v_peak, "unit": "Tlane-op/s", "frac": v_ach / v_peak,
+            "issue_frac": issue, "lane_activity": lanes / 64,
+            "lane_activity_camera": prof["kinds"]["camera"]["per_dispatch"]["lane_activity"],
+            "lane_activity_later": prof["kinds"]["later"]["per_dispatch"]["lane_activity"],
+            "fp32_peak_tflops": FP32_PEAK_TFLOPS, "frac_of_fp32_peak": 2 * v_ach / FP32_PEAK_TFLOPS,
+            "peak_basis": "measured: fp32 FMA probe (profiling.py valu), one wave64 VALU instruction per SIMD "
+                          "per 4 cycles; 157.3 TFLOP/s counts packed FP32 (2 per lane), which this build "
+                          "does not emit",
+            "source": src}
+    return per["hbm_bytes"], l2, valu
 
 
 def main():
@@ -509,8 +586,12 @@ def main():
         per_launch = bytes_total / extend_launches
         avg_ms = extend_ms / extend_launches
         achieved = per_launch / (avg_ms / 1e3) / 1e9
+        traffic, l2, valu = counter_rooflines(counter_profile(cfg), avg_ms)
         roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(cfg),
+                    "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "l2": l2, "valu": valu,
+                    "achieved_basis": "algorithmic bytes (64 B per node, 36 B per triangle, 52 B per instance, "
+                                      "60 B per light record, 56 B per ray) per launch over the measured launch "
+                                      "time; most are served by L2, see l2",
                     "kernel": "k_extend (closest hit)", "bytes_per_launch": per_launch,
                     "avg_launch_ms": avg_ms, "launches": extend_launches,
                     "per_ray": {"nodes": cw["nodes_visited"] / max(1, cw["rays_closest"]),

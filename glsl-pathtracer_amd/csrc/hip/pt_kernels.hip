@@ -201,9 +201,13 @@ __device__ __forceinline__ DevOpts job_opts(const DevOpts& op, const JobDev& J) 
 // hits are compacted into the shade queue. At bounce 0 (PRIMARY) the input is
 // every slot of the pass and the camera ray is generated in place; BATCH
 // (bounce 0 of a batched pass) takes the slot count and each slot's job,
-// sample and pixel from the device-side pass state.
+// sample and pixel from the device-side pass state. Seven waves per SIMD (72
+// registers): at eight (64) more values spill and the traversal loop reloads
+// them around its stack pushes, which costs issue slots in a stage bound by
+// VALU issue (1080p: 19.6 -> 17.9 ms per frame; 4K heavy scene: 630 -> 569 ms;
+// six waves, 80 registers, is slower on both).
 template <bool COUNT, bool PRIMARY, bool BATCH>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) k_extend(DevScene sc, DevOpts op, PathBuf pb, const int* __restrict__ qin,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7, 7))) k_extend(DevScene sc, DevOpts op, PathBuf pb, const int* __restrict__ qin,
                                                 const int* count_in, int npaths, int npix, int first_sample,
                                                 int* hitq, int* hit_count, int* work, int S, int* spill,
                                                 unsigned long long* counters, int* ovf_flag, BatchBuf bb) {
