@@ -322,8 +322,11 @@ def heavy_throughput(steps, warmup, count=True):
     cfg = dict(block_size=256, stack_depth=24, wave_spp=4, compact_every=1, bvh_width=2)
     cfg.update(tuned("c2") or {})
     # four-wide BLAS nodes: same hits but for exact ties, and 6% faster on
-    # this scene's deep BLASes (closest-hit 670 -> 633 ms per 4-spp frame)
+    # this scene's deep BLASes (closest-hit 670 -> 633 ms per 4-spp frame);
+    # no closest-hit step cap: resuming this scene's long traversals costs more
+    # than it saves (closest-hit 565 -> 603 ms per 4-spp frame at a cap of 32)
     cfg["bvh_width"] = 4
+    cfg["extend_cap"] = 0
     r = Renderer(0)
     r.upload_scene(sc)
     samples = rs.width * rs.height * rs.spp
@@ -488,7 +491,7 @@ def counter_rooflines(prof, avg_ms):
     time measured in this run."""
     if prof is None:
         return None, None, None
-    per = prof["frame"]["per_dispatch"]
+    per = prof["frame"]["per_step"]
     cal = prof["calibration"]
     sec = avg_ms / 1e3
     src = "profiles/extend_c2_1080p_counters.json"
@@ -504,8 +507,8 @@ def counter_rooflines(prof, avg_ms):
     issue = per["SQ_INSTS_VALU"] / sec / cal["valu_wave_instr_per_s"]
     valu = {"achieved": v_ach, "peak": v_peak, "unit": "Tlane-op/s", "frac": v_ach / v_peak,
             "issue_frac": issue, "lane_activity": lanes / 64,
-            "lane_activity_camera": prof["kinds"]["camera"]["per_dispatch"]["lane_activity"],
-            "lane_activity_later": prof["kinds"]["later"]["per_dispatch"]["lane_activity"],
+            "lane_activity_camera": prof["kinds"]["camera"]["per_step"]["lane_activity"],
+            "lane_activity_later": prof["kinds"]["later"]["per_step"]["lane_activity"],
             "fp32_peak_tflops": FP32_PEAK_TFLOPS, "frac_of_fp32_peak": 2 * v_ach / FP32_PEAK_TFLOPS,
             "peak_basis": "measured: fp32 FMA probe (profiling.py valu), one wave64 VALU instruction per SIMD "
                           "per 4 cycles; 157.3 TFLOP/s counts packed FP32 (2 per lane), which this build "

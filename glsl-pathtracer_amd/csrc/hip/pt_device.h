@@ -799,19 +799,44 @@ __device__ __forceinline__ bool alpha_transparent(const DevScene& sc, int row, i
     return (mode == ALPHA_MODE_MASK && opacity < ap.z) || (mode == ALPHA_MODE_BLEND && rng && rnd(*rng) > opacity);
 }
 
+// Position of a closest-hit traversal between two launches: the node or leaf
+// it processes next (ref), the stack depth (entries [0, sp) are on the
+// Stack), the stack depth at which the current instance's BLAS began (-1 in
+// the TLAS) and that instance. {root, 0, -1, -1} starts a traversal.
+struct TravState { int ref, sp, blas_base, cur_inst; };
+
 // Two-level traversal. ANY: return at the first accepted hit closer than
 // tmax; with `alpha` set, candidates are alpha-tested first (AnyHit with
 // OPT_ALPHA_TEST and no media).
-template <bool ANY, bool COUNT>
+// RESUME (closest hit only): start from *ts (with the stack entries and h
+// holding the best hit so far, tmax = h.t) and stop after `budget` loop
+// iterations (budget < 0: no limit); returns true when the traversal ended
+// and false when it stopped, its position saved in *ts. The iterations and
+// their arithmetic are the same however the traversal is split: the object-
+// space ray of the current instance is recomputed from the same record.
+template <bool ANY, bool COUNT, bool RESUME = false>
 __device__ bool traverse(const DevScene& sc, vec3 O, vec3 D, float tmax, Stack& stk, Hit& h, TravCount& cnt,
-                         bool& overflow, bool alpha = false, Rng* rng = nullptr) {
+                         bool& overflow, bool alpha = false, Rng* rng = nullptr, TravState* ts = nullptr,
+                         int budget = -1) {
+    static_assert(!(ANY && RESUME), "only closest-hit traversals are resumed");
     if (COUNT) cnt.rays++;
     const vec3 invD = V(1.0f / D.x, 1.0f / D.y, 1.0f / D.z);
     vec3 o = O, d = D, inv = invD;
     float best = tmax;
     int sp = 0, blas_base = -1, cur_inst = -1, cur_mat = 0;
     int ref = sc.root;
+    if (RESUME) {
+        ref = ts->ref; sp = ts->sp; blas_base = ts->blas_base; cur_inst = ts->cur_inst;
+        if (blas_base >= 0) {
+            inst_to_object(sc.inst[cur_inst], O, D, o, d);
+            inv = V(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+        }
+    }
     for (;;) {
+        if (RESUME && budget-- == 0) {
+            ts->ref = ref; ts->sp = sp; ts->blas_base = blas_base; ts->cur_inst = cur_inst;
+            return false;
+        }
         bool do_pop = false;
         if (ref >= 0 && blas_base >= 0 && sc.nodes4) {
             // Four-wide BLAS node: the hit children in order of the reference's
@@ -924,7 +949,7 @@ __device__ bool traverse(const DevScene& sc, vec3 O, vec3 D, float tmax, Stack& 
             ref = stk.pop(--sp);
         }
     }
-    return false;
+    return RESUME;
 }
 
 // Light loop of ClosestHit (closest_hit.glsl:35-85): closest analytic light.

@@ -20,6 +20,20 @@ struct PathBuf {
     int2* hit2;    // instance, light
 };
 
+// Closest-hit traversals the capped extend stage stopped (pt_options.
+// extend_cap), resumed by a second, compacted launch. Record r: the path
+// slot, the traversal position, the best hit so far and the stack entries
+// (entry k at stack[k * cap + r]).
+constexpr int CONT_STACK = 24;  // a traversal deeper than this is not stopped
+struct ContBuf {
+    int* slot;
+    int4* st;     // ref, sp, blas_base, cur_inst
+    float4* hf;   // best hit t, u, v
+    int4* hi;     // best hit triangle row, instance, light
+    int* stack;   // [CONT_STACK * cap]
+    int cap;      // records
+};
+
 // Shadow (NEE visibility) queue: one record per path with up to two rays
 // (k = 0 environment, 1 analytic light) from the same scatter point.
 struct ShadowBuf {
@@ -91,7 +105,13 @@ void launch_plain(hipStream_t s, int blocks, int bs, const ptd::DevScene& sc, co
 void launch_extend(hipStream_t s, int blocks, int bs, const ptd::DevScene& sc, const ptd::DevOpts& op,
                    const ptd::PathBuf& pb, bool primary, const int* qin, const int* count_in, int npaths, int npix,
                    int first_sample, int* hitq, int* hit_count, int* work, int S, int* spill,
-                   unsigned long long* counters, int* ovf, bool count);
+                   unsigned long long* counters, int* ovf, bool count, int budget = 0,
+                   const ptd::ContBuf* cb = nullptr, int* cont_count = nullptr);
+// A later bounce with budget > 0 (and not counting): traversals stop after
+// `budget` iterations into cb; launch_extend_resume then finishes them.
+void launch_extend_resume(hipStream_t s, int blocks, int bs, const ptd::DevScene& sc, const ptd::DevOpts& op,
+                          const ptd::PathBuf& pb, const ptd::ContBuf& cb, const int* cont_count, int* hitq,
+                          int* hit_count, int* work, int S, int* spill, int* ovf);
 // Shade stage; bb: the batched form's pass state (null: one frame); inline_nee:
 // NEE visibility resolved in the stage (stochastic alpha, volume MIS), else
 // queued as shadow records. S and spill: the traversal stack (inline_nee).
@@ -137,12 +157,12 @@ size_t env_cdf_scratch_bytes(int n);
 // translation unit compiled them; the host side compares them with its own
 // at pt_create, so a library linked from objects built against different
 // headers refuses to run instead of handing the kernels misread pointers.
-struct LayoutSizes { size_t scene, opts, path, shadow, batch, replay, inst, node, job; };
+struct LayoutSizes { size_t scene, opts, path, shadow, batch, replay, inst, node, job, cont; };
 LayoutSizes layout_sizes();
 inline LayoutSizes layout_sizes_here() {
     return LayoutSizes{sizeof(ptd::DevScene), sizeof(ptd::DevOpts), sizeof(ptd::PathBuf), sizeof(ptd::ShadowBuf),
                        sizeof(ptd::BatchBuf), sizeof(ptd::ReplayBuf), sizeof(ptd::GInst), sizeof(ptd::GNode),
-                       sizeof(ptd::JobDev)};
+                       sizeof(ptd::JobDev), sizeof(ptd::ContBuf)};
 }
 void launch_env_cdf(hipStream_t s, const float* img, int n, float* cdf, float* total, int* ok, void* scratch);
 void launch_fill(hipStream_t s, int* p, int n, int v);

@@ -196,6 +196,37 @@ __device__ __forceinline__ DevOpts job_opts(const DevOpts& op, const JobDev& J) 
     return o;
 }
 
+// End of a path's closest-hit step: a miss finishes the path here (environment
+// or uniform light), a hit is recorded for the shade stage (returns true).
+template <bool PRIMARY>
+__device__ __forceinline__ bool finish_extend(const DevScene& sc, const DevOpts& op, const PathBuf& pb, int slot,
+                                              PathRegs& p, const Hit& h) {
+    if (h.t == INF) {
+        if (!PRIMARY) {
+            const float4 o4 = pb.ro[slot], t4 = pb.thr[slot], r4 = pb.rad[slot];
+            p.prev_pdf = o4.w; p.thr = xyz(t4); p.rad = xyz(r4); p.alpha = r4.w;
+            p.surface_scatter = (pb.meta[slot].y >> 17) & 1;
+        }
+        on_miss(sc, op, p);
+        pb.rad[slot] = F4(p.rad, p.alpha);
+        if (!PRIMARY) pb.meta[slot].y = PATH_DEAD;
+        return false;
+    }
+    pb.hit[slot] = make_float4(h.t, h.u, h.v, __int_as_float(h.tri));
+    pb.hit2[slot] = make_int2(h.inst, h.light);
+    return true;
+}
+
+// Record r of the continuation buffer: a stopped traversal and its stack.
+__device__ __forceinline__ void save_cont(const ContBuf& cb, int r, int slot, const TravState& ts, const Hit& h,
+                                          Stack& stk) {
+    cb.slot[r] = slot;
+    cb.st[r] = make_int4(ts.ref, ts.sp, ts.blas_base, ts.cur_inst);
+    cb.hf[r] = make_float4(h.t, h.u, h.v, 0.f);
+    cb.hi[r] = make_int4(h.tri, h.inst, h.light, 0);
+    for (int k = 0; k < ts.sp; ++k) cb.stack[static_cast<size_t>(k) * cb.cap + r] = stk.pop(k);
+}
+
 // Extend: closest hit of every queued path. Misses finish here (environment
 // or uniform light, pathtrace.glsl:305-339) and write only their radiance;
 // hits are compacted into the shade queue. At bounce 0 (PRIMARY) the input is
@@ -206,11 +237,18 @@ __device__ __forceinline__ DevOpts job_opts(const DevOpts& op, const JobDev& J) 
 // them around its stack pushes, which costs issue slots in a stage bound by
 // VALU issue (1080p: 19.6 -> 17.9 ms per frame; 4K heavy scene: 630 -> 569 ms;
 // six waves, 80 registers, is slower on both).
-template <bool COUNT, bool PRIMARY, bool BATCH>
+// CAPPED (later bounces): a traversal runs at most `budget` loop iterations
+// here; a wave does not wait on its longest rays. Stopped traversals go to the
+// continuation buffer (while it has room and their stack is at most
+// CONT_STACK deep; otherwise they finish here) and k_extend_resume finishes
+// them, densely packed. The hits are the same either way.
+template <bool COUNT, bool PRIMARY, bool BATCH, bool CAPPED = false>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7, 7))) k_extend(DevScene sc, DevOpts op, PathBuf pb, const int* __restrict__ qin,
                                                 const int* count_in, int npaths, int npix, int first_sample,
                                                 int* hitq, int* hit_count, int* work, int S, int* spill,
-                                                unsigned long long* counters, int* ovf_flag, BatchBuf bb) {
+                                                unsigned long long* counters, int* ovf_flag, BatchBuf bb,
+                                                int budget, ContBuf cb, int* cont_count) {
+    static_assert(!CAPPED || (!COUNT && !PRIMARY), "only uninstrumented later bounces are capped");
     extern __shared__ int lds[];
     Stack stk{lds, spill, static_cast<int>(threadIdx.x), static_cast<int>(blockDim.x), S,
               blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x,
@@ -220,12 +258,14 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7, 7))
     const int count = PRIMARY ? (BATCH ? bb.cur[3] : npaths) : *count_in;
     for (WorkCursor<PRIMARY ? RUN_CAMERA : RUN_TRAVERSE> wc; wc.base < count; wc.next(work)) {
         const int i = wc.base + lane_id();
-        bool keep = false;
+        bool keep = false, valid = false, done = false;
         int slot = -1;
+        PathRegs p;
+        Hit h;
+        TravState ts;
         if (i < count) {
             slot = PRIMARY ? i : qin[i];
-            PathRegs p;
-            bool valid = true;
+            valid = true;
             if (PRIMARY && BATCH) {
                 int job, smp, item, px, py;
                 batch_decode(bb, slot, job, smp, item);
@@ -247,27 +287,69 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7, 7))
                 }
             }
             if (valid) {
-                Hit h = closest<COUNT>(sc, op, p.ro, p.rd, p.depth, stk, cnt, ovf);
-                if (h.t == INF) {
-                    if (!PRIMARY) {
-                        const float4 o4 = pb.ro[slot], t4 = pb.thr[slot], r4 = pb.rad[slot];
-                        p.prev_pdf = o4.w; p.thr = xyz(t4); p.rad = xyz(r4); p.alpha = r4.w;
-                        p.surface_scatter = (pb.meta[slot].y >> 17) & 1;
-                    }
-                    on_miss(sc, op, p);
-                    pb.rad[slot] = F4(p.rad, p.alpha);
-                    if (!PRIMARY) pb.meta[slot].y = PATH_DEAD;
+                if (CAPPED) {
+                    done = closest_begin(sc, op, p.ro, p.rd, p.depth, stk, ovf, h, ts, budget);
                 } else {
-                    pb.hit[slot] = make_float4(h.t, h.u, h.v, __int_as_float(h.tri));
-                    pb.hit2[slot] = make_int2(h.inst, h.light);
-                    keep = true;
+                    h = closest<COUNT>(sc, op, p.ro, p.rd, p.depth, stk, cnt, ovf);
+                    done = true;
                 }
             }
         }
+        if (CAPPED) {
+            bool susp = valid && !done && ts.sp <= CONT_STACK;
+            const int r = wave_append(susp, cont_count);
+            if (susp && r >= cb.cap) susp = false;  // no room: finish here
+            if (susp) {
+                save_cont(cb, r, slot, ts, h, stk);
+            } else if (valid && !done) {
+                traverse<false, false, true>(sc, p.ro, p.rd, h.t, stk, h, cnt, ovf, false, nullptr, &ts, -1);
+                done = true;
+            }
+        }
+        if (valid && done) keep = finish_extend<PRIMARY>(sc, op, pb, slot, p, h);
         const int pos = wave_append(keep, hit_count);
         if (pos >= 0) hitq[pos] = slot;
     }
     flush_counts<COUNT>(cnt, counters);
+    if (ovf) atomicOr(ovf_flag, 1);
+}
+
+// The traversals a capped extend stage stopped, packed densely: each record's
+// stack back on this thread's stack, the traversal run to its end, then the
+// same finish as the extend stage (records [0, min(*cont_count, cap))).
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7, 7)))
+k_extend_resume(DevScene sc, DevOpts op, PathBuf pb, ContBuf cb, const int* cont_count, int* hitq, int* hit_count,
+                int* work, int S, int* spill, int* ovf_flag) {
+    extern __shared__ int lds[];
+    Stack stk{lds, spill, static_cast<int>(threadIdx.x), static_cast<int>(blockDim.x), S,
+              blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x,
+              static_cast<size_t>(gridDim.x) * blockDim.x};
+    TravCount cnt{0, 0, 0, 0, 0, 0};
+    bool ovf = false;
+    const int count = min(*cont_count, cb.cap);
+    for (WorkCursor<RUN_TRAVERSE> wc; wc.base < count; wc.next(work)) {
+        const int r = wc.base + lane_id();
+        bool keep = false;
+        int slot = -1;
+        if (r < count) {
+            slot = cb.slot[r];
+            const int4 st = cb.st[r];
+            TravState ts{st.x, st.y, st.z, st.w};
+            const float4 hf = cb.hf[r];
+            const int4 hi = cb.hi[r];
+            Hit h;
+            h.t = hf.x; h.u = hf.y; h.v = hf.z; h.tri = hi.x; h.inst = hi.y; h.light = hi.z;
+            for (int k = 0; k < ts.sp; ++k) stk.push(k, cb.stack[static_cast<size_t>(k) * cb.cap + r]);
+            PathRegs p;
+            const float4 o4 = pb.ro[slot], d4 = pb.rd[slot];
+            p.ro = xyz(o4); p.rd = xyz(d4);
+            p.depth = pb.meta[slot].x;
+            traverse<false, false, true>(sc, p.ro, p.rd, h.t, stk, h, cnt, ovf, false, nullptr, &ts, -1);
+            keep = finish_extend<false>(sc, op, pb, slot, p, h);
+        }
+        const int pos = wave_append(keep, hit_count);
+        if (pos >= 0) hitq[pos] = slot;
+    }
     if (ovf) atomicOr(ovf_flag, 1);
 }
 
@@ -1220,13 +1302,25 @@ void launch_plain(hipStream_t s, int blocks, int bs, const DevScene& sc, const D
 void launch_extend(hipStream_t s, int blocks, int bs, const DevScene& sc, const DevOpts& op, const PathBuf& pb,
                    bool primary, const int* qin, const int* count_in, int npaths, int npix, int first_sample,
                    int* hitq, int* hit_count, int* work, int S, int* spill, unsigned long long* counters, int* ovf,
-                   bool count) {
+                   bool count, int budget, const ContBuf* cb, int* cont_count) {
     const size_t lds = static_cast<size_t>(S) * bs * sizeof(int);
     const BatchBuf nb{};
-#define PT_EXT(C, P) hipLaunchKernelGGL((k_extend<C, P, false>), dim3(blocks), dim3(bs), lds, s, sc, op, pb, qin, count_in, npaths, npix, first_sample, hitq, hit_count, work, S, spill, counters, ovf, nb)
+    const ContBuf nc{};
+    if (budget > 0 && !primary && !count && cb && cont_count) {
+        hipLaunchKernelGGL((k_extend<false, false, false, true>), dim3(blocks), dim3(bs), lds, s, sc, op, pb, qin, count_in, npaths, npix, first_sample, hitq, hit_count, work, S, spill, counters, ovf, nb, budget, *cb, cont_count);
+        return;
+    }
+#define PT_EXT(C, P) hipLaunchKernelGGL((k_extend<C, P, false>), dim3(blocks), dim3(bs), lds, s, sc, op, pb, qin, count_in, npaths, npix, first_sample, hitq, hit_count, work, S, spill, counters, ovf, nb, 0, nc, nullptr)
     if (count) { if (primary) PT_EXT(true, true); else PT_EXT(true, false); }
     else { if (primary) PT_EXT(false, true); else PT_EXT(false, false); }
 #undef PT_EXT
+}
+
+void launch_extend_resume(hipStream_t s, int blocks, int bs, const DevScene& sc, const DevOpts& op, const PathBuf& pb,
+                          const ContBuf& cb, const int* cont_count, int* hitq, int* hit_count, int* work, int S,
+                          int* spill, int* ovf) {
+    const size_t lds = static_cast<size_t>(S) * bs * sizeof(int);
+    hipLaunchKernelGGL(k_extend_resume, dim3(blocks), dim3(bs), lds, s, sc, op, pb, cb, cont_count, hitq, hit_count, work, S, spill, ovf);
 }
 
 void launch_shade(hipStream_t s, int blocks, int bs, const DevScene& sc, const DevOpts& op, const PathBuf& pb,
@@ -1268,7 +1362,7 @@ void launch_pass_end(hipStream_t s, const BatchBuf& bb) {
 void launch_extend_batch(hipStream_t s, int blocks, int bs, const DevScene& sc, const DevOpts& op, const PathBuf& pb,
                          const BatchBuf& bb, int* hitq, int* hit_count, int* work, int S, int* spill, int* ovf) {
     const size_t lds = static_cast<size_t>(S) * bs * sizeof(int);
-    hipLaunchKernelGGL((k_extend<false, true, true>), dim3(blocks), dim3(bs), lds, s, sc, op, pb, nullptr, nullptr, 0, 0, 0, hitq, hit_count, work, S, spill, nullptr, ovf, bb);
+    hipLaunchKernelGGL((k_extend<false, true, true>), dim3(blocks), dim3(bs), lds, s, sc, op, pb, nullptr, nullptr, 0, 0, 0, hitq, hit_count, work, S, spill, nullptr, ovf, bb, 0, ContBuf{}, nullptr);
 }
 
 void launch_accum_batch(hipStream_t s, int blocks, const PathBuf& pb, const BatchBuf& bb) {

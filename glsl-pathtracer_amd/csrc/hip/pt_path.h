@@ -61,6 +61,18 @@ __device__ __forceinline__ Hit closest(const DevScene& sc, const DevOpts& op, ve
     return h;
 }
 
+// closest() in pieces: the analytic lights, then at most `budget` traversal
+// iterations (budget < 0: all). Returns false when stopped, with the position
+// in ts and the best hit so far in h; traverse<false, false, true> resumes it.
+__device__ __forceinline__ bool closest_begin(const DevScene& sc, const DevOpts& op, vec3 ro, vec3 rd, int depth,
+                                              Stack& stk, bool& ovf, Hit& h, TravState& ts, int budget) {
+    TravCount cnt{0, 0, 0, 0, 0, 0};
+    h.t = INF; h.u = 0.f; h.v = 0.f; h.tri = -1; h.inst = -1; h.light = -1;
+    if ((op.flags & OPT_LIGHTS) && (!(op.flags & OPT_HIDE_EMITTERS) || depth > 0)) hit_lights<false>(sc, ro, rd, h, cnt);
+    ts = TravState{sc.root, 0, -1, -1};
+    return traverse<false, false, true>(sc, ro, rd, h.t, stk, h, cnt, ovf, false, nullptr, &ts, budget);
+}
+
 // EvalTransmittance (pathtrace.glsl:118-156): product of medium
 // transmittances along a shadow ray, zero when an opaque surface blocks it.
 template <bool COUNT>

@@ -142,6 +142,10 @@ struct pt_renderer_s {
     DevBuf<int> b_cur;
     DevBuf<float4> b_accum;
     DevBuf<int> qa, qb, hq, counters;
+    // capped extend stage: stopped traversals (ptd::ContBuf)
+    DevBuf<int> c_slot, c_stack;
+    DevBuf<int4> c_st, c_hi;
+    DevBuf<float4> c_hf;
     DevBuf<int> spill;
     DevBuf<unsigned long long> work_counts;  // [0,6) closest-hit work, [8,14) shadow work: nodes, tris, insts, lights, spills, rays
     DevBuf<int> ovf;
@@ -167,6 +171,7 @@ struct pt_renderer_s {
         p_ro.free(); p_rd.free(); p_thr.free(); p_rad.free(); p_aux.free(); p_med.free(); p_hit.free();
         p_rng.free(); p_meta.free(); p_hit2.free(); s_idx.free(); s_o.free(); s_t.free(); s_d.free(); s_c.free();
         a_rad.free(); a_vis.free(); a_dimg.free(); a_grad.free(); a_flags.free();
+        c_slot.free(); c_stack.free(); c_st.free(); c_hi.free(); c_hf.free();
         b_jobs.free(); b_planes.free(); b_segs.free(); b_passes.free(); b_cur.free(); b_accum.free();
         work_counts.free(); ldr.free();
         t_xf.free(); t_local.free(); t_box.free(); t_cen.free(); t_blas.free(); t_blas4.free(); t_mat.free(); t_ord.free();
@@ -261,6 +266,16 @@ struct pt_renderer_s {
         work_counts.ensure(16);
         ovf.ensure(4);
     }
+    // Continuation buffer for passes of up to max_paths paths: room for a
+    // quarter of them (later bounces hold only the paths still alive, and of
+    // those the capped stage stops the slow tail; a traversal that finds no
+    // room finishes in the capped stage).
+    ptd::ContBuf cont_buf(size_t max_paths) {
+        const size_t cap = std::min<size_t>(std::max<size_t>(max_paths / 4, 64), INT32_MAX / ptd::CONT_STACK);
+        c_slot.ensure(cap); c_st.ensure(cap); c_hf.ensure(cap); c_hi.ensure(cap);
+        c_stack.ensure(cap * ptd::CONT_STACK);
+        return ptd::ContBuf{c_slot.p, c_st.p, c_hf.p, c_hi.p, c_stack.p, static_cast<int>(cap)};
+    }
 };
 
 namespace {
@@ -305,7 +320,11 @@ void check_ready(pt_renderer_s* h) {
 //   [D+1, 2D+2)         hits entering shade at bounce b
 //   [2D+2, 3D+3)        shadow rays of bounce b
 //   [3D+3, 6D+6)        work counters (extend, shade, shadow) of bounce b
-int counter_cells(int D) { return 6 * D + 6; }
+// Per pass: extend-queue counts [0, D], hit counts [D+1, 2D+1], shadow counts
+// [2D+2, 3D+2], three work counters per bounce [3D+3, 6D+3), then per bounce
+// the continuation count and the resume stage's work counter [6D+6, 8D+6).
+int counter_cells(int D) { return 8 * D + 6; }
+int* cont_cells(int* cnt, int D, int b) { return cnt + 6 * D + 6 + 2 * b; }
 
 // NEE visibility is resolved inside the shade stage (shadow records are not
 // used) when AnyHit draws from the path's random numbers between NEE sampling
@@ -336,6 +355,8 @@ struct Wavefront {
     int npix, spp_pass, D, bs, S, b_ext, b_sha, b_sdw;
     bool cw, inline_nee;
     size_t max_paths;
+    int cap;            // extend_cap of later bounces (0: off)
+    ptd::ContBuf cb;    // continuation buffer when cap > 0
 };
 
 // adjoint: NEE always in the shadow stage, which records the visibility bits
@@ -364,6 +385,9 @@ Wavefront wavefront_setup(pt_renderer_s* h, const DevOpts& op, int num_samples, 
     w.S = h->stack_S();
     w.cw = h->opts.count_work != 0;
     w.b_ext = ptk::persistent_blocks(ptk::KIND_EXTEND, w.bs, w.S, w.cw, h->num_cu);
+    // instrumented runs and the adjoint's passes traverse uncapped
+    w.cap = (adjoint || w.cw) ? 0 : h->opts.extend_cap;
+    w.cb = w.cap > 0 ? h->cont_buf(max_paths) : ptd::ContBuf{};
     w.b_sha = ptk::persistent_blocks(w.inline_nee ? ptk::KIND_SHADE_INLINE : ptk::KIND_SHADE, w.bs, w.S, w.cw, h->num_cu);
     w.b_sdw = ptk::persistent_blocks(ptk::KIND_SHADOW, w.bs, w.S, w.cw, h->num_cu);
     // every traversing grid (extend, shadow, an inline-NEE shade) indexes the
@@ -396,9 +420,15 @@ void run_bounces(pt_renderer_s* h, const DevOpts& op, const Wavefront& w, const 
         // extend re-reads this queue and skips ended paths.
         const bool compact = primary || (b % cadence) == 0;
         int* c_next = compact ? cnt + b + 1 : c_ext;
+        int* c_cont = cont_cells(cnt, D, b);
+        const bool capped = w.cap > 0 && !primary;
         h->stage_begin(PT_STAGE_EXTEND, ev);
         ptk::launch_extend(s, w.b_ext, w.bs, h->sc, op, pb, primary, qin, c_ext, npaths, w.npix, fs, h->hq.p, c_hit,
-                           wk + 0, w.S, h->spill.p, h->work_counts.p, h->ovf.p, w.cw);
+                           wk + 0, w.S, h->spill.p, h->work_counts.p, h->ovf.p, w.cw, capped ? w.cap : 0, &w.cb,
+                           c_cont);
+        if (capped)
+            ptk::launch_extend_resume(s, w.b_ext, w.bs, h->sc, op, pb, w.cb, c_cont, h->hq.p, c_hit, c_cont + 1, w.S,
+                                      h->spill.p, h->ovf.p);
         h->stage_end(PT_STAGE_EXTEND, ev);
         h->stage_begin(PT_STAGE_SHADE, ev);
         ptk::launch_shade(s, w.b_sha, w.bs, h->sc, op, pb, sb, nullptr, primary, w.inline_nee, h->hq.p, c_hit, w.npix,
@@ -659,6 +689,8 @@ void batch_render(pt_renderer_s* h, const pt_job* jobs, int n) {
     h->ensure_misc();
     const int bs = h->block_size(), S = h->stack_S();
     const int b_ext = ptk::persistent_blocks(ptk::KIND_EXTEND, bs, S, false, h->num_cu);
+    const int cap = h->opts.extend_cap;
+    const ptd::ContBuf cb = cap > 0 ? h->cont_buf(max_paths) : ptd::ContBuf{};
     const int b_sha = ptk::persistent_blocks(inl ? ptk::KIND_SHADE_BATCH_INLINE : ptk::KIND_SHADE_BATCH, bs, S, false,
                                              h->num_cu);
     const int b_sdw = ptk::persistent_blocks(ptk::KIND_SHADOW, bs, S, false, h->num_cu);
@@ -700,10 +732,17 @@ void batch_render(pt_renderer_s* h, const pt_job* jobs, int n) {
             const bool primary = b == 0;
             const bool compact = primary || (b % cadence) == 0;
             int* c_next = compact ? cnt + b + 1 : c_ext;
+            int* c_cont = cont_cells(cnt, D, b);
             stage(PT_STAGE_EXTEND, [&] {
-                if (primary) ptk::launch_extend_batch(s, b_ext, bs, h->sc, op, pb, bb, h->hq.p, c_hit, wk, S, h->spill.p, h->ovf.p);
-                else ptk::launch_extend(s, b_ext, bs, h->sc, op, pb, false, qin, c_ext, 0, 0, 0, h->hq.p, c_hit, wk, S,
-                                        h->spill.p, h->work_counts.p, h->ovf.p, false);
+                if (primary) {
+                    ptk::launch_extend_batch(s, b_ext, bs, h->sc, op, pb, bb, h->hq.p, c_hit, wk, S, h->spill.p, h->ovf.p);
+                } else {
+                    ptk::launch_extend(s, b_ext, bs, h->sc, op, pb, false, qin, c_ext, 0, 0, 0, h->hq.p, c_hit, wk, S,
+                                       h->spill.p, h->work_counts.p, h->ovf.p, false, cap, &cb, c_cont);
+                    if (cap > 0)
+                        ptk::launch_extend_resume(s, b_ext, bs, h->sc, op, pb, cb, c_cont, h->hq.p, c_hit, c_cont + 1,
+                                                  S, h->spill.p, h->ovf.p);
+                }
             });
             stage(PT_STAGE_SHADE, [&] {
                 ptk::launch_shade(s, b_sha, bs, h->sc, op, pb, sb, &bb, primary, inl, h->hq.p, c_hit, 0, 0, qout,
@@ -1201,6 +1240,7 @@ int pt_set_options(pt_handle h, const pt_options* o) {
         if (o->bvh_width != 0 && o->bvh_width != 2 && o->bvh_width != 4)
             return Fail(h, PT_ERR_INVALID_ARG, "bvh_width must be 2 or 4");
         if (o->nee_inline != 0 && o->nee_inline != 1) return Fail(h, PT_ERR_INVALID_ARG, "nee_inline must be 0 or 1");
+        if (o->extend_cap < 0 || o->extend_cap > 4096) return Fail(h, PT_ERR_INVALID_ARG, "extend_cap must be in [0, 4096]");
         HIPCHK(hipSetDevice(h->device));
         h->opts = *o;
         h->has_opts = true;

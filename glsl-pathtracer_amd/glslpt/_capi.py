@@ -80,7 +80,7 @@ class PtOptions(C.Structure):
                 ("enable_volume_mis", i32), ("opengl_normal_map", i32),
                 ("mode", i32), ("block_size", i32), ("stack_depth", i32), ("wave_spp", i32),
                 ("use_graph", i32), ("count_work", i32), ("compact_every", i32), ("bvh_width", i32),
-                ("nee_inline", i32)]
+                ("nee_inline", i32), ("extend_cap", i32)]
 
 
 class PtStats(C.Structure):

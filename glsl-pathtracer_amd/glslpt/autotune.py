@@ -1,8 +1,9 @@
 """Launch-configuration autotuner for the wavefront renderer.
 
 Sweeps workgroup size {64, 128, 256}, LDS traversal-stack depth {16, 24, 32},
-samples per wavefront pass, queue-compaction cadence, BLAS node width {2, 4}
-and where NEE rays are traced (shadow stage or inside shade) on the actual
+samples per wavefront pass, queue-compaction cadence, BLAS node width {2, 4},
+where NEE rays are traced (shadow stage or inside shade) and the closest-hit
+step cap of later bounces (0 or 32; the 1080p workload only) on the actual
 workload, timing it on the GPU, and stores the fastest configuration as JSON
 in glslpt/tuning/<key>.json, where bench.py and callers look it up.
 
@@ -25,11 +26,12 @@ TUNING_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning")
 BLOCK_SIZES = (64, 128, 256)
 STACK_DEPTHS = (16, 24, 32)
 QUICK = dict(block_size=(64, 128), stack_depth=(16, 24), wave_spp=(16, 32, 64), compact_every=(1, 2),
-             bvh_width=(2, 4), nee_inline=(0, 1))
+             bvh_width=(2, 4), nee_inline=(0, 1), extend_cap=(0, 32))
 WAVE_SPP = (8, 16, 32, 64)
 CADENCE = (1, 2)
 BVH_WIDTHS = (2, 4)
 NEE_INLINE = (0, 1)
+EXTEND_CAP = (0, 32)
 
 
 def tuned(key):
@@ -56,7 +58,7 @@ def _time(r, sc, rs, spp, reps, **kw):
 
 def sweep(r, sc, rs, spp=16, reps=2, grid=None, log=print):
     grid = grid or dict(block_size=BLOCK_SIZES, stack_depth=STACK_DEPTHS, wave_spp=WAVE_SPP, compact_every=CADENCE,
-                        bvh_width=BVH_WIDTHS, nee_inline=NEE_INLINE)
+                        bvh_width=BVH_WIDTHS, nee_inline=NEE_INLINE, extend_cap=EXTEND_CAP)
     keys = list(grid)
     results = []
     for vals in itertools.product(*[grid[k] for k in keys]):

@@ -169,6 +169,10 @@ typedef struct pt_options {
                                 itself (no shadow stage); 0 = a shadow stage does, except where the
                                 scene needs them inline (stochastic alpha, volume MIS). Same image
                                 either way; pt_adjoint always uses the shadow stage */
+    int32_t extend_cap;      /* wavefront and batched forms, bounces after the first: a closest-hit
+                                traversal stops after this many steps and a second, densely packed
+                                launch finishes the stopped ones, so waves do not wait on their
+                                slowest rays (0 = off). Same hits, same image */
 } pt_options;
 
 typedef struct pt_stats {

@@ -338,20 +338,30 @@ def summarize(root, out_json):
       l2rate.json, valu.json  the probes' own timed lines (no profiler)
       c2.json                 the line of `profiling.py c2 64` without a profiler
                               (launch configuration, stage times)
-    Writes OUT_JSON with the per-dispatch averages the bench line uses."""
+    Writes OUT_JSON with the per-step averages (one step = one closest-hit
+    stage launch of the renderer, which may run two kernels) the bench line
+    uses."""
     import json
-    ext = {}
+    # every k_extend* dispatch of the frame: the camera-ray launch, and for later
+    # bounces the (capped) closest-hit launch and, with a step cap, its resume
+    # launch; figures are per closest-hit stage step (the stage's launches in
+    # the unprofiled frame: one camera-ray step, the rest later bounces)
+    frame = json.load(open(os.path.join(root, "c2.json")))
+    steps = frame["launches"]["extend"]
+    kinds = {"camera": {"kernels": set(), "dispatches": 0, "steps": 1, "sums": {}},
+             "later": {"kernels": set(), "dispatches": 0, "steps": steps - 1, "sums": {}}}
+    seen = set()
     for sub in ("fetch", "write", "tcc", "sq"):
         for name, ds in _one(root, sub).items():
+            kind = "camera" if name.endswith("<false, true, false>") else "later"
+            k = kinds[kind]
+            k["kernels"].add(name)
             for did, cv in ds.items():
-                ext.setdefault(name, {}).setdefault(sub, []).append(cv)
-    kinds = {}
-    for name, subs in ext.items():
-        kind = "camera" if name.endswith("<false, true, false>") else "later"
-        k = kinds.setdefault(kind, {"kernel": name, "dispatches": 0, "sums": {}})
-        k["dispatches"] = len(subs["tcc"])
-        for sub, lst in subs.items():
-            for cv in lst:
+                if (sub, did) in seen:
+                    continue
+                seen.add((sub, did))
+                if sub == "tcc":
+                    k["dispatches"] += 1
                 for c, v in cv.items():
                     k["sums"][c] = k["sums"].get(c, 0.0) + v
     cf = _one(root, "copy_fetch")["__amd_rocclr_copyBuffer"]
@@ -361,7 +371,6 @@ def summarize(root, out_json):
     write_scale = nbytes / (next(iter(cw.values()))["WRITE_SIZE"] * 1024)
     l2 = json.load(open(os.path.join(root, "l2rate.json")))
     vl = json.load(open(os.path.join(root, "valu.json")))
-    frame = json.load(open(os.path.join(root, "c2.json")))
     l2req = _one(root, "l2req")["l2_read"]
     req = sum(d["TCC_REQ_sum"] for d in l2req.values()) / len(l2req)
     bytes_per_req = l2["read_bytes_per_launch"] / req
@@ -377,17 +386,16 @@ def summarize(root, out_json):
                                "valu_lane_op_per_s": vl["lane_fma_per_s"]}}
     tot = {}
     for kind, k in kinds.items():
-        n, sm = k["dispatches"], k["sums"]
+        n, sm = k["steps"], k["sums"]
         per = {c: v / n for c, v in sm.items()}
         per["hbm_bytes"] = per["FETCH_SIZE"] * 1024 * fetch_scale + per["WRITE_SIZE"] * 1024 * write_scale
         per["l2_bytes"] = per["TCC_REQ_sum"] * bytes_per_req
         per["lane_activity"] = sm["SQ_THREAD_CYCLES_VALU"] / (64 * sm["SQ_ACTIVE_INST_VALU"])
-        summary["kinds"][kind] = {"kernel": k["kernel"], "dispatches": n, "per_dispatch": per}
+        summary["kinds"][kind] = {"kernels": sorted(k["kernels"]), "dispatches": k["dispatches"], "steps": n,
+                                  "per_step": per}
         for c in ("hbm_bytes", "l2_bytes", "SQ_INSTS_VALU", "SQ_THREAD_CYCLES_VALU", "SQ_ACTIVE_INST_VALU"):
             tot[c] = tot.get(c, 0.0) + per[c] * n
-        tot["dispatches"] = tot.get("dispatches", 0) + n
-    nd = tot.pop("dispatches")
-    summary["frame"] = {"dispatches": nd, "per_dispatch": {c: v / nd for c, v in tot.items()},
+    summary["frame"] = {"steps": steps, "per_step": {c: v / steps for c, v in tot.items()},
                         "lane_activity": tot["SQ_THREAD_CYCLES_VALU"] / (64 * tot["SQ_ACTIVE_INST_VALU"])}
     with open(out_json, "w") as f:
         json.dump(summary, f, indent=1)

@@ -75,3 +75,21 @@ def adjoint(sc, rs, dl_dimage, first, n, materials=None, medium=False, **opts):
     if medium:
         return g, mg, dropped.value, mismatch.value
     return g, dropped.value, mismatch.value
+
+
+def closest_split(sc, rs, origins, dirs, budget, **opts):
+    """(rays whose hit differs in any bit between the whole traversal and the
+    traversal in pieces of `budget` iterations with the stack moved between
+    pieces, pieces traced) on the CPU."""
+    v, _ = _view(sc)
+    o = options_for(rs, **opts)
+    cam = sc.camera()
+    ro = np.ascontiguousarray(origins, np.float32)
+    rd = np.ascontiguousarray(dirs, np.float32)
+    differ, pieces = C.c_int(0), C.c_longlong(0)
+    rc = lib().host_closest_split(C.byref(v), C.byref(o), C.byref(cam), int(ro.shape[0]),
+                                  ro.ctypes.data_as(P(C.c_float)), rd.ctypes.data_as(P(C.c_float)), int(budget),
+                                  C.byref(differ), C.byref(pieces))
+    if rc != 0:
+        raise RuntimeError("host_closest_split failed")
+    return differ.value, pieces.value

@@ -8,6 +8,7 @@
 #include <omp.h>
 
 #include <cstdio>
+#include <cstring>
 #include <vector>
 #include "pt_path.h"
 #include "pt_replay.h"
@@ -190,6 +191,60 @@ extern "C" int host_render(const pt_scene_view* v, const pt_options* o, const pt
         return 0;
     } catch (const std::exception& e) {
         fprintf(stderr, "host_render: %s\n", e.what());
+        return -1;
+    }
+}
+
+// Closest hits of n rays traced whole and traced in pieces of `budget`
+// traversal iterations, the stack moved to a fresh stack between pieces (as
+// the capped extend stage and its resume stage do, with 16 entries in "LDS"
+// and the rest in a backing store). Writes the number of rays whose hit
+// differs in any bit and the number of pieces traced.
+extern "C" int host_closest_split(const pt_scene_view* v, const pt_options* o, const pt_camera* cam, int n,
+                                  const float* orig, const float* dir, int budget, int* differ, long long* pieces) {
+    try {
+        HostScene H(v, o, cam);
+        const DevScene& sc = H.sc;
+        const DevOpts op = H.op;
+        int bad = 0;
+        long long np = 0;
+#pragma omp parallel reduction(+ : bad, np)
+        {
+            const int S = 16;
+            std::vector<int> l0(S), s0(MAXD + 1), l1(S), s1(MAXD + 1), l2(MAXD + 1), s2(MAXD + 1);
+            Stack a{l0.data(), s0.data(), 0, 1, S, 0, 1}, b{l1.data(), s1.data(), 0, 1, S, 0, 1};
+            Stack whole{l2.data(), s2.data(), 0, 1, MAXD, 0, 1};
+            TravCount cnt{0, 0, 0, 0, 0, 0};
+#pragma omp for schedule(dynamic, 256)
+            for (int i = 0; i < n; ++i) {
+                const vec3 ro = V(orig[3 * i], orig[3 * i + 1], orig[3 * i + 2]);
+                const vec3 rd = V(dir[3 * i], dir[3 * i + 1], dir[3 * i + 2]);
+                bool ovf0 = false, ovf1 = false;
+                const Hit ref = closest<false>(sc, op, ro, rd, 1, whole, cnt, ovf0);
+                Hit h;
+                TravState ts;
+                Stack* cur = &a;
+                Stack* nxt = &b;
+                bool done = closest_begin(sc, op, ro, rd, 1, *cur, ovf1, h, ts, budget);
+                np++;
+                while (!done) {
+                    for (int k = 0; k < ts.sp; ++k) nxt->push(k, cur->pop(k));
+                    std::swap(cur, nxt);
+                    done = traverse<false, false, true>(sc, ro, rd, h.t, *cur, h, cnt, ovf1, false, nullptr, &ts,
+                                                        budget);
+                    np++;
+                }
+                const bool same = memcmp(&ref.t, &h.t, 4) == 0 && memcmp(&ref.u, &h.u, 4) == 0 &&
+                                  memcmp(&ref.v, &h.v, 4) == 0 && ref.tri == h.tri && ref.inst == h.inst &&
+                                  ref.light == h.light && ovf0 == ovf1;
+                if (!same) bad++;
+            }
+        }
+        *differ = bad;
+        *pieces = np;
+        return 0;
+    } catch (const std::exception& e) {
+        fprintf(stderr, "host_closest_split: %s\n", e.what());
         return -1;
     }
 }

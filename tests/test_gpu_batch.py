@@ -34,13 +34,16 @@ def small_jobs():
                              depth_range=(1, 7), spp_range=(1, 6))
 
 
-@pytest.mark.parametrize("graph,wave_spp,bs,stack,cadence", [(0, 1, 256, 24, 1), (1, 1, 64, 16, 2),
-                                                              (1, 3, 128, 32, 1), (1, 64, 256, 2, 3)])
-def test_batch_equals_single_renders(renderer, graph, wave_spp, bs, stack, cadence):
+@pytest.mark.parametrize("graph,wave_spp,bs,stack,cadence,cap", [(0, 1, 256, 24, 1, 0), (1, 1, 64, 16, 2, 0),
+                                                                  (1, 3, 128, 32, 1, 0), (1, 64, 256, 2, 3, 0),
+                                                                  (1, 3, 64, 16, 1, 2), (0, 4, 256, 24, 2, 6)])
+def test_batch_equals_single_renders(renderer, graph, wave_spp, bs, stack, cadence, cap):
+    """cap: pt_options.extend_cap of the batch (its resume launches captured
+    in the pass graph); the single renders run uncapped."""
     sc, rs, jobs = small_jobs()
     renderer.upload_scene(sc)
     kw = dict(block_size=bs, stack_depth=stack, compact_every=cadence)
-    renderer.set_options(options_for(rs, wave_spp=wave_spp, use_graph=graph, **kw))
+    renderer.set_options(options_for(rs, wave_spp=wave_spp, use_graph=graph, extend_cap=cap, **kw))
     outs = renderer.batch_render(jobs)
     st = renderer.stats()
     print(f"graph={graph} wave_spp={wave_spp}: {st['ms_last_render']:.3f} ms")

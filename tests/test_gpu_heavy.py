@@ -108,3 +108,19 @@ def test_heavy_full_size_sample3_finite(renderer):
     print(f"pixel (3219, 1547): {a[1547, 3219].tolist()}")
     assert np.isfinite(a).all() and (a[..., :3] >= 0).all()
     assert np.array_equal(a, b)
+
+
+def test_heavy_small_capped_extend_identical(renderer):
+    """The capped extend stage on the heavy scene (alpha pass-through, media,
+    four-wide nodes): bit-identical to the plain kernel."""
+    sc, rs = small_heavy()
+    renderer.upload_scene(sc)
+    out = []
+    for mode, kw in ((PT_MODE_PLAIN, {}), (PT_MODE_WAVEFRONT, dict(extend_cap=8, wave_spp=4, bvh_width=4))):
+        renderer.set_options(options_for(rs, mode=mode, **kw))
+        renderer.set_camera(sc.camera())
+        renderer.clear()
+        renderer.render(0, 4)
+        renderer.synchronize()
+        out.append(renderer.read_accum())
+    assert np.array_equal(out[0], out[1])

@@ -161,3 +161,29 @@ def test_ldr_matches_oracle_tonemap(renderer, tm, aces, simple, bg):
     print(f"tm={tm} aces={aces} simple={simple} bg={bg}: max diff {d.max()}, off-by-one {(d > 0).mean():.2e}, "
           f"alpha<1 px {(acc[..., 3] < 3).mean():.3f}")
     assert d.max() <= 1 and (d > 0).mean() < 1e-3
+
+
+@pytest.mark.parametrize("cap", [1, 3, 32])
+@pytest.mark.parametrize("width", [2, 4])
+def test_capped_extend_identical(renderer, cap, width):
+    """pt_options.extend_cap: later bounces' traversals stopped after `cap`
+    steps and finished by the densely packed resume launch (cap 1 and 3 stop
+    almost every ray and overflow the continuation buffer, so some finish in
+    the capped launch) give the image of the uncapped wavefront form and of
+    the plain kernel, bit for bit, and the oracle's on every pixel."""
+    sc, rs = small_instanced()
+    out = []
+    for mode, kw in ((PT_MODE_PLAIN, {}), (PT_MODE_WAVEFRONT, {}), (PT_MODE_WAVEFRONT, dict(extend_cap=cap))):
+        renderer.upload_scene(sc)
+        renderer.set_options(options_for(rs, mode=mode, wave_spp=4, bvh_width=width, **kw))
+        renderer.set_camera(sc.camera())
+        renderer.clear()
+        renderer.render(0, 4)
+        renderer.synchronize()
+        out.append(renderer.read_accum())
+    assert np.array_equal(out[0], out[1]) and np.array_equal(out[1], out[2])
+    sa, cam, o = oracle_setup(sc, rs)
+    r = oracle.render(sa, cam, o, 0, 4)
+    e = rel_err(out[2][..., :3], r[..., :3], floor=1e-4).max(axis=2)
+    print(f"cap {cap} bvh{width}: pixels within 1e-4 = {(e <= 1e-4).mean():.6f}")
+    assert (e <= 1e-4).all()

@@ -11,7 +11,7 @@ import pytest
 
 import hostpath
 import oracle
-from common import oracle_setup, small_instanced
+from common import camera_rays, oracle_setup, random_dirs, small_instanced
 from glslpt import scenes
 from glslpt._capi import ADJOINT_FIELDS, ADJOINT_MEDIUM_FIELDS, ADJOINT_MEDIUM_NAMES, ADJOINT_NAMES
 
@@ -123,3 +123,26 @@ def test_host_replay_radiance_equals_primal(kind):
     print(f"{kind}: dropped {dropped}, mismatched paths {mismatch}, |g| {np.abs(g).max():.3g}")
     assert mismatch == 0 and dropped == 0
     assert np.isfinite(g).all()
+
+
+@pytest.mark.parametrize("width", [2, 4])
+@pytest.mark.parametrize("kind", ["cornell", "instanced", "heavy"])
+def test_split_traversal_equals_whole(kind, width):
+    """The capped extend stage's resumable traversal: stopped every `budget`
+    iterations, its stack moved to another stack (16 entries in LDS, the rest
+    in the backing store), and resumed, it gives every ray's hit bit for bit,
+    for budgets from 1 to 32, on camera rays and rays from inside the scene."""
+    sc, rs = _scene(kind)
+    rng = np.random.default_rng(61)
+    ro, rd = camera_rays(sc.camera(), rs.width, rs.height, rng, 4000)
+    lo, hi = {"cornell": ((0.05,) * 3, (0.5,) * 3), "instanced": ((-6,) * 3, (6,) * 3),
+              "heavy": ((-20, 0, -20), (20, 8, 20))}[kind]
+    ri = rng.uniform(lo, hi, (4000, 3)).astype(np.float32)
+    ro = np.concatenate([ro, ri])
+    rd = np.concatenate([rd, random_dirs(rng, 4000)])
+    for budget in (1, 2, 3, 7, 16, 32):
+        differ, pieces = hostpath.closest_split(sc, rs, ro, rd, budget, bvh_width=width)
+        print(f"{kind} bvh{width} budget {budget}: {len(ro)} rays in {pieces} pieces, differing {differ}")
+        assert differ == 0
+        if budget == 1:
+            assert pieces > 5 * len(ro)
