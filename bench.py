@@ -531,6 +531,8 @@ def main():
     ap.add_argument("--stack-depth", type=int, default=None)
     ap.add_argument("--compact-every", type=int, default=None)
     ap.add_argument("--bvh-width", type=int, default=None, choices=(2, 4))
+    ap.add_argument("--nee-inline", type=int, default=None, choices=(0, 1))
+    ap.add_argument("--extend-cap", type=int, default=None)
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-count", action="store_true")
@@ -560,7 +562,7 @@ def main():
     cfg = dict(block_size=256, stack_depth=24, wave_spp=4, compact_every=1, bvh_width=2)
     cfg.update(tuned("c2") or {})
     for k in cfg:
-        if getattr(args, k) is not None:
+        if getattr(args, k, None) is not None:
             cfg[k] = getattr(args, k)
     opts_kw = dict(cfg)
     samples_per_step = rs.width * rs.height * args.spp

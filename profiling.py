@@ -353,7 +353,7 @@ def summarize(root, out_json):
     seen = set()
     for sub in ("fetch", "write", "tcc", "sq"):
         for name, ds in _one(root, sub).items():
-            kind = "camera" if name.endswith("<false, true, false>") else "later"
+            kind = "camera" if "k_extend<false, true, false" in name else "later"
             k = kinds[kind]
             k["kernels"].add(name)
             for did, cv in ds.items():
