@@ -927,9 +927,14 @@ __device__ bool traverse(const DevScene& sc, vec3 O, vec3 D, float tmax, Stack& 
                 float det = dot(e0, pv);
                 vec3 tv = o - v0;
                 vec3 qv = cross(tv, e0);
-                float uu = dot(tv, pv) / det;
-                float vv = dot(d, qv) / det;
-                float tt = dot(e1, qv) / det;
+                // uvt.xyz / det (closest_hit.glsl:140) as one correctly rounded
+                // reciprocal and three products: within GLSL's 2.5 ulp for a
+                // division, the oracle's arithmetic, two division sequences
+                // fewer per triangle
+                const float rdet = 1.0f / det;
+                float uu = dot(tv, pv) * rdet;
+                float vv = dot(d, qv) * rdet;
+                float tt = dot(e1, qv) * rdet;
                 float ww = 1.0f - uu - vv;
                 if (uu >= 0.0f && vv >= 0.0f && tt >= 0.0f && ww >= 0.0f && tt < best) {
                     if (ANY && !(alpha && alpha_transparent(sc, first + i, cur_mat, uu, vv, ww, rng))) return true;

@@ -9,7 +9,9 @@
 // min(x,y) = y < x ? y : x, max(x,y) = x < y ? y : x (GLSL spec definitions);
 // mat * v = sum of columns scaled by components, in column order; inverse() by
 // cofactor expansion; texture() = GL_LINEAR bilinear with GL_REPEAT per the GL
-// spec formula, full-precision weights; fragment centre = pixel + 0.5.
+// spec formula, full-precision weights; fragment centre = pixel + 0.5; the
+// triangle test's uvt.xyz / det = uvt.xyz * (1 / det), the reciprocal correctly
+// rounded (GLSL allows 2.5 ulp for a division; this is within 1.5).
 //
 // What pins each part (tests/test_oracle_*.py):
 //   pcg4d / rand / seeds          numpy uint32 re-implementation, known answers
@@ -898,7 +900,8 @@ template <class T> bool ClosestHit(Ctx<T>& c, const Ray<T>& r, State<T>& state, 
                 uvt.x = dot(tv, pv);
                 uvt.y = dot(rTrans.direction, qv);
                 uvt.z = dot(e1, qv);
-                uvt.x = uvt.x / det; uvt.y = uvt.y / det; uvt.z = uvt.z / det;
+                const T rdet = T(1) / det;  // uvt.xyz / det as one reciprocal and three products
+                uvt.x = uvt.x * rdet; uvt.y = uvt.y * rdet; uvt.z = uvt.z * rdet;
                 uvt.w = T(1) - uvt.x - uvt.y;
                 if (uvt.x >= T(0) && uvt.y >= T(0) && uvt.z >= T(0) && uvt.w >= T(0) && uvt.z < t) {
                     t = uvt.z;
@@ -1046,7 +1049,8 @@ template <class T> bool AnyHit(Ctx<T>& c, const Ray<T>& r, T maxDist) {
                 uvt.x = dot(tv, pv);
                 uvt.y = dot(rTrans.direction, qv);
                 uvt.z = dot(e1, qv);
-                uvt.x = uvt.x / det; uvt.y = uvt.y / det; uvt.z = uvt.z / det;
+                const T rdet = T(1) / det;  // uvt.xyz / det as one reciprocal and three products
+                uvt.x = uvt.x * rdet; uvt.y = uvt.y * rdet; uvt.z = uvt.z * rdet;
                 uvt.w = T(1) - uvt.x - uvt.y;
                 if (uvt.x >= T(0) && uvt.y >= T(0) && uvt.z >= T(0) && uvt.w >= T(0) && uvt.z < maxDist) {
                     if (alpha) {
