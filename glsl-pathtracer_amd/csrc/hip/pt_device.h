@@ -814,11 +814,17 @@ struct TravState { int ref, sp, blas_base, cur_inst; };
 // and false when it stopped, its position saved in *ts. The iterations and
 // their arithmetic are the same however the traversal is split: the object-
 // space ray of the current instance is recomputed from the same record.
-template <bool ANY, bool COUNT, bool RESUME = false>
+// UNORDERED (any hit without random draws): the children a node's boxes hit
+// are visited in child order instead of nearest first. Whether anything is
+// hit does not depend on the order; what would is a stochastic alpha test,
+// whose draws follow the visit order, so callers that pass an RNG keep the
+// reference's order.
+template <bool ANY, bool COUNT, bool RESUME = false, bool UNORDERED = false>
 __device__ bool traverse(const DevScene& sc, vec3 O, vec3 D, float tmax, Stack& stk, Hit& h, TravCount& cnt,
                          bool& overflow, bool alpha = false, Rng* rng = nullptr, TravState* ts = nullptr,
                          int budget = -1) {
     static_assert(!(ANY && RESUME), "only closest-hit traversals are resumed");
+    static_assert(!UNORDERED || ANY, "only any-hit traversals ignore the visit order");
     if (COUNT) cnt.rays++;
     const vec3 invD = V(1.0f / D.x, 1.0f / D.y, 1.0f / D.z);
     vec3 o = O, d = D, inv = invD;
@@ -858,6 +864,15 @@ __device__ bool traverse(const DevScene& sc, vec3 O, vec3 D, float tmax, Stack& 
             const int c = int(ha) + int(hb) + int(hc) + int(hd);
             if (c == 0) {
                 do_pop = true;
+            } else if (UNORDERED) {
+                // the first hit child next, the others pushed
+                if (sp + c - 1 > MAXD) { overflow = true; break; }
+                int first = EMPTY4;
+                if (ha) first = rf.x;
+                if (hb) { if (first == EMPTY4) first = rf.y; else stk.push(sp++, rf.y); }
+                if (hc) { if (first == EMPTY4) first = rf.z; else stk.push(sp++, rf.z); }
+                if (hd) { if (first == EMPTY4) first = rf.w; else stk.push(sp++, rf.w); }
+                ref = first;
             } else {
                 const float big = __builtin_inff();
                 float q0 = ha ? ka : big, q1 = hb ? kb : big, q2 = hc ? kc : big, q3 = hd ? kd : big;
@@ -893,8 +908,8 @@ __device__ bool traverse(const DevScene& sc, vec3 O, vec3 D, float tmax, Stack& 
             bool hl = kl > 0.0f && !(tl0 > best);
             bool hr = kr > 0.0f && !(tr0 > best);
             if (hl && hr) {
-                int nearer = (kl > kr) ? n.d.y : n.d.x;
-                int farther = (kl > kr) ? n.d.x : n.d.y;
+                int nearer = (!UNORDERED && kl > kr) ? n.d.y : n.d.x;
+                int farther = (!UNORDERED && kl > kr) ? n.d.x : n.d.y;
                 if (sp >= MAXD) { overflow = true; break; }
                 if (COUNT && sp >= stk.S) cnt.spills++;
                 stk.push(sp++, farther);

@@ -495,7 +495,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))
             for (int k = 0; k < 2; ++k) {
                 if (!((id.y >> k) & 1)) continue;
                 const float4 d4 = sb.d[2 * i + k];
-                if (occluded<COUNT>(sc, op, xyz(o4), xyz(d4), d4.w, stk, cnt, ovf)) continue;
+                if (occluded<COUNT, true>(sc, op, xyz(o4), xyz(d4), d4.w, stk, cnt, ovf)) continue;
                 Ld = Ld + xyz(sb.c[2 * i + k]);
                 lit |= 1 << k;
             }
@@ -1203,7 +1203,7 @@ __global__ void __launch_bounds__(256) k_query_any(DevScene sc, DevOpts op, cons
     bool ovf = false;
     vec3 o = V(orig[3 * i], orig[3 * i + 1], orig[3 * i + 2]);
     vec3 d = V(dir[3 * i], dir[3 * i + 1], dir[3 * i + 2]);
-    occ[i] = occluded<false>(sc, op, o, d, maxd[i], stk, cnt, ovf) ? 1 : 0;
+    occ[i] = occluded<false, true>(sc, op, o, d, maxd[i], stk, cnt, ovf) ? 1 : 0;
     if (ovf) atomicOr(ovf_flag, 1);
 }
 

@@ -42,12 +42,15 @@ __device__ void camera_ray(const DevOpts& op, int px, int py, Rng& rng, vec3& ro
 
 // AnyHit (anyhit.glsl:25-215): lights, then the BVH; with OPT_ALPHA_TEST and
 // no media, alpha-tested candidates are skipped (BLEND draws from `rng`).
-template <bool COUNT>
+// UNORDERED: the caller passes no RNG (no stochastic alpha in this query),
+// so children are visited in child order (see traverse).
+template <bool COUNT, bool UNORDERED = false>
 __device__ __forceinline__ bool occluded(const DevScene& sc, const DevOpts& op, vec3 ro, vec3 rd, float maxDist,
                                          Stack& stk, TravCount& cnt, bool& ovf, Rng* rng = nullptr) {
     if ((op.flags & OPT_LIGHTS) && any_light<COUNT>(sc, ro, rd, maxDist, cnt)) return true;
     Hit h;
     const bool alpha = (op.flags & OPT_ALPHA_TEST) && !(op.flags & OPT_MEDIUM);
+    if (UNORDERED) return traverse<true, COUNT, false, true>(sc, ro, rd, maxDist, stk, h, cnt, ovf, alpha, nullptr);
     return traverse<true, COUNT>(sc, ro, rd, maxDist, stk, h, cnt, ovf, alpha, rng);
 }
 

@@ -93,3 +93,21 @@ def closest_split(sc, rs, origins, dirs, budget, **opts):
     if rc != 0:
         raise RuntimeError("host_closest_split failed")
     return differ.value, pieces.value
+
+
+def any_orders(sc, rs, origins, dirs, maxd, **opts):
+    """(rays whose any-hit answer differs between nearest-first and child-order
+    visits, rays occluded) on the CPU."""
+    v, _ = _view(sc)
+    o = options_for(rs, **opts)
+    cam = sc.camera()
+    ro = np.ascontiguousarray(origins, np.float32)
+    rd = np.ascontiguousarray(dirs, np.float32)
+    md = np.ascontiguousarray(maxd, np.float32)
+    differ, hits = C.c_int(0), C.c_int(0)
+    rc = lib().host_any_orders(C.byref(v), C.byref(o), C.byref(cam), int(ro.shape[0]), ro.ctypes.data_as(P(C.c_float)),
+                               rd.ctypes.data_as(P(C.c_float)), md.ctypes.data_as(P(C.c_float)), C.byref(differ),
+                               C.byref(hits))
+    if rc != 0:
+        raise RuntimeError("host_any_orders failed")
+    return differ.value, hits.value

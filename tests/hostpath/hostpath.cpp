@@ -248,3 +248,37 @@ extern "C" int host_closest_split(const pt_scene_view* v, const pt_options* o, c
         return -1;
     }
 }
+
+// Any-hit answers of n rays (segments of length maxd) with children visited
+// nearest first and in child order: the number of rays whose answers differ.
+extern "C" int host_any_orders(const pt_scene_view* v, const pt_options* o, const pt_camera* cam, int n,
+                               const float* orig, const float* dir, const float* maxd, int* differ, int* hits) {
+    try {
+        HostScene H(v, o, cam);
+        const DevScene& sc = H.sc;
+        const DevOpts op = H.op;
+        int bad = 0, nh = 0;
+#pragma omp parallel reduction(+ : bad, nh)
+        {
+            std::vector<int> l0(16), s0(MAXD + 1);
+            Stack stk{l0.data(), s0.data(), 0, 1, 16, 0, 1};
+            TravCount cnt{0, 0, 0, 0, 0, 0};
+#pragma omp for schedule(dynamic, 256)
+            for (int i = 0; i < n; ++i) {
+                const vec3 ro = V(orig[3 * i], orig[3 * i + 1], orig[3 * i + 2]);
+                const vec3 rd = V(dir[3 * i], dir[3 * i + 1], dir[3 * i + 2]);
+                bool ovf = false;
+                const bool a = occluded<false>(sc, op, ro, rd, maxd[i], stk, cnt, ovf);
+                const bool b = occluded<false, true>(sc, op, ro, rd, maxd[i], stk, cnt, ovf);
+                if (a != b) bad++;
+                if (a) nh++;
+            }
+        }
+        *differ = bad;
+        *hits = nh;
+        return 0;
+    } catch (const std::exception& e) {
+        fprintf(stderr, "host_any_orders: %s\n", e.what());
+        return -1;
+    }
+}

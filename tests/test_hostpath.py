@@ -146,3 +146,21 @@ def test_split_traversal_equals_whole(kind, width):
         assert differ == 0
         if budget == 1:
             assert pieces > 5 * len(ro)
+
+
+@pytest.mark.parametrize("width", [2, 4])
+@pytest.mark.parametrize("kind", ["instanced", "heavy", "alpha_mask"])
+def test_any_hit_unordered_equals_ordered(kind, width):
+    """The shadow stage's any-hit traversal visits a node's hit children in
+    child order (no random draws in it); its answers equal the reference's
+    nearest-first order on every ray, with and without alpha masks."""
+    sc, rs = _scene(kind)
+    rng = np.random.default_rng(62)
+    lo, hi = {"instanced": ((-6,) * 3, (6,) * 3), "heavy": ((-20, 0, -20), (20, 8, 20)),
+              "alpha_mask": ((-3, 0, -3), (3, 3, 3))}[kind]
+    ro = rng.uniform(lo, hi, (20000, 3)).astype(np.float32)
+    rd = random_dirs(rng, 20000)
+    md = rng.uniform(0.1, 30, 20000).astype(np.float32)
+    differ, hits = hostpath.any_orders(sc, rs, ro, rd, md, bvh_width=width)
+    print(f"{kind} bvh{width}: {len(ro)} rays, occluded {hits}, differing {differ}")
+    assert differ == 0 and 0 < hits < len(ro)
