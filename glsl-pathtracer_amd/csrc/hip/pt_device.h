@@ -32,11 +32,12 @@ struct alignas(16) GTri {
     float4 e1;
 };
 
-// Four-wide BLAS node (the reference's BVH2 collapsed: each node holds up to
-// four of its descendants): the children's boxes as six rows of four (min x,
-// max x, min y, max y, min z, max z of children 0..3) and their references
-// (internal: index of a GNode4; leaf: -1 - (row << 2 | count) as in GNode;
-// EMPTY4: no child). Two 64-byte lines.
+// Four-wide node (the reference's BVH2 collapsed: each node holds up to four
+// of its descendants), for every BLAS and the TLAS: the children's boxes as
+// six rows of four (min x, max x, min y, max y, min z, max z of children
+// 0..3) and their references (internal: index of a GNode4; leaf: as in GNode,
+// -1 - (row << 2 | count) in a BLAS and -1 - instance in the TLAS; EMPTY4: no
+// child). Two 64-byte lines.
 struct alignas(16) GNode4 {
     float4 lox, hix, loy, hiy, loz, hiz;
     int4 ref;
@@ -70,7 +71,7 @@ struct alignas(16) GLight {
 
 struct DevScene {
     const GNode* __restrict__ nodes;
-    const GNode4* __restrict__ nodes4;  // four-wide BLAS nodes; null: BLAS traversed in BVH2 form
+    const GNode4* __restrict__ nodes4;  // four-wide TLAS and BLAS nodes; null: BVH2 form
     const GTri* __restrict__ tris;
     const int4* __restrict__ vidx;      // vertIndices rows (x, y, z, unused)
     const float4* __restrict__ verts;   // verticesUVX
@@ -81,7 +82,9 @@ struct DevScene {
     const uint8_t* __restrict__ tex;    // RGBA8 layers
     const float* __restrict__ env_img;  // RGB32F
     const float* __restrict__ env_cdf;
-    int root;                           // TLAS root reference
+    int root;                           // TLAS root reference (in nodes4 when that is set)
+    int root_bin;                       // EMPTY4, or the BVH2 TLAS root that any-hit traversals drawing
+                                        // random numbers walk (nodes4 set and a material blends)
     int num_lights;
     int tex_w, tex_h, num_tex;
     int env_w, env_h;
@@ -844,10 +847,10 @@ __device__ bool traverse(const DevScene& sc, vec3 O, vec3 D, float tmax, Stack& 
             return false;
         }
         bool do_pop = false;
-        if (ref >= 0 && blas_base >= 0 && sc.nodes4) {
-            // Four-wide BLAS node: the hit children in order of the reference's
-            // key (nearest first; equal keys in child order), the nearest
-            // visited next, the others pushed farthest first.
+        if (ref >= 0 && sc.nodes4) {
+            // Four-wide node (TLAS or BLAS): the hit children in order of the
+            // reference's key (nearest first; equal keys in child order), the
+            // nearest visited next, the others pushed farthest first.
             if (COUNT) cnt.nodes += 2;
             const GNode4& n = sc.nodes4[ref];
             const float4 lx = n.lox, hx = n.hix, ly = n.loy, hy = n.hiy, lz = n.loz, hz = n.hiz;

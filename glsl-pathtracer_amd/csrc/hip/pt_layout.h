@@ -42,13 +42,10 @@ inline GInst make_inst(const float* m, int blas_ref, int material, int blas4_ref
     return g;
 }
 
-// Four-wide BLAS nodes from the BVH2 re-layout. Each four-wide node of a BVH2
-// node gathers its descendants: starting from the two children, while fewer
-// than four are held, the internal one of largest surface area (the first on
-// ties) is replaced in place by its two children, so the children stay in the
-// BVH2's left-to-right order. Leaves keep their references; a node's internal
-// children get consecutive indices when the node is converted (depth first).
-// root4 maps each BLAS root's BVH2 reference to its four-wide one.
+// Four-wide BLAS nodes from the BVH2 re-layout, each from its BVH2 node by
+// ptd::expand4. A node's internal children get consecutive indices when the
+// node is converted (depth first). root4 maps each BLAS root's BVH2 reference
+// to its four-wide one.
 struct Wide4 {
     std::vector<ptd::GNode4> nodes;
     std::unordered_map<int, int> root4;
@@ -59,16 +56,6 @@ inline void collapse4_root(const std::vector<GNode>& n2, int root, Wide4& W) {
         W.root4[root] = root;
         return;
     }
-    struct Ch { float lo[3], hi[3]; int ref; };
-    auto kids = [&](int r, Ch& l, Ch& rr) {
-        const GNode& g = n2[r];
-        l = Ch{{g.a.x, g.a.y, g.a.z}, {g.a.w, g.b.x, g.b.y}, g.d.x};
-        rr = Ch{{g.b.z, g.b.w, g.c.x}, {g.c.y, g.c.z, g.c.w}, g.d.y};
-    };
-    auto area = [](const Ch& c) {
-        const double dx = double(c.hi[0]) - c.lo[0], dy = double(c.hi[1]) - c.lo[1], dz = double(c.hi[2]) - c.lo[2];
-        return dx * dy + dy * dz + dz * dx;
-    };
     struct Job { int ref2, slot; };  // BVH2 node to convert, its GNode4 index
     std::vector<Job> todo;
     auto alloc = [&](int ref2) {
@@ -81,41 +68,45 @@ inline void collapse4_root(const std::vector<GNode>& n2, int root, Wide4& W) {
     while (!todo.empty()) {
         const Job jb = todo.back();
         todo.pop_back();
-        std::vector<Ch> ch(2);
-        kids(jb.ref2, ch[0], ch[1]);
-        while (ch.size() < 4) {
-            int best = -1;
-            double ba = -1.0;
-            for (int k = 0; k < static_cast<int>(ch.size()); ++k)
-                if (ch[k].ref >= 0 && area(ch[k]) > ba) { ba = area(ch[k]); best = k; }
-            if (best < 0) break;
-            Ch l, r;
-            kids(ch[best].ref, l, r);
-            ch[best] = l;
-            ch.insert(ch.begin() + best + 1, r);
-        }
-        float lo[3][4], hi[3][4];
+        ptd::Child4 ch[4];
+        const int k = ptd::expand4(n2.data(), 0, jb.ref2, ch);
         int ref[4];
-        for (int k = 0; k < 4; ++k) {
-            for (int a = 0; a < 3; ++a) {
-                lo[a][k] = k < static_cast<int>(ch.size()) ? ch[k].lo[a] : 0.f;
-                hi[a][k] = k < static_cast<int>(ch.size()) ? ch[k].hi[a] : 0.f;
-            }
-            ref[k] = k < static_cast<int>(ch.size()) ? ch[k].ref : ptd::EMPTY4;
-        }
+        for (int j = 0; j < k; ++j) ref[j] = ch[j].ref;
         // the first child is converted next (depth first)
-        for (int k = static_cast<int>(ch.size()) - 1; k >= 0; --k)
-            if (ref[k] >= 0) ref[k] = alloc(ref[k]);
-        ptd::GNode4& g = W.nodes[jb.slot];
-        g.lox = make_float4(lo[0][0], lo[0][1], lo[0][2], lo[0][3]);
-        g.hix = make_float4(hi[0][0], hi[0][1], hi[0][2], hi[0][3]);
-        g.loy = make_float4(lo[1][0], lo[1][1], lo[1][2], lo[1][3]);
-        g.hiy = make_float4(hi[1][0], hi[1][1], hi[1][2], hi[1][3]);
-        g.loz = make_float4(lo[2][0], lo[2][1], lo[2][2], lo[2][3]);
-        g.hiz = make_float4(hi[2][0], hi[2][1], hi[2][2], hi[2][3]);
-        g.ref = make_int4(ref[0], ref[1], ref[2], ref[3]);
-        g.pad = make_int4(0, 0, 0, 0);
+        for (int j = k - 1; j >= 0; --j)
+            if (ref[j] >= 0) ref[j] = alloc(ref[j]);
+        W.nodes[jb.slot] = ptd::make_node4(ch, k, ref);
     }
+}
+
+// Four-wide TLAS from the BVH2 TLAS (n2[r - off] holds node r, root2 the
+// root), appended to out in breadth-first order (the root first, then each
+// level's internal children in order), references counted from base (the
+// index of the first appended entry), then zero entries up to `room` in all.
+// The device collapse after a TLAS rebuild (k_tlas4) writes the same array.
+// Returns the four-wide root reference (root2 itself when it is a leaf).
+inline int collapse4_tlas(const GNode* n2, int off, int root2, int base, size_t room, std::vector<ptd::GNode4>& out) {
+    const size_t first = out.size();
+    int root4 = root2;
+    if (root2 >= 0) {
+        root4 = base;
+        std::vector<int> fifo{root2};
+        for (size_t q = 0; q < fifo.size(); ++q) {
+            ptd::Child4 ch[4];
+            const int k = ptd::expand4(n2, off, fifo[q], ch);
+            int ref[4];
+            for (int j = 0; j < k; ++j) {
+                ref[j] = ch[j].ref;
+                if (ref[j] >= 0) {
+                    ref[j] = base + static_cast<int>(fifo.size());
+                    fifo.push_back(ch[j].ref);
+                }
+            }
+            out.push_back(ptd::make_node4(ch, k, ref));
+        }
+    }
+    if (out.size() - first < room) out.resize(first + room, ptd::GNode4{});
+    return root4;
 }
 
 // Re-layout of the reference's flat node array (bvh_translator.cpp:32-137)
@@ -127,8 +118,10 @@ struct Relayout {
     std::vector<int> inst_blas_ref, inst_material;
     std::vector<float> inst_local;  // box of each instance's BLAS root: min xyz, max xyz
     std::unordered_map<int, int> root_ref;  // flat index of each BLAS root a TLAS leaf names -> its reference
-    Wide4 wide;                             // four-wide BLAS nodes and their roots
+    Wide4 wide;                             // four-wide BLAS nodes and their roots, then the four-wide TLAS
     std::vector<int> inst_blas4_ref;        // each instance's four-wide BLAS root reference
+    int tlas4_base = 0;                     // first four-wide TLAS entry in wide.nodes (at least max(n - 1, 1) follow)
+    int root4 = 0;                          // four-wide TLAS root reference
     // The TLAS as the device rebuild writes it: one instance per leaf, every
     // instance once, internal nodes numbered in preorder from tlas_base to the
     // end of the node array.
@@ -248,7 +241,11 @@ inline Relayout relayout(const pt_bvh_node* flat, int num_nodes, int top, int nu
         collapse4_root(R.nodes, R.inst_blas_ref[i], R.wide);
         R.inst_blas4_ref[i] = R.wide.root4[R.inst_blas_ref[i]];
     }
-    if (R.wide.nodes.empty()) R.wide.nodes.emplace_back();  // never read: keeps the array non-empty
+    // The four-wide TLAS after the BLASes, room for n - 1 nodes (what any
+    // TLAS over n instances collapses to at most; a device rebuild may change
+    // the shape).
+    R.tlas4_base = static_cast<int>(R.wide.nodes.size());
+    R.root4 = collapse4_tlas(R.nodes.data(), 0, R.root, R.tlas4_base, std::max(num_instances - 1, 1), R.wide.nodes);
     return R;
 }
 

@@ -51,7 +51,16 @@ __device__ __forceinline__ bool occluded(const DevScene& sc, const DevOpts& op, 
     Hit h;
     const bool alpha = (op.flags & OPT_ALPHA_TEST) && !(op.flags & OPT_MEDIUM);
     if (UNORDERED) return traverse<true, COUNT, false, true>(sc, ro, rd, maxDist, stk, h, cnt, ovf, alpha, nullptr);
-    return traverse<true, COUNT>(sc, ro, rd, maxDist, stk, h, cnt, ovf, alpha, rng);
+    // A stochastic alpha test draws in visit order, and four-wide nodes visit
+    // children nearest first among up to four, not in the binary tree's order
+    // (anyhit.glsl:118-141 over the reference's BVH2): with blending materials
+    // this traversal walks the binary form.
+    DevScene s = sc;
+    if (rng && alpha && sc.root_bin != EMPTY4) {
+        s.nodes4 = nullptr;
+        s.root = sc.root_bin;
+    }
+    return traverse<true, COUNT>(s, ro, rd, maxDist, stk, h, cnt, ovf, alpha, rng);
 }
 
 template <bool COUNT>

@@ -85,8 +85,10 @@ struct pt_renderer_s {
 
     // scene (GPU layout)
     DevBuf<GNode> nodes;
-    DevBuf<ptd::GNode4> nodes4;            // four-wide BLAS nodes
+    DevBuf<ptd::GNode4> nodes4;            // four-wide BLAS nodes, then the four-wide TLAS
     std::unordered_map<int, int> root4;    // BLAS root: BVH2 reference -> four-wide reference
+    int root2 = 0, root4_tlas = 0;         // TLAS root in the BVH2 and in the four-wide form
+    int t4_base = 0, t4_room = 0;          // four-wide TLAS entries [t4_base, t4_base + t4_room) of nodes4
     DevBuf<GTri> tris;
     DevBuf<int4> vidx;
     DevBuf<float4> verts, norms, mats;
@@ -240,7 +242,17 @@ struct pt_renderer_s {
     }
 
     // The BLAS form the kernels traverse follows pt_options.bvh_width.
-    void select_bvh() { sc.nodes4 = (opts.bvh_width == 4 && nodes4.p) ? nodes4.p : nullptr; }
+    void select_bvh() {
+        sc.nodes4 = (opts.bvh_width == 4 && nodes4.p) ? nodes4.p : nullptr;
+        sc.root = sc.nodes4 ? root4_tlas : root2;
+        sc.root_bin = (sc.nodes4 && any_blend) ? root2 : ptd::EMPTY4;
+    }
+    void set_tlas_roots(const Relayout& R) {
+        root2 = R.root;
+        root4_tlas = R.root4;
+        t4_base = R.tlas4_base;
+        t4_room = static_cast<int>(R.wide.nodes.size()) - R.tlas4_base;
+    }
     // Four-wide BLAS roots of BVH2 BLAS root references.
     std::vector<int> blas4_of(const std::vector<int>& blas2) const {
         std::vector<int> out(blas2.size());
@@ -995,7 +1007,7 @@ int pt_upload_scene(pt_handle h, const pt_scene_view* v) {
         sc.tex = (v->num_textures > 0 && v->texture_maps) ? h->tex.p : nullptr;
         sc.env_img = h->has_env ? h->env_img.p : nullptr;
         sc.env_cdf = h->has_env ? h->env_cdf.p : nullptr;
-        sc.root = R.root;
+        h->set_tlas_roots(R);
         h->select_bvh();
         sc.num_lights = v->num_lights;
         sc.tex_w = v->tex_width; sc.tex_h = v->tex_height;
@@ -1058,9 +1070,16 @@ int pt_update_instances(pt_handle h, const float* transforms, int32_t num_instan
                     local[6 * i + j] = h->flat[T.roots[i]].bboxmin[j];
                     local[6 * i + 3 + j] = h->flat[T.roots[i]].bboxmax[j];
                 }
-            if (num_instances > 1)
+            if (num_instances > 1) {
                 HIPCHK(hipMemcpyAsync(h->nodes.p + h->tlas_base, T.nodes.data(), T.nodes.size() * sizeof(GNode),
                                       hipMemcpyHostToDevice, h->stream));
+                // the four-wide TLAS of the new nodes, in place (its root stays at t4_base)
+                std::vector<ptd::GNode4> t4;
+                ptl::collapse4_tlas(T.nodes.data(), h->tlas_base, h->tlas_base, h->t4_base, h->t4_room, t4);
+                HIPCHK(hipMemcpyAsync(h->nodes4.p + h->t4_base, t4.data(), t4.size() * sizeof(ptd::GNode4),
+                                      hipMemcpyHostToDevice, h->stream));
+                HIPCHK(hipStreamSynchronize(h->stream));
+            }
             h->t_local.upload(local.data(), local.size(), h->stream);
             h->t_blas.upload(T.blas.data(), T.blas.size(), h->stream);
             h->t_blas4.upload(blas4.data(), blas4.size(), h->stream);
@@ -1097,7 +1116,7 @@ int pt_update_instances(pt_handle h, const float* transforms, int32_t num_instan
             h->tlas_base = R.tlas_base;
             h->tlas_on_device = false;
             h->sc.nodes = h->nodes.p;
-            h->sc.root = R.root;
+            h->set_tlas_roots(R);
             h->select_bvh();
         } else {
             // TLAS unchanged: only the instance records.
@@ -1171,6 +1190,12 @@ int pt_rebuild_instances(pt_handle h, const float* transforms, int32_t num_insta
         HIPCHK(hipMemsetAsync(a.negz, 0, sizeof(int), s));
         ptk::launch_tlas_rebuild(s, a);
         HIPCHK(hipGetLastError());
+        if (n >= 2) {
+            // the four-wide TLAS of the rebuilt nodes (the order arrays are free again)
+            ptk::launch_tlas4(s, h->nodes.p, h->tlas_base, h->nodes4.p + h->t4_base, h->t4_base, n - 1, h->t_ord.p,
+                              h->t_ord.p + nn);
+            HIPCHK(hipGetLastError());
+        }
         HIPCHK(hipStreamSynchronize(s));
         h->tlas_on_device = true;
         return PT_OK;
@@ -1194,6 +1219,22 @@ int pt_read_tlas(pt_handle h, float* nodes, int32_t num_nodes, float* instances,
                                   h->stream));
         HIPCHK(hipStreamSynchronize(h->stream));
         if (root) *root = h->sc.root;
+        return PT_OK;
+    });
+}
+
+int pt_read_tlas4(pt_handle h, float* nodes, int32_t num_nodes, int32_t* room, int32_t* root) {
+    return Guard(h, [&] {
+        if (!h->has_scene) return Fail(h, PT_ERR_STATE, "no scene uploaded (pt_upload_scene)");
+        if (room) *room = h->t4_room;
+        if (nodes) {
+            if (num_nodes != h->t4_room) return Fail(h, PT_ERR_INVALID_ARG, "nodes must hold the four-wide TLAS room");
+            HIPCHK(hipSetDevice(h->device));
+            HIPCHK(hipMemcpyAsync(nodes, h->nodes4.p + h->t4_base, static_cast<size_t>(h->t4_room) * sizeof(ptd::GNode4),
+                                  hipMemcpyDeviceToHost, h->stream));
+            HIPCHK(hipStreamSynchronize(h->stream));
+        }
+        if (root) *root = h->root4_tlas;
         return PT_OK;
     });
 }

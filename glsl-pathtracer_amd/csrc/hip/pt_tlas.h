@@ -47,6 +47,68 @@ __host__ __device__ __forceinline__ bool inst_record(const float* m, int blas_re
     return true;
 }
 
+// One four-wide node from BVH2 node r (n2[r - off] holds it): starting from
+// its two children, while fewer than four are held, the internal one of
+// largest surface area (the first on ties; area in double, no contraction) is
+// replaced in place by its two children, so the children keep the BVH2's
+// left-to-right order. Returns the number of children (2..4); internal
+// children keep their BVH2 references, leaves their leaf references. Shared
+// by the host collapse of every BLAS and TLAS and the device TLAS collapse.
+struct Child4 { float lo[3], hi[3]; int ref; };
+__host__ __device__ __forceinline__ int expand4(const GNode* n2, int off, int r, Child4 ch[4]) {
+    auto kids = [&](int q, Child4& l, Child4& rr) {
+        const GNode g = n2[q - off];
+        l.lo[0] = g.a.x; l.lo[1] = g.a.y; l.lo[2] = g.a.z; l.hi[0] = g.a.w; l.hi[1] = g.b.x; l.hi[2] = g.b.y;
+        l.ref = g.d.x;
+        rr.lo[0] = g.b.z; rr.lo[1] = g.b.w; rr.lo[2] = g.c.x; rr.hi[0] = g.c.y; rr.hi[1] = g.c.z; rr.hi[2] = g.c.w;
+        rr.ref = g.d.y;
+    };
+    auto area = [](const Child4& c) {
+        const double dx = double(c.hi[0]) - c.lo[0], dy = double(c.hi[1]) - c.lo[1], dz = double(c.hi[2]) - c.lo[2];
+        return dx * dy + dy * dz + dz * dx;
+    };
+    kids(r, ch[0], ch[1]);
+    int k = 2;
+    while (k < 4) {
+        int best = -1;
+        double ba = -1.0;
+        for (int j = 0; j < k; ++j)
+            if (ch[j].ref >= 0 && area(ch[j]) > ba) { ba = area(ch[j]); best = j; }
+        if (best < 0) break;
+        Child4 l, rr;
+        kids(ch[best].ref, l, rr);
+        for (int j = k; j > best + 1; --j) ch[j] = ch[j - 1];
+        ch[best] = l;
+        ch[best + 1] = rr;
+        ++k;
+    }
+    return k;
+}
+
+// GNode4 of up to four children; refs[] are the references written (internal
+// children already renumbered), EMPTY4 past count.
+__host__ __device__ __forceinline__ GNode4 make_node4(const Child4 ch[4], int count, const int refs[4]) {
+    float lo[3][4], hi[3][4];
+    int rf[4];
+    for (int k = 0; k < 4; ++k) {
+        for (int a = 0; a < 3; ++a) {
+            lo[a][k] = k < count ? ch[k].lo[a] : 0.f;
+            hi[a][k] = k < count ? ch[k].hi[a] : 0.f;
+        }
+        rf[k] = k < count ? refs[k] : EMPTY4;
+    }
+    GNode4 g;
+    g.lox = make_float4(lo[0][0], lo[0][1], lo[0][2], lo[0][3]);
+    g.hix = make_float4(hi[0][0], hi[0][1], hi[0][2], hi[0][3]);
+    g.loy = make_float4(lo[1][0], lo[1][1], lo[1][2], lo[1][3]);
+    g.hiy = make_float4(hi[1][0], hi[1][1], hi[1][2], hi[1][3]);
+    g.loz = make_float4(lo[2][0], lo[2][1], lo[2][2], lo[2][3]);
+    g.hiz = make_float4(hi[2][0], hi[2][1], hi[2][2], hi[2][3]);
+    g.ref = make_int4(rf[0], rf[1], rf[2], rf[3]);
+    g.pad = make_int4(0, 0, 0, 0);
+    return g;
+}
+
 // Segment table of the build: one segment per node of the level under
 // construction (elements [start, start + count) of the order array), as word
 // arrays of h = n / 2 + 1 entries each, in this order:
@@ -94,6 +156,11 @@ struct TlasBuild {
 namespace ptk {
 // Instance records, world boxes and the TLAS nodes of a rebuild (a.n >= 1).
 void launch_tlas_rebuild(hipStream_t s, const ptd::TlasBuild& a);
+// Four-wide TLAS (room entries at out, references counted from base) of the
+// BVH2 TLAS rooted at root2 >= 0 with at most room internal nodes; qa and qb
+// are room-entry scratch arrays.
+void launch_tlas4(hipStream_t s, const ptd::GNode* nodes, int root2, ptd::GNode4* out, int base, int room, int* qa,
+                  int* qb);
 // Record sizes as the rebuild's unit compiled them (compared with the host
 // code's at pt_create).
 struct TlasLayout { size_t build, inst, node; };

@@ -38,8 +38,12 @@ namespace ptk {
 namespace {
 
 using ptd::GInst;
+using ptd::Child4;
 using ptd::GNode;
+using ptd::GNode4;
 using ptd::TlasBuild;
+using ptd::expand4;
+using ptd::make_node4;
 using ptd::kTlasSegWords;
 
 constexpr int kBuildThreads = 1024;
@@ -497,6 +501,71 @@ __global__ __launch_bounds__(kBuildThreads) void k_tlas_build(TlasBuild a) {
     }
 }
 
+// Four-wide TLAS of a rebuilt BVH2 TLAS (one workgroup), the array the host's
+// collapse4_tlas writes: breadth first, level by level. Each thread converts
+// one node of the level (expand4); the places of its internal children in the
+// next level come from a workgroup prefix sum over the threads' counts, so
+// the numbering is the host's first-in-first-out order. out holds `room`
+// entries; the tail after the last node is zeroed. qa and qb hold the BVH2
+// references of the current and the next level (room entries each).
+constexpr int kTlas4Threads = 256;
+__global__ __launch_bounds__(kTlas4Threads) void k_tlas4(const GNode* __restrict__ nodes, int root2,
+                                                         GNode4* __restrict__ out, int base, int room, int* qa,
+                                                         int* qb) {
+    constexpr int kWaves = kTlas4Threads / 64;
+    __shared__ int wsum[kWaves];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    if (tid == 0) qa[0] = root2;
+    __syncthreads();
+    int cur = 1, start = 0;  // nodes of this level, index of its first node
+    // a tree of at most room internal nodes never fails the bound (it guards the loop)
+    while (cur > 0 && start + cur <= room) {
+        int nxt = 0;
+        for (int c0 = 0; c0 < cur; c0 += kTlas4Threads) {
+            const int i = c0 + tid;
+            Child4 ch[4];
+            int k = 0, inner = 0;
+            if (i < cur) {
+                k = expand4(nodes, 0, qa[i], ch);
+                for (int j = 0; j < k; ++j) inner += ch[j].ref >= 0 ? 1 : 0;
+            }
+            int incl = inner;
+            for (int d = 1; d < 64; d <<= 1) {
+                const int t = __shfl_up(incl, d, 64);
+                if (lane >= d) incl += t;
+            }
+            if (lane == 63) wsum[wv] = incl;
+            __syncthreads();
+            int woff = 0, total = 0;
+            for (int w = 0; w < kWaves; ++w) {
+                if (w < wv) woff += wsum[w];
+                total += wsum[w];
+            }
+            if (i < cur) {
+                int p = nxt + woff + incl - inner;
+                int ref[4];
+                for (int j = 0; j < k; ++j) {
+                    ref[j] = ch[j].ref;
+                    if (ref[j] >= 0) {
+                        qb[p] = ch[j].ref;
+                        ref[j] = base + start + cur + p;
+                        ++p;
+                    }
+                }
+                out[start + i] = make_node4(ch, k, ref);
+            }
+            nxt += total;
+            __syncthreads();
+        }
+        start += cur;
+        cur = nxt;
+        int* t = qa;
+        qa = qb;
+        qb = t;
+    }
+    for (int i = start + tid; i < room; i += kTlas4Threads) out[i] = GNode4{};
+}
+
 void launch_tlas_rebuild(hipStream_t s, const TlasBuild& a) {
     hipLaunchKernelGGL(k_tlas_prep, dim3((a.n + 255) / 256), dim3(256), 0, s, a);
     if (a.n >= 2) {
@@ -521,6 +590,10 @@ void launch_tlas_rebuild(hipStream_t s, const TlasBuild& a) {
         }
         hipLaunchKernelGGL(k_tlas_build, dim3(1), dim3(kBuildThreads), words * sizeof(int), s, b);
     }
+}
+
+void launch_tlas4(hipStream_t s, const GNode* nodes, int root2, GNode4* out, int base, int room, int* qa, int* qb) {
+    hipLaunchKernelGGL(k_tlas4, dim3(1), dim3(kTlas4Threads), 0, s, nodes, root2, out, base, room, qa, qb);
 }
 
 TlasLayout tlas_layout() { return tlas_layout_here(); }

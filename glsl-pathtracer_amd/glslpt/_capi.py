@@ -159,6 +159,7 @@ _HIP_SIGS = {
     "pt_read_env_cdf": [C.c_void_p, P(f32), C.c_size_t, P(f32)],
     "pt_rebuild_instances": [C.c_void_p, P(f32), i32, P(i32)],
     "pt_read_tlas": [C.c_void_p, P(f32), i32, P(f32), i32, P(i32)],
+    "pt_read_tlas4": [C.c_void_p, P(f32), i32, P(i32), P(i32)],
     "pt_env_cdf_noalloc": [P(f32), i32, i32, P(f32), P(f32), C.c_void_p],
 }
 

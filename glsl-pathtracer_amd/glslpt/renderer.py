@@ -111,6 +111,16 @@ class Renderer:
             self._h, api.fptr(nodes), nodes.shape[0], api.fptr(insts), num_instances, C.byref(root)))
         return nodes, insts, root.value
 
+    def read_tlas4(self):
+        """(nodes (room, 32) float32, root) of the four-wide TLAS; columns 24-27
+        hold the child references as int bits, breadth first from the root."""
+        room, root = C.c_int32(0), C.c_int32(0)
+        self._check("pt_read_tlas4", self._lib.pt_read_tlas4(self._h, None, 0, C.byref(room), None))
+        nodes = np.zeros((room.value, 32), np.float32)
+        self._check("pt_read_tlas4", self._lib.pt_read_tlas4(
+            self._h, api.fptr(nodes), room.value, None, C.byref(root)))
+        return nodes, root.value
+
     def set_options(self, opts=None, **kw):
         o = opts if opts is not None else default_options(**kw)
         self._check("pt_set_options", self._lib.pt_set_options(self._h, C.byref(o)))

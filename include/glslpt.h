@@ -235,6 +235,16 @@ int pt_rebuild_instances(pt_handle h, const float* transforms, int32_t num_insta
  * the root reference. Any output may be NULL. */
 int pt_read_tlas(pt_handle h, float* nodes, int32_t num_nodes, float* instances, int32_t num_instances,
                  int32_t* root);
+/* Copies the four-wide TLAS that pt_options.bvh_width = 4 traverses: `room`
+ * entries of 32 floats (children's boxes as min x, max x, min y, max y, min z,
+ * max z rows of four, then four references as int bits: internal ones index
+ * the renderer's four-wide node array, in which the first entry copied is the
+ * root, leaves are -1 - instance, empty INT32_MIN; 4 unused), breadth first
+ * from the root, unused entries zero; room is at least num_instances - 1.
+ * root gets the root's index (or -1 - instance when the TLAS is one
+ * instance). nodes may be NULL to query room. No counterpart in the reference
+ * (its TLAS is binary): the inspection hook of this form, beside pt_read_tlas. */
+int pt_read_tlas4(pt_handle h, float* nodes, int32_t num_nodes, int32_t* room, int32_t* root);
 int pt_set_options(pt_handle h, const pt_options* o);          /* InitShaders defines + uniforms */
 /* Replaces the environment map of the uploaded scene: uploads the RGB32F image
  * (width*height*3 floats, rows top-down as EnvironmentMap::LoadMap reads them)

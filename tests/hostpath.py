@@ -111,3 +111,17 @@ def any_orders(sc, rs, origins, dirs, maxd, **opts):
     if rc != 0:
         raise RuntimeError("host_any_orders failed")
     return differ.value, hits.value
+
+
+def tlas4(sc):
+    """(nodes (room, 32) float32, root) of the host re-layout's four-wide TLAS:
+    breadth first, the root first, so entry i has array index root + i (root
+    is -1 - instance when the TLAS is one instance)."""
+    v, _ = _view(sc)
+    room, root = C.c_int(0), C.c_int(0)
+    if lib().host_tlas4(C.byref(v), None, 0, C.byref(room), C.byref(root)) != 0:
+        raise RuntimeError("host_tlas4 failed")
+    nodes = np.zeros((room.value, 32), np.float32)
+    if lib().host_tlas4(C.byref(v), nodes.ctypes.data_as(P(C.c_float)), room.value, C.byref(room), C.byref(root)) != 0:
+        raise RuntimeError("host_tlas4 failed")
+    return nodes, root.value

@@ -49,7 +49,11 @@ struct HostScene {
         sc.tex = (v->num_textures > 0 && v->texture_maps) ? v->texture_maps : nullptr;
         sc.env_img = has_env ? v->env_img : nullptr;
         sc.env_cdf = has_env ? v->env_cdf : nullptr;
-        sc.root = R.root;
+        sc.root = sc.nodes4 ? R.root4 : R.root;
+        bool any_blend = false;
+        for (int i = 0; i < v->num_materials; ++i)
+            if ((int)v->materials[i].alphaMode == ALPHA_MODE_BLEND) any_blend = true;
+        sc.root_bin = (sc.nodes4 && any_blend) ? R.root : EMPTY4;
         sc.num_lights = v->num_lights;
         sc.tex_w = v->tex_width; sc.tex_h = v->tex_height;
         sc.num_tex = sc.tex ? v->num_textures : 0;
@@ -279,6 +283,23 @@ extern "C" int host_any_orders(const pt_scene_view* v, const pt_options* o, cons
         return 0;
     } catch (const std::exception& e) {
         fprintf(stderr, "host_any_orders: %s\n", e.what());
+        return -1;
+    }
+}
+
+// The four-wide TLAS of the host re-layout (what pt_upload_scene uploads):
+// room entries of 32 floats written to out (when cap >= room), the room and
+// the root reference.
+extern "C" int host_tlas4(const pt_scene_view* v, float* out, int cap, int* room, int* root) {
+    try {
+        const ptl::Relayout R = ptl::relayout(v->nodes, v->num_nodes, v->top_level_index, v->num_tris, v->num_instances);
+        const int m = static_cast<int>(R.wide.nodes.size()) - R.tlas4_base;
+        *room = m;
+        *root = R.root4;
+        if (out && cap >= m) memcpy(out, R.wide.nodes.data() + R.tlas4_base, sizeof(GNode4) * static_cast<size_t>(m));
+        return 0;
+    } catch (const std::exception& e) {
+        fprintf(stderr, "host_tlas4: %s\n", e.what());
         return -1;
     }
 }

@@ -1,8 +1,10 @@
 """Device TLAS rebuild (pt_rebuild_instances) against the host path: the scene
 builder's RebuildInstances, which test_tlas_rebuild pins bit for bit to the
 reference's Scene::RebuildInstances, uploaded with pt_update_instances. The
-device nodes, instance records and root are identical, and so are the images
-rendered after either path."""
+device nodes, instance records and root are identical, so is the four-wide
+TLAS that the device collapses from them to the host's collapse (pinned by
+test_tlas4), and so are the images rendered after either path, in both
+forms."""
 import numpy as np
 import pytest
 
@@ -30,6 +32,10 @@ def same(x, y):
     return np.array_equal(bits(x[0]), bits(y[0])) and np.array_equal(bits(x[1]), bits(y[1])) and x[2] == y[2]
 
 
+def same4(x, y):
+    return x[0].shape == y[0].shape and np.array_equal(bits(x[0]), bits(y[0])) and x[1] == y[1]
+
+
 def host_path(r, sc, xf):
     for i in range(xf.shape[0]):
         sc.SetInstanceTransform(i, xf[i])
@@ -45,15 +51,21 @@ def check_case(r, case, n, seed):
     sc = tlas_cases.build_scene(ms, insts)
     r.upload_scene(sc)
     uploaded = r.read_tlas(n)
-    # unchanged transforms: the rebuild reproduces the uploaded TLAS
+    uploaded4 = r.read_tlas4()
+    # unchanged transforms: the rebuild reproduces the uploaded TLAS, and the
+    # device collapse the host's four-wide one
     r.rebuild_instances(xf0)
     assert same(r.read_tlas(n), uploaded), f"{case} n={n}: rebuild of the uploaded transforms"
+    assert same4(r.read_tlas4(), uploaded4), f"{case} n={n}: four-wide TLAS of the uploaded transforms"
     r.rebuild_instances(xf1)
     dev = r.read_tlas(n)
+    dev4 = r.read_tlas4()
     host = host_path(r, sc, xf1)
+    host4 = r.read_tlas4()
     assert np.array_equal(bits(dev[0]), bits(host[0])), f"{case} n={n}: nodes"
     assert np.array_equal(bits(dev[1]), bits(host[1])), f"{case} n={n}: instance records"
     assert dev[2] == host[2]
+    assert same4(dev4, host4), f"{case} n={n}: four-wide TLAS"
 
 
 @pytest.mark.parametrize("case", tlas_cases.CASES)
@@ -92,7 +104,8 @@ def test_repeated_rebuilds(renderer):
         assert same(dev, host), f"edit {k}"
 
 
-def test_render_after_rebuild_equals_host(renderer):
+@pytest.mark.parametrize("width", [2, 4])
+def test_render_after_rebuild_equals_host(renderer, width):
     n = 40
     ms = tlas_cases.meshes("random", 9)
     insts, xf1 = tlas_cases.instances("random", n, 9)
@@ -100,7 +113,7 @@ def test_render_after_rebuild_equals_host(renderer):
     sc = tlas_cases.build_scene(ms, insts)
     renderer.upload_scene(sc)
     renderer.set_options(default_options(width=48, height=32, max_depth=3, enable_uniform_light=1,
-                                          uniform_light_col=(1.0, 1.0, 1.0)))
+                                          uniform_light_col=(1.0, 1.0, 1.0), bvh_width=width))
     renderer.set_camera(sc.camera())
     images = []
     for path in ("device", "host"):
@@ -166,10 +179,12 @@ def test_update_without_nodes_keeps_tlas(renderer):
     renderer.upload_scene(sc)
     renderer.rebuild_instances(xf1)
     dev = renderer.read_tlas(n)
+    dev4 = renderer.read_tlas4()
     xf = np.ascontiguousarray(xf1, np.float32)
     rc = renderer._lib.pt_update_instances(renderer._h, api.fptr(xf), n, None, 0, None, 0, 0)
     assert rc == api.PT_OK
     assert same(renderer.read_tlas(n), dev)
+    assert same4(renderer.read_tlas4(), dev4)
     # a partial node range after a device rebuild is refused
     v = sc.view()
     rc = renderer._lib.pt_update_instances(renderer._h, api.fptr(xf), n, None, 0,
@@ -190,6 +205,8 @@ def test_tlas_range_update_equals_full(renderer):
     sc.RebuildInstances()
     renderer.update_instances(sc)
     part = renderer.read_tlas(n)
+    part4 = renderer.read_tlas4()
     renderer.update_instances(sc, first_node=0)
     full = renderer.read_tlas(n)
     assert same(part, full)
+    assert same4(part4, renderer.read_tlas4())
