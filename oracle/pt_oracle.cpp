@@ -1,4 +1,4 @@
-// CPU oracle — TEST INFRASTRUCTURE (checker and CPU baseline), never product code.
+// CPU oracle — TEST INFRASTRUCTURE (the parity reference and CPU baseline), never product code.
 //
 // Restatement of the reference GLSL path, function by function, in the same
 // arithmetic order (left-to-right as the GLSL source groups it), templated on

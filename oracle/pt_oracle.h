@@ -3,9 +3,9 @@
  *
  * A C++ restatement of the reference's GLSL path (src/shaders/tile.glsl and
  * src/shaders/common/{globals,intersection,sampling,envmap,closest_hit,anyhit,
- * disney,pathtrace}.glsl) over the reference's flat scene arrays. It is used by
- * tests/ as the parity checker, by __graft_entry__.smoke() as the checker and
- * by bench.py as the timed CPU baseline ("port"). Templated on the scalar
+ * disney,pathtrace}.glsl) over the reference's flat scene arrays. tests/ and
+ * __graft_entry__.smoke() compare the HIP renderer's results with it, and
+ * bench.py times it as the CPU baseline ("port"). Templated on the scalar
  * (float for parity, double for the high-spp FP64 reference image).
  *
  * Pinning: see the header of pt_oracle.cpp for which check pins which function.

@@ -1,7 +1,7 @@
 // Test-infrastructure harness around the reference's own host C++ (Scene,
 // Mesh, Camera, RadeonRays Bvh/SplitBvh/BvhTranslator), compiled from the
 // reference sources by oracle/Makefile into oracle/_ref/libref_scene.so.
-// It is a checker: the tests call it to obtain the reference's flattened
+// It serves the tests only: they call it to obtain the reference's flattened
 // arrays for given triangle soups and instances, and compare the product's
 // scene builder against them bit for bit. Nothing in the product links it.
 //

@@ -1,6 +1,7 @@
 """ctypes face of oracle/_ref/liboracle.so — the CPU transcription of the
 reference shaders. Test infrastructure: only tests/, __graft_entry__.smoke()
-and bench.py's cpu_baseline leg use it, always as the checker."""
+and bench.py's cpu_baseline leg use it, always as the reference the HIP
+renderer's results are compared with (or timed beside)."""
 import ctypes as C
 import os
 import subprocess
