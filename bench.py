@@ -509,6 +509,9 @@ def counter_rooflines(prof, avg_ms):
             "issue_frac": issue, "lane_activity": lanes / 64,
             "lane_activity_camera": prof["kinds"]["camera"]["per_step"]["lane_activity"],
             "lane_activity_later": prof["kinds"]["later"]["per_step"]["lane_activity"],
+            "waves_per_simd_camera": prof["kinds"]["camera"]["per_step"].get("waves_per_simd"),
+            "waves_per_simd_later": prof["kinds"]["later"]["per_step"].get("waves_per_simd"),
+            "waves_per_simd_launched": prof["kinds"]["camera"]["per_step"].get("waves_per_simd_launched"),
             "fp32_peak_tflops": FP32_PEAK_TFLOPS, "frac_of_fp32_peak": 2 * v_ach / FP32_PEAK_TFLOPS,
             "peak_basis": "measured: fp32 FMA probe (profiling.py valu), one wave64 VALU instruction per SIMD "
                           "per 4 cycles; 157.3 TFLOP/s counts packed FP32 (2 per lane), which this build "

@@ -328,6 +328,9 @@ def _one(root, sub):
     return _counters(files[0])
 
 
+MI355X_SIMDS = 256 * 4  # 256 CUs of four SIMDs
+
+
 def summarize(root, out_json):
     """Counter summary of the 1080p frame's closest-hit stage and the probes,
     from the rocprofv3 runs under ROOT:
@@ -392,6 +395,10 @@ def summarize(root, out_json):
         per["hbm_bytes"] = per["FETCH_SIZE"] * 1024 * fetch_scale + per["WRITE_SIZE"] * 1024 * write_scale
         per["l2_bytes"] = per["TCC_REQ_sum"] * bytes_per_req
         per["lane_activity"] = sm["SQ_THREAD_CYCLES_VALU"] / (64 * sm["SQ_ACTIVE_INST_VALU"])
+        # occupancy: SQ_WAVE_CYCLES counts resident wave-cycles in units of 4 cycles; GRBM_GUI_ACTIVE
+        # sums the busy cycles of the 8 XCDs
+        per["waves_per_simd"] = 4 * sm["SQ_WAVE_CYCLES"] / (sm["GRBM_GUI_ACTIVE"] / 8 * MI355X_SIMDS)
+        per["waves_per_simd_launched"] = sm["SQ_WAVES"] / k["dispatches"] / MI355X_SIMDS
         summary["kinds"][kind] = {"kernels": sorted(k["kernels"]), "dispatches": k["dispatches"], "steps": n,
                                   "per_step": per}
         for c in ("hbm_bytes", "l2_bytes", "SQ_INSTS_VALU", "SQ_THREAD_CYCLES_VALU", "SQ_ACTIVE_INST_VALU"):
