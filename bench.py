@@ -72,6 +72,13 @@ METRIC = "Msamples/s at 1080p depth 8 (batched vs plain kernel); backward/forwar
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3  # MI355X vector fp32, packed
 B_NODE, B_TRI, B_INST, B_LIGHT, B_RAY = 64, 36, 52, 60, 56
+# Algorithmic flops of the same work, counted from the kernels' expressions: a BVH2-node
+# equivalent is two slab tests (per box 6 subtractions, 6 products, 10 min/max, 2 compares); a
+# triangle test 52 (two cross products, four dot products, one reciprocal, 3 products, 2
+# subtractions, 5 compares); an instance entry 36 (the affine origin and linear direction
+# transforms, 3 reciprocals); a light test 38 (the 1080p scene's mean of 2 rect tests at 44 and
+# one sphere test at 25, every branch taken).
+F_NODE, F_TRI, F_INST, F_LIGHT = 48, 52, 36, 38
 
 
 def build_scene(args):
@@ -595,6 +602,16 @@ def main():
         avg_ms = extend_ms / extend_launches
         achieved = per_launch / (avg_ms / 1e3) / 1e9
         traffic, l2, valu = counter_rooflines(counter_profile(cfg), avg_ms)
+        flops = scale * (F_NODE * cw["nodes_visited"] + F_TRI * cw["tris_tested"] +
+                         F_INST * cw["instances_entered"] + F_LIGHT * cw["lights_tested"]) / extend_launches
+        if valu is not None:
+            tf = flops / (avg_ms / 1e3) / 1e12
+            valu.update({"algorithmic_flops_per_launch": flops, "algorithmic_tflops": tf,
+                         "algorithmic_frac_of_fp32_peak": tf / FP32_PEAK_TFLOPS,
+                         "algorithmic_frac_of_probe": tf * 1e12 / (2 * valu["peak"] * 1e12),
+                         "algorithmic_basis": "flops per node, triangle, instance and light test counted from "
+                                              "the kernels' expressions (bench.py F_*) times the instrumented "
+                                              "frame's counts, over the measured launch time"})
         roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "l2": l2, "valu": valu,
                     "achieved_basis": "algorithmic bytes (64 B per node, 36 B per triangle, 52 B per instance, "
