@@ -11,6 +11,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_DIR = os.path.join(_HERE, "lib")
 
 PT_OK = 0
+PT_ERR_INVALID_ARG = -1
 PT_ERR_OVERFLOW = -4
 PT_MODE_WAVEFRONT = 0
 PT_MODE_PLAIN = 1

@@ -164,6 +164,15 @@ def test_errors_leave_state_unchanged(renderer):
     with pytest.raises(PtError):
         renderer.rebuild_instances(xf1, np.full(n, 3, np.int32))
     assert same(renderer.read_tlas(n), before)
+    # pt_read_tlas4 copies exactly the four-wide room, nothing else
+    import ctypes as C
+
+    from glslpt import _capi as api
+    room = renderer.read_tlas4()[0].shape[0]
+    assert room >= n - 1
+    buf = np.zeros((room + 1, 32), np.float32)
+    rc = renderer._lib.pt_read_tlas4(renderer._h, api.fptr(buf), room + 1, None, None)
+    assert rc == api.PT_ERR_INVALID_ARG and not buf.any()
 
 
 def test_update_without_nodes_keeps_tlas(renderer):
