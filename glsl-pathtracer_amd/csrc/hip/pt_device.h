@@ -734,7 +734,10 @@ struct TravCount { uint32_t nodes, tris, insts, lights, spills, rays; };
 // Traversal stack: entries [0, S) in LDS (entry i of lane tid at lds[i*bs + tid],
 // conflict-free), deeper entries in a global backing store indexed by the
 // thread's slot. MAXD bounds the total depth; deeper trees set `overflow`.
-constexpr int MAXD = 64;
+// The reference's binary traversal keeps 64 entries (closest_hit.glsl:89);
+// four-wide nodes push up to three entries per level over half as many
+// levels, so this stack has half as much room again.
+constexpr int MAXD = 96;
 
 struct Stack {
     int* lds;          // this block's LDS array
