@@ -328,12 +328,10 @@ def heavy_throughput(steps, warmup, count=True):
                                               scenes.HEAVY_LEAF_QUADS))
     cfg = dict(block_size=256, stack_depth=24, wave_spp=4, compact_every=1, bvh_width=2)
     cfg.update(tuned("c2") or {})
-    # four-wide BLAS nodes: same hits but for exact ties, and 6% faster on
-    # this scene's deep BLASes (closest-hit 670 -> 633 ms per 4-spp frame);
-    # no closest-hit step cap: resuming this scene's long traversals costs more
-    # than it saves (closest-hit 565 -> 603 ms per 4-spp frame at a cap of 32)
-    cfg["bvh_width"] = 4
-    cfg["extend_cap"] = 0
+    # this scene's stored configuration: the 1080p one (four-wide nodes) with a
+    # longer closest-hit step cap, 128 (closest-hit 477 -> 432 ms per 4-spp
+    # frame; 32 is slower than none)
+    cfg.update(tuned("c5") or {"bvh_width": 4, "extend_cap": 0})
     r = Renderer(0)
     r.upload_scene(sc)
     samples = rs.width * rs.height * rs.spp

@@ -79,7 +79,7 @@ def heavy(spp):
     sc, rs = heavy_mixed(spp=spp)
     cfg = dict(block_size=256, stack_depth=24, wave_spp=4, compact_every=1)
     cfg.update(tuned("c2") or {})
-    cfg["extend_cap"] = 0  # as bench.py's heavy leg: the step cap is slower on this scene
+    cfg.update(tuned("c5") or {"bvh_width": 4, "extend_cap": 0})  # as bench.py's heavy leg
     r = Renderer(0)
     r.upload_scene(sc)
     out = {}
