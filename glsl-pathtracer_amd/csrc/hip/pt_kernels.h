@@ -108,10 +108,12 @@ void launch_extend(hipStream_t s, int blocks, int bs, const ptd::DevScene& sc, c
                    unsigned long long* counters, int* ovf, bool count, int budget = 0,
                    const ptd::ContBuf* cb = nullptr, int* cont_count = nullptr);
 // A later bounce with budget > 0 (and not counting): traversals stop after
-// `budget` iterations into cb; launch_extend_resume then finishes them.
+// `budget` iterations into cb, each keeping a place in the hit queue;
+// launch_extend_resume then finishes them and fills those places (the slot,
+// or -1 for a miss).
 void launch_extend_resume(hipStream_t s, int blocks, int bs, const ptd::DevScene& sc, const ptd::DevOpts& op,
                           const ptd::PathBuf& pb, const ptd::ContBuf& cb, const int* cont_count, int* hitq,
-                          int* hit_count, int* work, int S, int* spill, int* ovf);
+                          int* work, int S, int* spill, int* ovf);
 // Shade stage; bb: the batched form's pass state (null: one frame); inline_nee:
 // NEE visibility resolved in the stage (stochastic alpha, volume MIS), else
 // queued as shadow records. S and spill: the traversal stack (inline_nee).

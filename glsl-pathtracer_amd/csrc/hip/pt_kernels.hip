@@ -217,13 +217,14 @@ __device__ __forceinline__ bool finish_extend(const DevScene& sc, const DevOpts&
     return true;
 }
 
-// Record r of the continuation buffer: a stopped traversal and its stack.
+// Record r of the continuation buffer: a stopped traversal, its stack and
+// the hit-queue position kept for it (pos).
 __device__ __forceinline__ void save_cont(const ContBuf& cb, int r, int slot, const TravState& ts, const Hit& h,
-                                          Stack& stk) {
+                                          Stack& stk, int pos) {
     cb.slot[r] = slot;
     cb.st[r] = make_int4(ts.ref, ts.sp, ts.blas_base, ts.cur_inst);
     cb.hf[r] = make_float4(h.t, h.u, h.v, 0.f);
-    cb.hi[r] = make_int4(h.tri, h.inst, h.light, 0);
+    cb.hi[r] = make_int4(h.tri, h.inst, h.light, pos);
     for (int k = 0; k < ts.sp; ++k) cb.stack[static_cast<size_t>(k) * cb.cap + r] = stk.pop(k);
 }
 
@@ -241,7 +242,10 @@ __device__ __forceinline__ void save_cont(const ContBuf& cb, int r, int slot, co
 // here; a wave does not wait on its longest rays. Stopped traversals go to the
 // continuation buffer (while it has room and their stack is at most
 // CONT_STACK deep; otherwise they finish here) and k_extend_resume finishes
-// them, densely packed. The hits are the same either way.
+// them, densely packed. The hits are the same either way. A stopped traversal
+// keeps its place in the hit queue beside the batch's finished hits, so the
+// shade stage reads the paths of a batch together (k_extend_resume writes the
+// slot there, or -1 for a miss, which the shade stage skips).
 template <bool COUNT, bool PRIMARY, bool BATCH, bool CAPPED = false>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7, 7))) k_extend(DevScene sc, DevOpts op, PathBuf pb, const int* __restrict__ qin,
                                                 const int* count_in, int npaths, int npix, int first_sample,
@@ -299,16 +303,19 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7, 7))
             bool susp = valid && !done && ts.sp <= CONT_STACK;
             const int r = wave_append(susp, cont_count);
             if (susp && r >= cb.cap) susp = false;  // no room: finish here
-            if (susp) {
-                save_cont(cb, r, slot, ts, h, stk);
-            } else if (valid && !done) {
+            if (!susp && valid && !done) {
                 traverse<false, false, true>(sc, p.ro, p.rd, h.t, stk, h, cnt, ovf, false, nullptr, &ts, -1);
                 done = true;
             }
+            if (valid && done) keep = finish_extend<PRIMARY>(sc, op, pb, slot, p, h);
+            const int pos = wave_append(keep || susp, hit_count);
+            if (keep) hitq[pos] = slot;
+            if (susp) save_cont(cb, r, slot, ts, h, stk, pos);
+        } else {
+            if (valid && done) keep = finish_extend<PRIMARY>(sc, op, pb, slot, p, h);
+            const int pos = wave_append(keep, hit_count);
+            if (pos >= 0) hitq[pos] = slot;
         }
-        if (valid && done) keep = finish_extend<PRIMARY>(sc, op, pb, slot, p, h);
-        const int pos = wave_append(keep, hit_count);
-        if (pos >= 0) hitq[pos] = slot;
     }
     flush_counts<COUNT>(cnt, counters);
     if (ovf) atomicOr(ovf_flag, 1);
@@ -316,9 +323,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7, 7))
 
 // The traversals a capped extend stage stopped, packed densely: each record's
 // stack back on this thread's stack, the traversal run to its end, then the
-// same finish as the extend stage (records [0, min(*cont_count, cap))).
+// same finish as the extend stage (records [0, min(*cont_count, cap))); the
+// slot, or -1 for a miss, goes to the hit-queue position the record kept.
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7, 7)))
-k_extend_resume(DevScene sc, DevOpts op, PathBuf pb, ContBuf cb, const int* cont_count, int* hitq, int* hit_count,
+k_extend_resume(DevScene sc, DevOpts op, PathBuf pb, ContBuf cb, const int* cont_count, int* hitq,
                 int* work, int S, int* spill, int* ovf_flag) {
     extern __shared__ int lds[];
     Stack stk{lds, spill, static_cast<int>(threadIdx.x), static_cast<int>(blockDim.x), S,
@@ -346,9 +354,8 @@ k_extend_resume(DevScene sc, DevOpts op, PathBuf pb, ContBuf cb, const int* cont
             p.depth = pb.meta[slot].x;
             traverse<false, false, true>(sc, p.ro, p.rd, h.t, stk, h, cnt, ovf, false, nullptr, &ts, -1);
             keep = finish_extend<false>(sc, op, pb, slot, p, h);
+            hitq[hi.w] = keep ? slot : -1;
         }
-        const int pos = wave_append(keep, hit_count);
-        if (pos >= 0) hitq[pos] = slot;
     }
     if (ovf) atomicOr(ovf_flag, 1);
 }
@@ -382,8 +389,8 @@ __global__ void __launch_bounds__(256) k_shade(DevScene sc, DevOpts op0, PathBuf
         NeeRay nee[2];
         nee[0].valid = nee[1].valid = false;
         vec3 thr_nee = V1(0.f);
-        if (i < count) {
-            slot = hitq[i];
+        if (i < count) slot = hitq[i];  // -1: a place a resumed traversal left (a miss)
+        if (slot >= 0) {
             PathRegs p;
             DevOpts op = op0;
             if (BATCH) {
@@ -1317,10 +1324,10 @@ void launch_extend(hipStream_t s, int blocks, int bs, const DevScene& sc, const 
 }
 
 void launch_extend_resume(hipStream_t s, int blocks, int bs, const DevScene& sc, const DevOpts& op, const PathBuf& pb,
-                          const ContBuf& cb, const int* cont_count, int* hitq, int* hit_count, int* work, int S,
-                          int* spill, int* ovf) {
+                          const ContBuf& cb, const int* cont_count, int* hitq, int* work, int S, int* spill,
+                          int* ovf) {
     const size_t lds = static_cast<size_t>(S) * bs * sizeof(int);
-    hipLaunchKernelGGL(k_extend_resume, dim3(blocks), dim3(bs), lds, s, sc, op, pb, cb, cont_count, hitq, hit_count, work, S, spill, ovf);
+    hipLaunchKernelGGL(k_extend_resume, dim3(blocks), dim3(bs), lds, s, sc, op, pb, cb, cont_count, hitq, work, S, spill, ovf);
 }
 
 void launch_shade(hipStream_t s, int blocks, int bs, const DevScene& sc, const DevOpts& op, const PathBuf& pb,

@@ -439,7 +439,7 @@ void run_bounces(pt_renderer_s* h, const DevOpts& op, const Wavefront& w, const 
                            wk + 0, w.S, h->spill.p, h->work_counts.p, h->ovf.p, w.cw, capped ? w.cap : 0, &w.cb,
                            c_cont);
         if (capped)
-            ptk::launch_extend_resume(s, w.b_ext, w.bs, h->sc, op, pb, w.cb, c_cont, h->hq.p, c_hit, c_cont + 1, w.S,
+            ptk::launch_extend_resume(s, w.b_ext, w.bs, h->sc, op, pb, w.cb, c_cont, h->hq.p, c_cont + 1, w.S,
                                       h->spill.p, h->ovf.p);
         h->stage_end(PT_STAGE_EXTEND, ev);
         h->stage_begin(PT_STAGE_SHADE, ev);
@@ -752,7 +752,7 @@ void batch_render(pt_renderer_s* h, const pt_job* jobs, int n) {
                     ptk::launch_extend(s, b_ext, bs, h->sc, op, pb, false, qin, c_ext, 0, 0, 0, h->hq.p, c_hit, wk, S,
                                        h->spill.p, h->work_counts.p, h->ovf.p, false, cap, &cb, c_cont);
                     if (cap > 0)
-                        ptk::launch_extend_resume(s, b_ext, bs, h->sc, op, pb, cb, c_cont, h->hq.p, c_hit, c_cont + 1,
+                        ptk::launch_extend_resume(s, b_ext, bs, h->sc, op, pb, cb, c_cont, h->hq.p, c_cont + 1,
                                                   S, h->spill.p, h->ovf.p);
                 }
             });
