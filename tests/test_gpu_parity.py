@@ -187,3 +187,25 @@ def test_capped_extend_identical(renderer, cap, width):
     e = rel_err(out[2][..., :3], r[..., :3], floor=1e-4).max(axis=2)
     print(f"cap {cap} bvh{width}: pixels within 1e-4 = {(e <= 1e-4).mean():.6f}")
     assert (e <= 1e-4).all()
+
+
+@pytest.mark.parametrize("cadence", [2, 3])
+@pytest.mark.parametrize("cap", [3, 32])
+def test_capped_extend_with_compaction_cadence(renderer, cap, cadence):
+    """The step cap with queues compacted every `cadence` bounces: a resumed
+    traversal that misses leaves -1 in its hit-queue place, and bounces that do
+    not compact keep ended paths in the queue; the image equals the uncapped
+    form's with the same cadence, bit for bit."""
+    sc, rs = small_instanced()
+    out = []
+    for kw in ({}, dict(extend_cap=cap)):
+        renderer.upload_scene(sc)
+        renderer.set_options(options_for(rs, mode=PT_MODE_WAVEFRONT, wave_spp=4, bvh_width=4,
+                                         compact_every=cadence, **kw))
+        renderer.set_camera(sc.camera())
+        renderer.clear()
+        renderer.render(0, 4)
+        renderer.synchronize()
+        out.append(renderer.read_accum())
+    assert np.abs(out[0]).max() > 0
+    assert np.array_equal(out[0], out[1])
